@@ -1,0 +1,251 @@
+"""clear_amd — MI355X (gfx950) LBVH build, closest-hit traversal and path tracing.
+
+Thin ctypes binding over libclear_amd.so (C-ABI in include/clear_amd.h).
+Device memory and streams come from PyTorch (torch.cuda tensors); the compute
+is the library's own HIP kernels. Importing this module does not need a GPU;
+calling into it does, and it raises if the shared library was not built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libclear_amd.so"
+
+FACE_DTYPE = np.dtype(
+    [("p", "<u4", 3), ("vn", "<u4", 3), ("vt", "<u4", 3), ("uv", "<u4", 3),
+     ("alphaTexPtr", "<u4"), ("matPtr", "<u2"), ("lightPtr", "<u2")]
+)
+assert FACE_DTYPE.itemsize == 56
+INODE_BYTES = 48
+LEAF_BYTES = 48
+
+CLR_OK = 0
+_lib = None
+
+
+class ClrError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"clear_amd error {status}: {message}")
+        self.status = status
+
+
+def library_path() -> Path:
+    return LIB_PATH
+
+
+def lib():
+    """Loads libclear_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ClrError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, u32, i32, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+    sig = {
+        "clr_create": (i32, [ctypes.POINTER(vp), i32]),
+        "clr_destroy": (i32, [vp]),
+        "clr_error_string": (ctypes.c_char_p, [i32]),
+        "clr_last_error": (ctypes.c_char_p, [vp]),
+        "clr_scratch_bytes": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "clr_set_profiling": (i32, [vp, i32]),
+        "clr_lbvh_last_profile": (i32, [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]),
+        "clr_lbvh_build": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
+        "clr_lbvh_build_soup": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "clr_lbvh_reserve": (i32, [vp, u32, u32]),
+        "clr_lbvh_build_segmented": (i32, [vp, vp, ctypes.POINTER(u32), u32, u32, vp, vp, vp]),
+        "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
+        "clr_gen_cluster_soup": (i32, [u32, i32, u32, f32, f32, f32, vp]),
+        "clr_gen_random_rays": (i32, [u32, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C-ABI functions this binding declares."""
+    lib()
+    return [n for n in dir(_lib) if n.startswith("clr_")]
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    if t is None:
+        return ctypes.c_void_p(0)
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    if isinstance(t, np.ndarray):
+        return ctypes.c_void_p(t.ctypes.data)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None) -> ctypes.c_void_p:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Context:
+    """Owns a clr_ctx (one device, one scratch pool)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        rc = lib().clr_create(ctypes.byref(self._h), device)
+        if rc != CLR_OK:
+            raise ClrError(rc, lib().clr_error_string(rc).decode())
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int) -> None:
+        if rc != CLR_OK:
+            detail = lib().clr_last_error(self._h).decode()
+            raise ClrError(rc, f"{lib().clr_error_string(rc).decode()}: {detail}")
+
+    def close(self) -> None:
+        if self._h:
+            lib().clr_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scratch_bytes(self) -> int:
+        v = ctypes.c_uint64()
+        self.check(lib().clr_scratch_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_profiling(self, on: bool) -> None:
+        self.check(lib().clr_set_profiling(self._h, int(on)))
+
+    def last_profile(self) -> list[float]:
+        buf = (ctypes.c_float * 8)()
+        n = ctypes.c_int()
+        self.check(lib().clr_lbvh_last_profile(self._h, buf, 8, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+
+def node_buffers(n: int, device="cuda"):
+    """Allocates (internalNodes, leafNodes) as uint8 tensors of the 48-B records."""
+    import torch
+
+    inodes = torch.empty(max(n - 1, 1) * INODE_BYTES, dtype=torch.uint8, device=device)
+    leaves = torch.empty(n * LEAF_BYTES, dtype=torch.uint8, device=device)
+    return inodes, leaves
+
+
+class LBVHBuilder:
+    """Mirror of the reference's LBVHBuilder (LBVHBuilder.h:32-75).
+
+    LBVHBuilder(ctx, numFaces) reserves scratch for numFaces triangles;
+    perform(...) enqueues one build on the stream (no host sync).
+    """
+
+    def __init__(self, ctx: Context, num_faces: int, bits_per_dim: int = 10):
+        self.ctx = ctx
+        if num_faces > 0:
+            ctx.check(lib().clr_lbvh_reserve(ctx.handle, num_faces, bits_per_dim))
+
+    def perform(self, vertices, faces, num_faces: int, num_bits_per_dim: int,
+                internal_nodes, leaf_nodes, stream=None) -> None:
+        self.ctx.check(lib().clr_lbvh_build(
+            self.ctx.handle, _ptr(vertices), _ptr(faces), num_faces, num_bits_per_dim,
+            _ptr(internal_nodes), _ptr(leaf_nodes), _stream(stream)))
+
+    def perform_soup(self, tris, num_tris: int, num_bits_per_dim: int,
+                     internal_nodes, leaf_nodes, stream=None) -> None:
+        self.ctx.check(lib().clr_lbvh_build_soup(
+            self.ctx.handle, _ptr(tris), num_tris, num_bits_per_dim,
+            _ptr(internal_nodes), _ptr(leaf_nodes), _stream(stream)))
+
+    def perform_segmented(self, tris, seg_offsets: np.ndarray, num_bits_per_dim: int,
+                          internal_nodes, leaf_nodes, stream=None) -> None:
+        offs = np.ascontiguousarray(seg_offsets, dtype=np.uint32)
+        self.ctx.check(lib().clr_lbvh_build_segmented(
+            self.ctx.handle, _ptr(tris), offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            len(offs) - 1, num_bits_per_dim, _ptr(internal_nodes), _ptr(leaf_nodes),
+            _stream(stream)))
+
+
+def trace_closest(ctx: Context, internal_nodes, leaf_nodes, num_faces: int, vertices, faces,
+                  rays, num_rays: int, hit_face, hit_t, stats=None, stream=None) -> None:
+    ctx.check(lib().clr_trace_closest(
+        ctx.handle, _ptr(internal_nodes), _ptr(leaf_nodes), num_faces, _ptr(vertices),
+        _ptr(faces), _ptr(rays), num_rays, _ptr(hit_face), _ptr(hit_t), _ptr(stats),
+        _stream(stream)))
+
+
+# ---- host scene generators ------------------------------------------------
+
+def gen_cluster_soup(num_tris: int, seed: int = 1, num_clusters: int = 64, sigma: float = 0.02,
+                     e_min: float = 1e-3, e_max: float = 1e-2) -> np.ndarray:
+    """Gaussian-cluster soup, float32 [num_tris, 9] (host)."""
+    out = np.empty((num_tris, 9), dtype=np.float32)
+    rc = lib().clr_gen_cluster_soup(num_tris, seed, num_clusters, sigma, e_min, e_max,
+                                    _ptr(out))
+    if rc:
+        raise ClrError(rc, "clr_gen_cluster_soup")
+    return out
+
+
+def gen_random_rays(num_rays: int, seed: int = 2) -> np.ndarray:
+    """Random rays, float32 SoA [7, num_rays] (host)."""
+    out = np.empty((7, num_rays), dtype=np.float32)
+    rc = lib().clr_gen_random_rays(num_rays, seed, _ptr(out))
+    if rc:
+        raise ClrError(rc, "clr_gen_random_rays")
+    return out
+
+
+def soup_to_mesh(soup: np.ndarray):
+    """Triangle soup [N,9] -> (vertices float32 [3N,4], faces FACE_DTYPE [N]).
+
+    Vertex 3i+k is corner k of triangle i; every other face attribute absent.
+    """
+    n = soup.shape[0]
+    verts = np.zeros((3 * n, 4), dtype=np.float32)
+    verts[:, :3] = soup.reshape(3 * n, 3)
+    faces = np.zeros(n, dtype=FACE_DTYPE)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    faces["p"] = idx
+    faces["vn"] = 0xFFFFFFFF
+    faces["vt"] = 0xFFFFFFFF
+    faces["uv"] = 0xFFFFFFFF
+    faces["alphaTexPtr"] = 0xFFFFFFFF
+    faces["matPtr"] = 0xFFFF
+    faces["lightPtr"] = 0xFFFF
+    return verts, faces
+
+
+def decode_inodes(buf: np.ndarray, n: int):
+    """48-B InternalNode records -> dict of numpy arrays (min, max, isChild, c)."""
+    raw = np.frombuffer(np.ascontiguousarray(buf).tobytes(), dtype=np.uint8)[: (n - 1) * 48]
+    rec = raw.reshape(-1, 48)
+    f = rec[:, :32].copy().view(np.float32).reshape(-1, 8)
+    return {
+        "min": f[:, 0:3], "max": f[:, 4:7],
+        "isChild": rec[:, 32:34].copy(),
+        "c": rec[:, 36:44].copy().view(np.uint32).reshape(-1, 2),
+    }
+
+
+def decode_leaves(buf: np.ndarray, n: int):
+    raw = np.frombuffer(np.ascontiguousarray(buf).tobytes(), dtype=np.uint8)[: n * 48]
+    rec = raw.reshape(-1, 48)
+    f = rec[:, :32].copy().view(np.float32).reshape(-1, 8)
+    return {"min": f[:, 0:3], "max": f[:, 4:7],
+            "objIdx": rec[:, 32:36].copy().view(np.uint32).reshape(-1)}

@@ -1,0 +1,254 @@
+// C-ABI entry points of libclear_amd (declared in include/clear_amd.h).
+//
+// Each entry point validates its arguments, makes the context's device
+// current for the duration of the call, enqueues work on the caller's stream
+// and returns a status; nothing here throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+namespace clr {
+
+int fail(clr_ctx* ctx, int status, const std::string& msg) {
+    if (ctx) ctx->lastError = msg;
+    return status;
+}
+
+int hipCheck(clr_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return CLR_OK;
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(ctx, CLR_ERR_HIP, m);
+}
+
+int ensurePool(clr_ctx* ctx, size_t bytes) {
+    if (ctx->poolBytes >= bytes) return CLR_OK;
+    // The pool may be in use by work still queued on any stream.
+    int rc = hipCheck(ctx, hipDeviceSynchronize(), "pool regrow sync");
+    if (rc) return rc;
+    if (ctx->pool) {
+        (void)hipFree(ctx->pool);
+        ctx->pool = nullptr;
+        ctx->poolBytes = 0;
+    }
+    size_t want = bytes + bytes / 8;  // headroom against small regrowths
+    hipError_t e = hipMalloc(&ctx->pool, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->pool = nullptr;
+        return fail(ctx, CLR_ERR_OUT_OF_MEMORY,
+                    "scratch pool of " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
+    }
+    ctx->poolBytes = want;
+    return CLR_OK;
+}
+
+// Sets the context's device for the scope of one call.
+struct DeviceScope {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceScope(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) err = hipSetDevice(dev);
+    }
+    ~DeviceScope() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace clr
+
+using namespace clr;
+
+extern "C" {
+
+const char* clr_error_string(int status) {
+    switch (status) {
+        case CLR_OK: return "ok";
+        case CLR_ERR_INVALID_ARG: return "invalid argument";
+        case CLR_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case CLR_ERR_HIP: return "HIP runtime error";
+        case CLR_ERR_UNSUPPORTED: return "unsupported parameter";
+        case CLR_ERR_NO_DEVICE: return "no usable device";
+        default: return "unknown status";
+    }
+}
+
+int clr_create(clr_ctx** out, int device) {
+    if (!out) return CLR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        (void)hipGetLastError();
+        return CLR_ERR_NO_DEVICE;
+    }
+    clr_ctx* ctx = new (std::nothrow) clr_ctx();
+    if (!ctx) return CLR_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    DeviceScope ds(device);
+    if (ds.err != hipSuccess) {
+        delete ctx;
+        return CLR_ERR_NO_DEVICE;
+    }
+    for (auto& e : ctx->events) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return CLR_ERR_HIP;
+        }
+    }
+    *out = ctx;
+    return CLR_OK;
+}
+
+int clr_destroy(clr_ctx* ctx) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    {
+        DeviceScope ds(ctx->device);
+        (void)hipDeviceSynchronize();
+        if (ctx->pool) (void)hipFree(ctx->pool);
+        for (auto& e : ctx->events)
+            if (e) (void)hipEventDestroy(e);
+    }
+    delete ctx;
+    return CLR_OK;
+}
+
+const char* clr_last_error(const clr_ctx* ctx) { return ctx ? ctx->lastError.c_str() : ""; }
+
+int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes) {
+    if (!ctx || !bytes) return CLR_ERR_INVALID_ARG;
+    *bytes = ctx->poolBytes;
+    return CLR_OK;
+}
+
+int clr_set_profiling(clr_ctx* ctx, int enable) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    ctx->profiling = enable != 0;
+    ctx->haveProfile = false;
+    return CLR_OK;
+}
+
+int clr_lbvh_last_profile(clr_ctx* ctx, float* ms, int maxStages, int* numStages) {
+    if (!ctx || !ms || !numStages || maxStages < 0) return CLR_ERR_INVALID_ARG;
+    if (!ctx->haveProfile) return fail(ctx, CLR_ERR_INVALID_ARG, "no profiled build recorded");
+    DeviceScope ds(ctx->device);
+    int rc = hipCheck(ctx, hipEventSynchronize(ctx->events[ctx->numEvents - 1]), "profile sync");
+    if (rc) return rc;
+    int stages = ctx->numEvents - 1;
+    *numStages = stages;
+    for (int k = 0; k < stages && k < maxStages; ++k) {
+        float v = 0.0f;
+        rc = hipCheck(ctx, hipEventElapsedTime(&v, ctx->events[k], ctx->events[k + 1]),
+                      "profile elapsed");
+        if (rc) return rc;
+        ms[k] = v;
+    }
+    return CLR_OK;
+}
+
+static int checkBuildArgs(clr_ctx* ctx, uint32_t n, uint32_t bits, const void* inodes,
+                          const void* leaves) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (n == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "numFaces must be >= 1");
+    if (n >= (1u << 30)) return fail(ctx, CLR_ERR_UNSUPPORTED, "numFaces must be < 2^30");
+    if (bits < 1 || bits > 21)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "numBitsPerDim must be in [1, 21]");
+    if (!leaves || (n > 1 && !inodes))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null node buffer");
+    if (!aligned16(leaves) || (inodes && !aligned16(inodes)))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "node buffers must be 16-byte aligned");
+    return CLR_OK;
+}
+
+int clr_lbvh_build(clr_ctx* ctx, const clr_float3* vertices, const clr_face* faces,
+                   uint32_t numFaces, uint32_t numBitsPerDim, clr_lbvh_inode* internalNodes,
+                   clr_lbvh_leaf* leafNodes, void* stream) {
+    int rc = checkBuildArgs(ctx, numFaces, numBitsPerDim, internalNodes, leafNodes);
+    if (rc) return rc;
+    if (!vertices || !faces) return fail(ctx, CLR_ERR_INVALID_ARG, "null vertices/faces");
+    if (!aligned16(vertices)) return fail(ctx, CLR_ERR_INVALID_ARG, "vertices must be 16-B aligned");
+    if ((reinterpret_cast<uintptr_t>(faces) & 7u) != 0)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "faces must be 8-B aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    LbvhInput in;
+    in.vertices = vertices;
+    in.faces = faces;
+    return lbvhBuild(ctx, in, numFaces, numBitsPerDim, internalNodes, leafNodes,
+                     static_cast<hipStream_t>(stream));
+}
+
+int clr_lbvh_build_soup(clr_ctx* ctx, const float* tris, uint32_t numTris, uint32_t numBitsPerDim,
+                        clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes, void* stream) {
+    int rc = checkBuildArgs(ctx, numTris, numBitsPerDim, internalNodes, leafNodes);
+    if (rc) return rc;
+    if (!tris) return fail(ctx, CLR_ERR_INVALID_ARG, "null triangle soup");
+    if ((reinterpret_cast<uintptr_t>(tris) & 3u) != 0)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "soup must be 4-B aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    LbvhInput in;
+    in.tris = tris;
+    return lbvhBuild(ctx, in, numTris, numBitsPerDim, internalNodes, leafNodes,
+                     static_cast<hipStream_t>(stream));
+}
+
+int clr_lbvh_reserve(clr_ctx* ctx, uint32_t maxFaces, uint32_t numBitsPerDim) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (maxFaces == 0 || maxFaces >= (1u << 30) || numBitsPerDim < 1 || numBitsPerDim > 21)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "reserve size out of range");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return ensurePool(ctx, lbvhScratchBytes(maxFaces, numBitsPerDim));
+}
+
+int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
+                             uint32_t numSegments, uint32_t numBitsPerDim,
+                             clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes,
+                             void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!tris || !segOffsets || numSegments == 0 || !internalNodes || !leafNodes)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null argument or zero segments");
+    if (numBitsPerDim < 1 || numBitsPerDim > 10)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "segmented build supports numBitsPerDim 1..10");
+    if (segOffsets[0] != 0) return fail(ctx, CLR_ERR_INVALID_ARG, "segOffsets[0] must be 0");
+    for (uint32_t s = 0; s < numSegments; ++s)
+        if (segOffsets[s + 1] <= segOffsets[s])
+            return fail(ctx, CLR_ERR_INVALID_ARG, "every segment needs >= 1 triangle");
+    if (segOffsets[numSegments] >= (1u << 30))
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "total triangles must be < 2^30");
+    if (!aligned16(internalNodes) || !aligned16(leafNodes))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "node buffers must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return lbvhBuildSegmented(ctx, tris, segOffsets, numSegments, numBitsPerDim, internalNodes,
+                              leafNodes, static_cast<hipStream_t>(stream));
+}
+
+int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
+                      const clr_lbvh_leaf* leafNodes, uint32_t numFaces,
+                      const clr_float3* vertices, const clr_face* faces, const float* rays,
+                      uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
+                      void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (numFaces == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "numFaces must be >= 1");
+    if (!leafNodes || (numFaces > 1 && !internalNodes) || !vertices || !faces)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null scene buffer");
+    if (numRays > 0 && (!rays || !hitFace || !hitT))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null ray buffer");
+    if (!aligned16(leafNodes) || (internalNodes && !aligned16(internalNodes)) ||
+        !aligned16(vertices))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return traceClosest(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
+                        hitFace, hitT, stats, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

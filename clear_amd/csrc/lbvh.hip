@@ -1,0 +1,627 @@
+// LBVH construction for gfx950 (wave64).
+//
+// Replaces LBVHBuilder::perform (OpenCL_PathTracer/LBVHBuilder.cpp:160-375)
+// and the kernels of OpenCL/lbvh_construction.cl. Same outputs bit for bit
+// (Morton quantisation, stable order with index tie-break, Karras topology and
+// numbering, fmin/fmax boxes), different schedule:
+//
+//   init        clear histogram / lookback / counter scratch (fill kernel, no
+//               hipMemset, so the sequence can be graph-captured)
+//   bounds      per-triangle box, block min/max, one ordered-int atomic per
+//               block and component: the scene box never leaves the device
+//               (the reference reads it back to the host, LBVHBuilder.cpp:229)
+//   morton      centroid quantisation (lbvh_construction.cl:169-181) into one
+//               interleaved key (bit 3k+0 = x_k, 3k+1 = y_k, 3k+2 = z_k, the
+//               order of the reference's digit x + 2y + 4z), plus the 8-bit
+//               digit histograms of every radix pass
+//   onesweep    one launch per 8-bit digit: stable LSD pass with wave64
+//               ballot ranking and decoupled look-back (the reference spends
+//               numBitsPerDim passes of 8 launches on 3-bit digits)
+//   hierarchy   one bottom-up pass that derives the Karras 2012 binary radix
+//               tree (node numbering of constructBinaryRadixTree,
+//               lbvh_construction.cl:342-410) and the refit boxes
+//               (calcNodeAABBs, :415-492) together: each leaf climbs, the
+//               second arrival at a parent merges fmin/fmax(left, right) and
+//               continues; release/acquire on the arrival word orders the box
+//               writes (the reference's atomic_inc has no fence).
+//
+// A node covering sorted range [l, r] is the left child of its parent when the
+// common prefix across its right boundary, delta(r, r+1), exceeds the one
+// across its left boundary, delta(l-1, l); the parent splits at r then, else at
+// l-1. Karras names a left child by its right end and a right child by its left
+// end, which fixes every node's index on the way up.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.h"
+
+namespace clr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSortBlock = 256;
+constexpr int kWaves = kSortBlock / 64;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kFlagMask = 3u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1;
+constexpr int kMaxPasses = 8;
+
+template <typename KeyT>
+struct SortCfg;
+template <>
+struct SortCfg<uint32_t> {
+    static constexpr int kItems = 16;
+};
+template <>
+struct SortCfg<uint64_t> {
+    static constexpr int kItems = 12;
+};
+
+// OpenCL fmin/fmax with a fixed answer for +-0 ties (left operand) and NaN
+// (the other operand); the oracle uses the same rule.
+__device__ __forceinline__ float omin(float a, float b) { return (b < a || a != a) ? b : a; }
+__device__ __forceinline__ float omax(float a, float b) { return (b > a || a != a) ? b : a; }
+
+// Total order on floats as unsigned ints (for atomicMin/Max).
+__device__ __forceinline__ uint32_t ordOf(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float floatOf(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ Box boxOf3(const float* p0, const float* p1, const float* p2) {
+    Box b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = omin(p0[k], omin(p1[k], p2[k]));
+        b.hi[k] = omax(p0[k], omax(p1[k], p2[k]));
+    }
+    return b;
+}
+
+// Primitive source: a 36-B triangle soup or a precomputed [n][6] box array.
+struct PrimSrc {
+    const float* tris;   // soup
+    const float* boxes;  // box array (drop-in path, written by the bounds kernel)
+};
+
+template <bool SOUP>
+__device__ __forceinline__ Box loadBox(const PrimSrc& s, uint32_t i) {
+    if constexpr (SOUP) {
+        const float* t = s.tris + 9 * (size_t)i;
+        float p0[3] = {t[0], t[1], t[2]};
+        float p1[3] = {t[3], t[4], t[5]};
+        float p2[3] = {t[6], t[7], t[8]};
+        return boxOf3(p0, p1, p2);
+    } else {
+        const float2* b = reinterpret_cast<const float2*>(s.boxes + 6 * (size_t)i);
+        float2 a = b[0], c = b[1], d = b[2];
+        Box r;
+        r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = c.x;
+        r.hi[0] = c.y; r.hi[1] = d.x; r.hi[2] = d.y;
+        return r;
+    }
+}
+
+__device__ __forceinline__ float waveMin(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = omin(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ float waveMax(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = omax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// ---------------------------------------------------------------- init ----
+__global__ __launch_bounds__(kBlock) void k_init(uint32_t* words, size_t count,
+                                                 uint32_t* sceneOrd) {
+    size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * kBlock;
+    for (; i < count; i += stride) words[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 6)
+        sceneOrd[threadIdx.x] = threadIdx.x < 3 ? ordOf(INFINITY) : ordOf(-INFINITY);
+}
+
+// -------------------------------------------------------------- bounds ----
+// SOUP: reads tris; else gathers faces + vertices and writes the box array.
+template <bool SOUP>
+__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris,
+                                                   const clr_float3* __restrict__ verts,
+                                                   const clr_face* __restrict__ faces, uint32_t n,
+                                                   float* __restrict__ boxesOut,
+                                                   uint32_t* __restrict__ sceneOrd) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        Box b;
+        if constexpr (SOUP) {
+            PrimSrc s{tris, nullptr};
+            b = loadBox<true>(s, i);
+        } else {
+            const uint32_t* f = reinterpret_cast<const uint32_t*>(faces + i);
+            float4 v0 = *reinterpret_cast<const float4*>(verts + f[0]);
+            float4 v1 = *reinterpret_cast<const float4*>(verts + f[1]);
+            float4 v2 = *reinterpret_cast<const float4*>(verts + f[2]);
+            float p0[3] = {v0.x, v0.y, v0.z}, p1[3] = {v1.x, v1.y, v1.z},
+                  p2[3] = {v2.x, v2.y, v2.z};
+            b = boxOf3(p0, p1, p2);
+            float2* o = reinterpret_cast<float2*>(boxesOut + 6 * (size_t)i);
+            o[0] = make_float2(b.lo[0], b.lo[1]);
+            o[1] = make_float2(b.lo[2], b.hi[0]);
+            o[2] = make_float2(b.hi[1], b.hi[2]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = omin(lo[k], b.lo[k]);
+            hi[k] = omax(hi[k], b.hi[k]);
+        }
+    }
+    __shared__ float s_red[kBlock / 64][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = waveMin(lo[k]);
+        hi[k] = waveMax(hi[k]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_red[wave][k] = lo[k];
+            s_red[wave][3 + k] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float v = s_red[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) v = k < 3 ? omin(v, s_red[w][k]) : omax(v, s_red[w][k]);
+        if (k < 3)
+            atomicMin(sceneOrd + k, ordOf(v));
+        else
+            atomicMax(sceneOrd + k, ordOf(v));
+    }
+}
+
+// -------------------------------------------------------------- morton ----
+template <typename KeyT>
+__device__ __forceinline__ KeyT spread3(uint32_t v);
+template <>
+__device__ __forceinline__ uint32_t spread3<uint32_t>(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+template <>
+__device__ __forceinline__ uint64_t spread3<uint64_t>(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// min(convert_uint_rtz(v), numDivs - 1), NaN and non-positive values to 0.
+__device__ __forceinline__ uint32_t quantize(float v, uint32_t numDivs) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return numDivs - 1u;
+    uint32_t q = (uint32_t)v;
+    return q < numDivs - 1u ? q : numDivs - 1u;
+}
+
+template <bool SOUP, typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint32_t bits,
+                                                   const uint32_t* __restrict__ sceneOrd,
+                                                   KeyT* __restrict__ keys,
+                                                   uint32_t* __restrict__ otherEnd,
+                                                   uint32_t* __restrict__ hist, int numPasses) {
+    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    for (int j = threadIdx.x; j < numPasses * 256; j += kBlock) s_hist[j] = 0u;
+    __syncthreads();
+
+    float smin[3], ssize[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        smin[k] = floatOf(sceneOrd[k]);
+        ssize[k] = floatOf(sceneOrd[3 + k]) - smin[k];
+    }
+    const uint32_t numDivs = 1u << bits;
+    const float fDivs = (float)numDivs;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        Box b = loadBox<SOUP>(src, i);
+        uint32_t q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float c = (b.lo[k] + b.hi[k]) * 0.5f;
+            float np = (c - smin[k]) / ssize[k];
+            q[k] = quantize(np * fDivs, numDivs);
+        }
+        KeyT key = spread3<KeyT>(q[0]) | (spread3<KeyT>(q[1]) << 1) | (spread3<KeyT>(q[2]) << 2);
+        keys[i] = key;
+        if (i + 1 < n) otherEnd[i] = kEmpty;
+        for (int p = 0; p < numPasses; ++p)
+            atomicAdd(&s_hist[p * 256 + (uint32_t)((key >> (8 * p)) & 0xFFu)], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < numPasses * 256; j += kBlock)
+        if (s_hist[j]) atomicAdd(hist + j, s_hist[j]);
+}
+
+// ------------------------------------------------------------ onesweep ----
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* s_tmp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; ++w) pre += s_tmp[w];
+    __syncthreads();
+    return pre + x - v;
+}
+
+template <typename KeyT, bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict__ keysIn,
+                                                         const uint32_t* __restrict__ valsIn,
+                                                         KeyT* __restrict__ keysOut,
+                                                         uint32_t* __restrict__ valsOut,
+                                                         uint32_t n, int shift,
+                                                         const uint32_t* __restrict__ passHist,
+                                                         uint32_t* __restrict__ status,
+                                                         uint32_t* __restrict__ partCounter) {
+    constexpr int kItems = SortCfg<KeyT>::kItems;
+    constexpr int kTile = kSortBlock * kItems;
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_waveHist[kWaves][256];
+    __shared__ uint32_t s_localStart[256];
+    __shared__ uint32_t s_digitBase[256];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ KeyT s_keys[kTile];
+    __shared__ uint32_t s_vals[kTile];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int j = tid; j < kWaves * 256; j += kSortBlock) (&s_waveHist[0][0])[j] = 0u;
+    if (tid == 0) s_part = atomicAdd(partCounter, 1u);
+    __syncthreads();
+    const uint32_t part = s_part;
+    const uint32_t tileBase = part * (uint32_t)kTile;
+    const uint32_t waveBase = tileBase + (uint32_t)wave * (64u * kItems);
+
+    KeyT keys[kItems];
+    uint32_t vals[kItems];
+    uint32_t rank[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        uint32_t idx = waveBase + (uint32_t)i * 64u + (uint32_t)lane;
+        if (idx < n) {
+            keys[i] = keysIn[idx];
+            vals[i] = FIRST ? idx : valsIn[idx];
+        } else {
+            keys[i] = ~KeyT(0);  // digit 0xFF at every shift, ranked last
+            vals[i] = 0u;
+        }
+    }
+
+    // Rank each item among equal digits of its wave (wave-level multisplit):
+    // 8 ballots give the set of lanes holding the same digit; the lowest such
+    // lane owns the wave's running counter for that digit.
+    const uint64_t ltMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t d = (uint32_t)(keys[i] >> shift) & 0xFFu;
+        uint64_t peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const int leader = __ffsll((unsigned long long)peers) - 1;
+        const uint32_t below = (uint32_t)__popcll(peers & ltMask);
+        uint32_t base = 0;
+        if (lane == leader) {
+            base = s_waveHist[wave][d];
+            s_waveHist[wave][d] = base + (uint32_t)__popcll(peers);
+        }
+        base = __shfl(base, leader);
+        rank[i] = base + below;
+    }
+    __syncthreads();
+
+    // Per digit: wave offsets, tile count, look-back for the tile's global
+    // offset. Thread t owns digit t.
+    uint32_t tileCount = 0;
+    {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            uint32_t c = s_waveHist[w][tid];
+            s_waveHist[w][tid] = sum;
+            sum += c;
+        }
+        tileCount = sum;
+    }
+    uint32_t exclusive = 0;
+    if (part == 0) {
+        __hip_atomic_store(status + tid, kFlagInc | tileCount, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagAgg | tileCount,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t look = (int64_t)part - 1;
+        while (true) {
+            uint32_t v = __hip_atomic_load(status + (size_t)look * 256 + tid, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t flag = v & kFlagMask;
+            if (flag == 0u) continue;
+            exclusive += v & kCountMask;
+            if (flag == kFlagInc) break;
+            --look;
+        }
+        __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagInc | (exclusive + tileCount),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t localStart = blockExclusiveScan256(tileCount, s_tmp);
+    const uint32_t globalStart = blockExclusiveScan256(passHist[tid], s_tmp);
+    s_localStart[tid] = localStart;
+    s_digitBase[tid] = globalStart + exclusive;
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t d = (uint32_t)(keys[i] >> shift) & 0xFFu;
+        const uint32_t pos = s_localStart[d] + s_waveHist[wave][d] + rank[i];
+        s_keys[pos] = keys[i];
+        s_vals[pos] = vals[i];
+    }
+    __syncthreads();
+    const uint32_t valid = min((uint32_t)kTile, n - tileBase);
+    for (uint32_t j = tid; j < valid; j += kSortBlock) {
+        const KeyT k = s_keys[j];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+        const uint32_t g = s_digitBase[d] + (j - s_localStart[d]);
+        keysOut[g] = k;
+        valsOut[g] = s_vals[j];
+    }
+}
+
+// ----------------------------------------------------------- hierarchy ----
+template <typename KeyT>
+__device__ __forceinline__ int clzKey(KeyT v);
+template <>
+__device__ __forceinline__ int clzKey<uint32_t>(uint32_t v) {
+    return __clz((int)v);
+}
+template <>
+__device__ __forceinline__ int clzKey<uint64_t>(uint64_t v) {
+    return __clzll((long long)v);
+}
+
+// delta(x, x+1) over extended keys (key, sorted position); the reference's
+// _numCommonBits (lbvh_construction.cl:322-332) for adjacent positions.
+template <typename KeyT>
+__device__ __forceinline__ int deltaAdj(const KeyT* __restrict__ k, uint32_t x, int keyBits) {
+    const KeyT a = k[x], b = k[x + 1];
+    if (a != b) return clzKey<KeyT>(a ^ b) - (int)(8 * sizeof(KeyT)) + keyBits;
+    return keyBits + __clz((int)(x ^ (x + 1u)));
+}
+
+__device__ __forceinline__ void writeLeaf(clr_lbvh_leaf* leaves, uint32_t i, const Box& b,
+                                          uint32_t obj) {
+    uint4* o = reinterpret_cast<uint4*>(leaves + i);
+    o[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]),
+                      __float_as_uint(b.lo[2]), 0u);
+    o[1] = make_uint4(__float_as_uint(b.hi[0]), __float_as_uint(b.hi[1]),
+                      __float_as_uint(b.hi[2]), 0u);
+    o[2] = make_uint4(obj, 0u, 0u, 0u);
+}
+
+__device__ __forceinline__ void writeNode(clr_lbvh_inode* nodes, uint32_t idx, const Box& b,
+                                          uint32_t split, uint32_t l, uint32_t r) {
+    uint4* o = reinterpret_cast<uint4*>(nodes + idx);
+    o[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]),
+                      __float_as_uint(b.lo[2]), 0u);
+    o[1] = make_uint4(__float_as_uint(b.hi[0]), __float_as_uint(b.hi[1]),
+                      __float_as_uint(b.hi[2]), 0u);
+    const uint32_t flags = (split == l ? 1u : 0u) | (split + 1u == r ? 0x100u : 0u);
+    o[2] = make_uint4(flags, split, split + 1u, 0u);
+}
+
+__device__ __forceinline__ Box readBox(const void* rec) {
+    const float4* p = reinterpret_cast<const float4*>(rec);
+    float4 a = p[0], b = p[1];
+    Box r;
+    r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z;
+    r.hi[0] = b.x; r.hi[1] = b.y; r.hi[2] = b.z;
+    return r;
+}
+
+template <bool SOUP, typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_hierarchy(PrimSrc src, const KeyT* __restrict__ skeys,
+                                                      const uint32_t* __restrict__ sidx,
+                                                      uint32_t n, int keyBits,
+                                                      uint32_t* __restrict__ otherEnd,
+                                                      clr_lbvh_inode* __restrict__ inodes,
+                                                      clr_lbvh_leaf* __restrict__ leaves) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t obj = sidx[i];
+    Box box = loadBox<SOUP>(src, obj);
+    writeLeaf(leaves, i, box, obj);
+    if (n == 1) return;
+
+    uint32_t l = i, r = i, split = 0;
+    bool internal = false;
+    while (true) {
+        if (l == 0 && r == n - 1) {
+            writeNode(inodes, 0, box, split, l, r);
+            return;
+        }
+        const int dl = l > 0 ? deltaAdj<KeyT>(skeys, l - 1, keyBits) : -1;
+        const int dr = r < n - 1 ? deltaAdj<KeyT>(skeys, r, keyBits) : -1;
+        const bool isLeft = dr > dl;
+        const uint32_t parentSplit = isLeft ? r : l - 1;
+        if (internal) writeNode(inodes, isLeft ? r : l, box, split, l, r);
+        const uint32_t other =
+            __hip_atomic_exchange(otherEnd + parentSplit, isLeft ? l : r, __ATOMIC_ACQ_REL,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+        if (other == kEmpty) return;  // first arrival: the sibling finishes the parent
+        if (isLeft) {
+            const uint32_t s = parentSplit + 1u;
+            const Box sb = (s == other) ? readBox(leaves + s) : readBox(inodes + s);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box.lo[k] = omin(box.lo[k], sb.lo[k]);
+                box.hi[k] = omax(box.hi[k], sb.hi[k]);
+            }
+            r = other;
+        } else {
+            const uint32_t s = parentSplit;
+            const Box sb = (s == other) ? readBox(leaves + s) : readBox(inodes + s);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box.lo[k] = omin(sb.lo[k], box.lo[k]);
+                box.hi[k] = omax(sb.hi[k], box.hi[k]);
+            }
+            l = other;
+        }
+        split = parentSplit;
+        internal = true;
+    }
+}
+
+struct Layout {
+    uint32_t* sceneOrd;
+    uint32_t* hist;
+    uint32_t* partCounters;
+    uint32_t* status;
+    size_t clearWords;  // hist + partCounters + status are contiguous
+    float* boxes;
+    void* keysA;
+    void* keysB;
+    uint32_t* valsA;
+    uint32_t* valsB;
+    uint32_t* otherEnd;
+    size_t bytes;
+};
+
+inline int numPassesFor(uint32_t bits) { return (int)((3 * bits + 7) / 8); }
+inline uint32_t tileFor(uint32_t bits) {
+    return bits <= 10 ? (uint32_t)(kSortBlock * SortCfg<uint32_t>::kItems)
+                      : (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems);
+}
+
+Layout layoutFor(void* base, uint32_t n, uint32_t bits) {
+    Carve c(base);
+    Layout L{};
+    const int P = numPassesFor(bits);
+    const uint32_t parts = (n + tileFor(bits) - 1) / tileFor(bits);
+    const size_t keyBytes = bits <= 10 ? 4 : 8;
+    L.sceneOrd = c.take<uint32_t>(8);
+    // hist, counters and status carved back to back so one fill clears them
+    L.hist = c.take<uint32_t>((size_t)kMaxPasses * 256 + 64 + (size_t)P * parts * 256);
+    L.partCounters = L.hist + (size_t)kMaxPasses * 256;
+    L.status = L.partCounters + 64;
+    L.clearWords = (size_t)kMaxPasses * 256 + 64 + (size_t)P * parts * 256;
+    L.boxes = c.take<float>(6 * (size_t)n);
+    L.keysA = c.take<char>(keyBytes * n);
+    L.keysB = c.take<char>(keyBytes * n);
+    L.valsA = c.take<uint32_t>(n);
+    L.valsB = c.take<uint32_t>(n);
+    L.otherEnd = c.take<uint32_t>(n);
+    L.bytes = c.off + 256;
+    return L;
+}
+
+inline unsigned gridFor(uint32_t n, unsigned cap) {
+    unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+    return g < 1 ? 1 : (g > cap ? cap : g);
+}
+
+template <typename KeyT, bool SOUP>
+int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
+             clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
+    Layout L = layoutFor(ctx->pool, n, bits);
+    const int P = numPassesFor(bits);
+    const uint32_t tile = tileFor(bits);
+    const uint32_t parts = (n + tile - 1) / tile;
+    const int keyBits = (int)(3 * bits);
+    auto ev = [&](int k) {
+        if (ctx->profiling) (void)hipEventRecord(ctx->events[k], stream);
+    };
+    ev(0);
+    k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0,
+             stream>>>(L.hist, L.clearWords, L.sceneOrd);
+    k_bounds<SOUP><<<gridFor(n, 4096), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n,
+                                                              L.boxes, L.sceneOrd);
+    ev(1);
+    PrimSrc src{in.tris, L.boxes};
+    k_morton<SOUP, KeyT><<<gridFor(n, 4096), kBlock, 0, stream>>>(
+        src, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.otherEnd, L.hist, P);
+    ev(2);
+    KeyT* kin = static_cast<KeyT*>(L.keysA);
+    KeyT* kout = static_cast<KeyT*>(L.keysB);
+    uint32_t* vin = L.valsA;
+    uint32_t* vout = L.valsB;
+    for (int p = 0; p < P; ++p) {
+        uint32_t* st = L.status + (size_t)p * parts * 256;
+        if (p == 0)
+            k_onesweep<KeyT, true><<<parts, kSortBlock, 0, stream>>>(
+                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p);
+        else
+            k_onesweep<KeyT, false><<<parts, kSortBlock, 0, stream>>>(
+                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    ev(3);
+    k_hierarchy<SOUP, KeyT><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        src, kin, vin, n, keyBits, L.otherEnd, inodes, leaves);
+    ev(4);
+    if (ctx->profiling) {
+        ctx->numEvents = 5;
+        ctx->haveProfile = true;
+        ctx->profStream = stream;
+    }
+    return hipCheck(ctx, hipGetLastError(), "LBVH build launch");
+}
+
+}  // namespace
+
+size_t lbvhScratchBytes(uint32_t n, uint32_t bitsPerDim) {
+    return layoutFor(nullptr, n, bitsPerDim).bytes;
+}
+
+int lbvhBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
+              clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
+    int rc = ensurePool(ctx, lbvhScratchBytes(n, bits));
+    if (rc) return rc;
+    const bool soup = in.tris != nullptr;
+    if (bits <= 10)
+        return soup ? runBuild<uint32_t, true>(ctx, in, n, bits, inodes, leaves, stream)
+                    : runBuild<uint32_t, false>(ctx, in, n, bits, inodes, leaves, stream);
+    return soup ? runBuild<uint64_t, true>(ctx, in, n, bits, inodes, leaves, stream)
+                : runBuild<uint64_t, false>(ctx, in, n, bits, inodes, leaves, stream);
+}
+
+}  // namespace clr

@@ -1,0 +1,173 @@
+// Device-side ray/box, ray/triangle and LBVH closest-hit routines shared by
+// the trace entry point and the renderers.
+//
+// Arithmetic is kept in the reference's operation order with no contraction
+// (the library is built with -ffp-contract=off) so that results equal the CPU
+// oracle's bit for bit: dot/cross left to right, IEEE division.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/clear_amd.h"
+
+namespace clr {
+namespace dev {
+
+constexpr int kStackDepth = 64;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float omax(float a, float b) { return (b > a || a != a) ? b : a; }
+__device__ __forceinline__ float omin(float a, float b) { return (b < a || a != a) ? b : a; }
+
+__device__ __forceinline__ V3 loadV3(const clr_float3* p, uint32_t i) {
+    const float4 v = *reinterpret_cast<const float4*>(p + i);
+    return {v.x, v.y, v.z};
+}
+
+struct GeomScene {
+    const clr_lbvh_inode* inodes;
+    const clr_lbvh_leaf* leaves;
+    uint32_t n;
+    const clr_float3* verts;
+    const clr_face* faces;
+};
+
+struct TraceStats {
+    uint32_t visits, tests;
+};
+
+// rayAABBIntersection (bvh_traversal.cl:160-179): per axis the candidate plane
+// on the origin's side, t = (plane - org) / dir (-1 when inside or dir == 0),
+// maxt = max t; reject if maxt < 0 or > tBound; the entry point must lie in the
+// box on every axis (or that axis produced maxt).
+__device__ __forceinline__ bool boxTest(float4 mn, float4 mx, V3 o, V3 d, float tBound) {
+    const int lx = o.x < mn.x ? -1 : (o.x > mx.x ? 1 : 0);
+    const int ly = o.y < mn.y ? -1 : (o.y > mx.y ? 1 : 0);
+    const int lz = o.z < mn.z ? -1 : (o.z > mx.z ? 1 : 0);
+    if ((lx | ly | lz) == 0) return true;
+    const float px = o.x < mn.x ? mn.x : (o.x > mx.x ? mx.x : 0.0f);
+    const float py = o.y < mn.y ? mn.y : (o.y > mx.y ? mx.y : 0.0f);
+    const float pz = o.z < mn.z ? mn.z : (o.z > mx.z ? mx.z : 0.0f);
+    const float tx = (lx != 0 && d.x != 0.0f) ? ((px - o.x) / d.x) : -1.0f;
+    const float ty = (ly != 0 && d.y != 0.0f) ? ((py - o.y) / d.y) : -1.0f;
+    const float tz = (lz != 0 && d.z != 0.0f) ? ((pz - o.z) / d.z) : -1.0f;
+    const float maxt = omax(tx, omax(ty, tz));
+    if (maxt < 0.0f || maxt > tBound) return false;
+    const float cx = o.x + maxt * d.x;
+    const float cy = o.y + maxt * d.y;
+    const float cz = o.z + maxt * d.z;
+    const bool hx = (cx >= mn.x && cx <= mx.x) || tx == maxt;
+    const bool hy = (cy >= mn.y && cy <= mx.y) || ty == maxt;
+    const bool hz = (cz >= mn.z && cz <= mx.z) || tz == maxt;
+    return hx && hy && hz;
+}
+
+// Geometric part of rayTriangleIntersection (bvh_traversal.cl:54-92) with a
+// 32-bit face index. On acceptance returns true and sets t, b1, b2.
+__device__ __forceinline__ bool triTest(const GeomScene& g, V3 o, V3 d, uint32_t faceIdx, float& t,
+                                        float& b1Out, float& b2Out) {
+    const uint32_t* f = reinterpret_cast<const uint32_t*>(g.faces + faceIdx);
+    const uint2 f01 = *reinterpret_cast<const uint2*>(f);
+    const uint32_t f2 = f[2];
+    const V3 p0 = loadV3(g.verts, f01.x);
+    const V3 p1 = loadV3(g.verts, f01.y);
+    const V3 p2 = loadV3(g.verts, f2);
+    const V3 edge01 = p1 - p0;
+    const V3 edge02 = p2 - p0;
+    const V3 p = cross(d, edge02);
+    const float det = dot(edge01, p);
+    if (det > -0.000001f && det < 0.000001f) return false;
+    const float invDet = 1.0f / det;
+    const V3 dd = o - p0;
+    const float b1 = dot(dd, p) * invDet;
+    if (b1 < 0.0f || b1 > 1.0f) return false;
+    const V3 q = cross(dd, edge01);
+    const float b2 = dot(d, q) * invDet;
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
+    const float tt = dot(edge02, q) * invDet;
+    if (tt < 0.0f || tt > t) return false;
+    t = tt;
+    b1Out = b1;
+    b2Out = b2;
+    return true;
+}
+
+// Alpha rejection hook: NoAlpha accepts every geometric hit.
+struct NoAlpha {
+    __device__ __forceinline__ bool reject(uint32_t, float, float) const { return false; }
+};
+
+// rayIntersection (bvh_traversal.cl:182-215). `stack` points at this lane's
+// first slot; slot k is stack[k * stride]. Returns the face or kNone; t, b1,
+// b2 describe the accepted hit. An alpha-rejected triangle leaves t unchanged
+// (the reference stores t before its alpha test, :87 vs :102-105, which lets
+// a masked-out triangle occlude; that is not reproduced).
+template <bool STATS, class Alpha = NoAlpha>
+__device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, float& t,
+                                               uint32_t* stack, int stride, TraceStats& st,
+                                               float* b1Out = nullptr, float* b2Out = nullptr,
+                                               const Alpha& alpha = Alpha()) {
+    uint32_t best = kNone;
+    float bb1 = 0.0f, bb2 = 0.0f;
+    auto tryLeaf = [&](uint32_t leafIdx) {
+        const float4* lp = reinterpret_cast<const float4*>(g.leaves + leafIdx);
+        const float4 mn = lp[0], mx = lp[1];
+        if (!boxTest(mn, mx, o, d, t)) return;
+        const uint32_t obj = reinterpret_cast<const uint32_t*>(lp + 2)[0];
+        if constexpr (STATS) ++st.tests;
+        float tt = t, b1, b2;
+        if (!triTest(g, o, d, obj, tt, b1, b2)) return;
+        if (alpha.reject(obj, b1, b2)) return;
+        t = tt;
+        best = obj;
+        bb1 = b1;
+        bb2 = b2;
+    };
+    if (g.n == 1) {
+        tryLeaf(0);
+    } else if (g.n > 1) {
+        int depth = 0;
+        stack[0] = 0u;
+        depth = 1;
+        while (depth > 0) {
+            --depth;
+            const uint32_t idx = stack[depth * stride];
+            const float4* np = reinterpret_cast<const float4*>(g.inodes + idx);
+            const float4 mn = np[0], mx = np[1];
+            if (!boxTest(mn, mx, o, d, t)) continue;
+            const uint4 c = reinterpret_cast<const uint4*>(np)[2];
+            if constexpr (STATS) ++st.visits;
+            if (c.x & 0xFFu)
+                tryLeaf(c.y);
+            else if (depth < kStackDepth)
+                stack[(depth++) * stride] = c.y;
+            if (c.x & 0xFF00u)
+                tryLeaf(c.z);
+            else if (depth < kStackDepth)
+                stack[(depth++) * stride] = c.z;
+        }
+    }
+    if (b1Out) *b1Out = bb1;
+    if (b2Out) *b2Out = bb2;
+    return best;
+}
+
+}  // namespace dev
+}  // namespace clr

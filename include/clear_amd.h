@@ -1,0 +1,164 @@
+/*
+ * clear_amd.h — C-ABI of the MI355X (gfx950) LBVH + path-tracer hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns an
+ * int status (CLR_OK == 0) and never throws. Device pointers are caller-owned
+ * and must live on the context's device. Work is enqueued on `stream`
+ * (a hipStream_t passed as void*; NULL = the default stream) and is
+ * stream-ordered: no entry point synchronises the host unless its comment says
+ * so. A clr_ctx is externally synchronised: one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   OpenCL_PathTracer/LBVHBuilder.h:32-75, LBVHBuilder.cpp:160-375  (LBVHBuilder)
+ *   OpenCL_PathTracer/OpenCL/bvh_traversal.cl:182-215                (rayIntersection)
+ *   OpenCL_PathTracer/OpenCL/pathtracer.cl:25-197                    (pathtracing)
+ *   OpenCL_PathTracer/main.cpp:475-494                               (render loop)
+ */
+#ifndef CLEAR_AMD_H
+#define CLEAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    CLR_OK = 0,
+    CLR_ERR_INVALID_ARG = 1,   /* null pointer, bad size, bad enum          */
+    CLR_ERR_OUT_OF_MEMORY = 2, /* scratch pool could not grow               */
+    CLR_ERR_HIP = 3,           /* a HIP runtime call failed                 */
+    CLR_ERR_UNSUPPORTED = 4,   /* parameter outside the supported range     */
+    CLR_ERR_NO_DEVICE = 5      /* no usable gfx950 device at that ordinal   */
+};
+
+/* ---- data ABI (byte-compatible with the reference host structs) --------- */
+
+/* cl_float3: 16 bytes, w unused. Vertices, normals and tangents use it
+ * (Scene.h:31-33, uploaded as-is at main.cpp:342). */
+typedef struct clr_float3 {
+    float x, y, z, w;
+} clr_float3;
+
+/* Face, 56 bytes (Face.hpp:15-73; scene.cl:22-29). UINT32_MAX = attribute
+ * absent; matPtr/lightPtr are byte offsets into materialsData, 0xFFFF = none. */
+typedef struct clr_face {
+    uint32_t p0, p1, p2;
+    uint32_t vn0, vn1, vn2;
+    uint32_t vt0, vt1, vt2;
+    uint32_t uv0, uv1, uv2;
+    uint32_t alphaTexPtr;
+    uint16_t matPtr, lightPtr;
+} clr_face;
+
+/* LBVH::InternalNode, 48 bytes (LBVHBuilder.h:17-22). Node 0 is the root.
+ * isChild[k] != 0 means child k is a leaf index, else an internal index. */
+typedef struct clr_lbvh_inode {
+    float min[4];
+    float max[4];
+    uint8_t isChild[2];
+    uint8_t dum0[2];
+    uint32_t c[2];
+    uint8_t dum1[4];
+} clr_lbvh_inode;
+
+/* LBVH::LeafNode, 48 bytes (LBVHBuilder.h:25-29). */
+typedef struct clr_lbvh_leaf {
+    float min[4];
+    float max[4];
+    uint32_t objIdx;
+    uint8_t dum0[12];
+} clr_lbvh_leaf;
+
+/* LightInfo, 8 bytes (Scene.h:23-27; scene.cl:32-35). */
+typedef struct clr_light_info {
+    uint8_t atInfinity;
+    uint8_t dum0[3];
+    uint32_t reference;
+} clr_light_info;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct clr_ctx clr_ctx;
+
+/* Creates a context on HIP device `device` (replaces the LBVHBuilder ctor,
+ * LBVHBuilder.cpp:19-48, minus the runtime compile: kernels are prebuilt). */
+int clr_create(clr_ctx** ctx, int device);
+int clr_destroy(clr_ctx* ctx);
+/* Static string for a status code. */
+const char* clr_error_string(int status);
+/* Detail of the last failure on this context ("" when none). */
+const char* clr_last_error(const clr_ctx* ctx);
+/* Bytes currently held by the context's scratch pool. */
+int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes);
+
+/* ---- LBVH build ---------------------------------------------------------- */
+
+/* Drop-in for LBVHBuilder::perform (LBVHBuilder.cpp:160-375).
+ *   vertices      clr_float3[]    (device, 16-B stride, as buf_vertices)
+ *   faces         clr_face[numFaces] (device)
+ *   numBitsPerDim quantisation bits per axis, 1..21 (reference main uses 20;
+ *                 10 gives the 30-bit Morton code)
+ *   internalNodes clr_lbvh_inode[numFaces-1] (device, written)
+ *   leafNodes     clr_lbvh_leaf[numFaces]    (device, written)
+ * Unlike the reference there is no host read-back of the scene bounds inside
+ * the build (LBVHBuilder.cpp:228-229): the call is fully stream-ordered and
+ * may be captured into a hipGraph once the scratch pool is large enough
+ * (see clr_lbvh_reserve). numFaces == 1 writes leafNodes[0] only. */
+int clr_lbvh_build(clr_ctx* ctx, const clr_float3* vertices, const clr_face* faces,
+                   uint32_t numFaces, uint32_t numBitsPerDim,
+                   clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes, void* stream);
+
+/* Same build from a triangle soup: tris[numTris][9] = p0.xyz p1.xyz p2.xyz
+ * (device, 36-B records). Identical output to clr_lbvh_build on the
+ * equivalent vertices/faces (vertex 3i+k = corner k of triangle i). */
+int clr_lbvh_build_soup(clr_ctx* ctx, const float* tris, uint32_t numTris,
+                        uint32_t numBitsPerDim, clr_lbvh_inode* internalNodes,
+                        clr_lbvh_leaf* leafNodes, void* stream);
+
+/* Pre-grows the scratch pool for builds of up to maxFaces triangles (mirrors
+ * the capacity argument of the LBVHBuilder ctor). Synchronous. */
+int clr_lbvh_reserve(clr_ctx* ctx, uint32_t maxFaces, uint32_t numBitsPerDim);
+
+/* Segmented build: numSegments independent scenes in one set of launches.
+ * segOffsets[numSegments+1] (host) are prefix offsets into the soup (in
+ * triangles); scene s owns tris[segOffsets[s] .. segOffsets[s+1]).
+ * Outputs are concatenated: scene s writes leafNodes[segOffsets[s] ...] and
+ * internalNodes[segOffsets[s] - s ...] (numTris_s - 1 nodes), with child
+ * indices local to the scene, exactly as clr_lbvh_build_soup would write them
+ * for that scene alone. Every segment must hold >= 1 triangle. */
+int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
+                             uint32_t numSegments, uint32_t numBitsPerDim,
+                             clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes,
+                             void* stream);
+
+/* Per-stage GPU times (ms) of the most recent build when profiling was
+ * enabled with clr_set_profiling(ctx, 1): [bounds, morton, histogram,
+ * sort, hierarchy]. Synchronises the stream of that build. */
+int clr_set_profiling(clr_ctx* ctx, int enable);
+int clr_lbvh_last_profile(clr_ctx* ctx, float* msPerStage, int maxStages, int* numStages);
+
+/* ---- closest-hit traversal ----------------------------------------------- */
+
+/* Closest hit of numRays rays against an LBVH (semantics of rayIntersection,
+ * bvh_traversal.cl:182-215: t starts at tmax, a triangle replaces the current
+ * hit when its t <= current t, Moller-Trumbore with |det| < 1e-6 culling).
+ *   rays   device float[7][numRays] SoA planes: ox, oy, oz, dx, dy, dz, tmax
+ *   hitFace device uint32_t[numRays]: face index or 0xFFFFFFFF on a miss
+ *   hitT    device float[numRays]: hit distance (tmax on a miss)
+ *   stats   optional device uint32_t[2*numRays]: internal-node visits and
+ *           triangle tests per ray (NULL to skip)
+ * Face indices are 32-bit (the reference truncates to 16 bits,
+ * bvh_traversal.cl:43,55). */
+int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
+                      const clr_lbvh_leaf* leafNodes, uint32_t numFaces,
+                      const clr_float3* vertices, const clr_face* faces,
+                      const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
+                      uint32_t* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLEAR_AMD_H */

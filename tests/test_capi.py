@@ -1,0 +1,80 @@
+"""Host-side checks of the C-ABI library (no GPU needed).
+
+The library must load, export every function include/*.h declares, report
+status strings, and refuse to create a context where there is no device.
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import clear_amd as ca
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(clr_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+
+    L = ctypes.CDLL(str(ca.library_path()))
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    assert len(declared_functions()) >= 12
+
+
+def test_binding_covers_every_declared_symbol():
+    ca.lib()
+    bound = set(ca.exported_symbols())
+    assert set(declared_functions()) <= bound
+
+
+def test_error_strings():
+    L = ca.lib()
+    assert L.clr_error_string(0) == b"ok"
+    assert L.clr_error_string(1) == b"invalid argument"
+    assert L.clr_error_string(99) == b"unknown status"
+
+
+def test_null_arguments_rejected_without_device():
+    import ctypes
+
+    L = ca.lib()
+    assert L.clr_create(None, 0) == 1
+    assert L.clr_destroy(None) == 1
+    assert L.clr_lbvh_build(None, None, None, 10, 10, None, None, None) == 1
+    assert L.clr_trace_closest(None, None, None, 1, None, None, None, 0, None, None, None,
+                               None) == 1
+
+
+def test_generators_are_deterministic():
+    a = ca.gen_cluster_soup(5000, seed=1)
+    b = ca.gen_cluster_soup(5000, seed=1)
+    c = ca.gen_cluster_soup(5000, seed=2)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    r1 = ca.gen_random_rays(1000, seed=2)
+    assert np.array_equal(r1, ca.gen_random_rays(1000, seed=2))
+    d = r1[3:6]
+    np.testing.assert_allclose((d * d).sum(0), 1.0, atol=1e-5)
+    assert np.isinf(r1[6]).all()
+    assert (r1[:3] >= 0).all() and (r1[:3] < 1).all()
+
+
+def test_cluster_soup_shape_statistics():
+    s = ca.gen_cluster_soup(200000, seed=1)
+    v = s.reshape(-1, 3, 3)
+    e1 = np.linalg.norm(v[:, 1] - v[:, 0], axis=1)
+    e2 = np.linalg.norm(v[:, 2] - v[:, 0], axis=1)
+    assert 1e-3 * 0.999 <= e1.min() and e1.max() <= 1e-2 * 1.001
+    assert 1e-3 * 0.999 <= e2.min() and e2.max() <= 1e-2 * 1.001
+    c = v.mean(axis=1)
+    # 64 clusters of sigma 0.02: the centroid cloud spans roughly the unit cube
+    assert c.min() > -0.2 and c.max() < 1.2

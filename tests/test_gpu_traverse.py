@@ -1,0 +1,68 @@
+"""GPU closest-hit parity: face IDs, hit distances and visit counts equal the
+CPU oracle's bit for bit (the oracle itself is pinned to brute force)."""
+import numpy as np
+import pytest
+
+import clear_amd as ca
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(ctx, o_inodes, o_leaves, n, v, f, rays, stats=True):
+    import torch
+
+    R = rays.shape[1]
+    ti = torch.from_numpy(np.ascontiguousarray(o_inodes).reshape(-1) if n > 1
+                          else np.zeros(48, np.uint8)).cuda()
+    tl = torch.from_numpy(np.ascontiguousarray(o_leaves).reshape(-1)).cuda()
+    tv = torch.from_numpy(v).cuda()
+    tf = torch.from_numpy(f.view(np.uint8)).cuda()
+    tr = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    hit = torch.empty(R, dtype=torch.int32, device="cuda")
+    t = torch.empty(R, dtype=torch.float32, device="cuda")
+    st = torch.empty(2 * R, dtype=torch.int32, device="cuda") if stats else None
+    ca.trace_closest(ctx, ti, tl, n, tv, tf, tr, R, hit, t, st)
+    torch.cuda.synchronize()
+    out = (hit.cpu().numpy().view(np.uint32), t.cpu().numpy())
+    if stats:
+        out += (st.cpu().numpy().view(np.uint32).reshape(R, 2),)
+    return out
+
+
+def check(ctx, soup, rays, bits=10):
+    n = soup.shape[0]
+    print(f"[trace parity] n={n} rays={rays.shape[1]} bits={bits}", flush=True)
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, bits)
+    h, t, st = orc.trace_closest(o["inodes"], o["leaves"], n, v, f, rays, stats=True)
+    gh, gt, gst = run_gpu(ctx, o["inodes"], o["leaves"], n, v, f, rays)
+    np.testing.assert_array_equal(gh, h)
+    np.testing.assert_array_equal(gt.view(np.uint32), t.view(np.uint32))
+    np.testing.assert_array_equal(gst, st)
+    return h
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1000])
+def test_small_scenes(gpu_ctx, n):
+    soup = ca.gen_cluster_soup(n, seed=n, num_clusters=4, sigma=0.1, e_min=0.02, e_max=0.1)
+    check(gpu_ctx, soup, ca.gen_random_rays(4096, seed=n + 1))
+
+
+def test_axis_aligned_and_finite_tmax(gpu_ctx):
+    soup = ca.gen_cluster_soup(20000, seed=3, num_clusters=8, sigma=0.05, e_min=0.005, e_max=0.03)
+    rays = ca.gen_random_rays(8192, seed=4)
+    rays[3:6, :2048] = 0.0
+    rays[3 + (np.arange(2048) % 3), np.arange(2048)] = np.where(np.arange(2048) % 2, 1.0, -1.0)
+    rays[6, 4096:] = np.float32(0.3)
+    h = check(gpu_ctx, soup, rays)
+    assert (h != 0xFFFFFFFF).sum() > 500
+
+
+def test_cluster_64k_tris_64k_rays(gpu_ctx):
+    check(gpu_ctx, ca.gen_cluster_soup(1 << 16, seed=1), ca.gen_random_rays(1 << 16, seed=2))
+
+
+def test_cluster_1m_tris_256k_rays(gpu_ctx):
+    h = check(gpu_ctx, ca.gen_cluster_soup(1 << 20, seed=1), ca.gen_random_rays(1 << 18, seed=2))
+    assert (h != 0xFFFFFFFF).mean() > 0.2

@@ -1,0 +1,71 @@
+"""Pins the CPU traversal oracle (oracle/traverse_oracle.cpp).
+
+The LBVH traversal must return exactly what a brute-force loop of the same
+triangle test over every face returns (its boxes only cull); closed-form
+single-triangle cases fix the triangle test itself.
+"""
+import numpy as np
+import pytest
+
+import clear_amd as ca
+import oracle as orc
+
+
+def scene(n, seed):
+    soup = ca.gen_cluster_soup(n, seed=seed, num_clusters=8, sigma=0.05, e_min=0.01, e_max=0.05)
+    v, f = ca.soup_to_mesh(soup)
+    return soup, v, f
+
+
+@pytest.mark.parametrize("n,seed", [(1, 3), (2, 4), (50, 5), (3000, 6)])
+def test_traversal_equals_bruteforce(n, seed):
+    soup, v, f = scene(n, seed)
+    o = orc.lbvh_build(v, f, 10)
+    rays = ca.gen_random_rays(2000, seed=seed + 100)
+    h, t = orc.trace_closest(o["inodes"], o["leaves"], n, v, f, rays)
+    hb, tb, ties = orc.trace_bruteforce(v, f, rays)
+    np.testing.assert_array_equal(t, tb)
+    untied = ties <= 1
+    np.testing.assert_array_equal(h[untied], hb[untied])
+    if n >= 3000:
+        assert (h != 0xFFFFFFFF).sum() > 100
+
+
+def test_triangle_closed_form():
+    # Unit right triangle in z = 2; rays along +z.
+    soup = np.float32([[0, 0, 2, 1, 0, 2, 0, 1, 2]])
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, 10)
+    rays = np.zeros((7, 4), np.float32)
+    rays[0] = [0.25, 0.9, 0.25, 0.25]
+    rays[1] = [0.25, 0.9, 0.25, 0.25]
+    rays[2] = [0, 0, 0, 3]
+    rays[5] = [1, 1, 1, 1]
+    rays[6] = np.inf
+    h, t = orc.trace_closest(o["inodes"], o["leaves"], 1, v, f, rays)
+    assert h.tolist() == [0, 0xFFFFFFFF, 0, 0xFFFFFFFF]
+    assert t[0] == 2.0 and t[2] == 2.0  # exact: (2 - 0) from dot/invDet with unit edges
+    # finite tmax below the hit distance: no hit, t stays at tmax
+    rays[6] = 1.5
+    h, t = orc.trace_closest(o["inodes"], o["leaves"], 1, v, f, rays)
+    assert (h == 0xFFFFFFFF).all() and (t == 1.5).all()
+
+
+def test_det_epsilon_culls_tiny_triangles():
+    # |det| = |e1 . (d x e2)| ~ 1e-7 for edges of 1e-3.5: below the 1e-6 cull.
+    s = 3e-4
+    soup = np.float32([[0, 0, 1, s, 0, 1, 0, s, 1]])
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, 10)
+    rays = np.zeros((7, 1), np.float32)
+    rays[0], rays[1], rays[5], rays[6] = s / 4, s / 4, 1, np.inf
+    h, _ = orc.trace_closest(o["inodes"], o["leaves"], 1, v, f, rays)
+    assert h[0] == 0xFFFFFFFF
+
+
+def test_stats_count_visits():
+    soup, v, f = scene(2000, 9)
+    o = orc.lbvh_build(v, f, 10)
+    rays = ca.gen_random_rays(500, seed=10)
+    h, t, st = orc.trace_closest(o["inodes"], o["leaves"], 2000, v, f, rays, stats=True)
+    assert (st[:, 0] >= 1).any() and (st[:, 1] >= (h != 0xFFFFFFFF)).all()
