@@ -53,6 +53,7 @@ def lib():
         "clr_error_string": (ctypes.c_char_p, [i32]),
         "clr_last_error": (ctypes.c_char_p, [vp]),
         "clr_scratch_bytes": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "clr_device_status": (i32, [vp, ctypes.POINTER(u32), i32]),
         "clr_set_profiling": (i32, [vp, i32]),
         "clr_lbvh_last_profile": (i32, [vp, ctypes.POINTER(f32), i32, ctypes.POINTER(i32)]),
         "clr_lbvh_build": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
@@ -127,6 +128,12 @@ class Context:
     def scratch_bytes(self) -> int:
         v = ctypes.c_uint64()
         self.check(lib().clr_scratch_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def device_status(self, clear: bool = True) -> int:
+        """Device-side failure flags since the last clear (0 = none); syncs."""
+        v = ctypes.c_uint32()
+        self.check(lib().clr_device_status(self._h, ctypes.byref(v), int(clear)))
         return v.value
 
     def set_profiling(self, on: bool) -> None:

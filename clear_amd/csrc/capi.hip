@@ -5,6 +5,7 @@
 // and returns a status; nothing here throws across the ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -44,6 +45,45 @@ int ensurePool(clr_ctx* ctx, size_t bytes) {
     }
     ctx->poolBytes = want;
     return CLR_OK;
+}
+
+int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut) {
+    if (nodes == 0) nodes = 1;
+    if (ctx->xchgNodes < nodes || ctx->epoch == 0xFFFFFFFFu) {
+        int rc = hipCheck(ctx, hipDeviceSynchronize(), "exchange regrow sync");
+        if (rc) return rc;
+        size_t want = std::max(nodes + nodes / 8, ctx->xchgNodes);
+        if (ctx->xchgNodes < nodes) {
+            if (ctx->xchg) (void)hipFree(ctx->xchg);
+            ctx->xchg = nullptr;
+            ctx->xchgNodes = 0;
+            hipError_t e = hipMalloc(&ctx->xchg, want * 6 * sizeof(uint64_t));
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->xchg = nullptr;
+                return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "hierarchy exchange buffer");
+            }
+            ctx->xchgNodes = want;
+        }
+        fillZero(reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12, nullptr);
+        rc = hipCheck(ctx, hipDeviceSynchronize(), "exchange clear");
+        if (rc) return rc;
+        ctx->epoch = 0;
+    }
+    *epochOut = ++ctx->epoch;
+    return CLR_OK;
+}
+
+int ensureStatus(clr_ctx* ctx) {
+    if (ctx->devStatus) return CLR_OK;
+    hipError_t e = hipMalloc(&ctx->devStatus, 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->devStatus = nullptr;
+        return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "device status word");
+    }
+    fillZero(ctx->devStatus, 64, nullptr);
+    return hipCheck(ctx, hipDeviceSynchronize(), "status clear");
 }
 
 // Sets the context's device for the scope of one call.
@@ -96,12 +136,12 @@ int clr_create(clr_ctx** out, int device) {
         delete ctx;
         return CLR_ERR_NO_DEVICE;
     }
-    for (auto& e : ctx->events) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete ctx;
-            return CLR_ERR_HIP;
-        }
-    }
+    for (auto& row : ctx->events)
+        for (auto& e : row)
+            if (hipEventCreate(&e) != hipSuccess) {
+                delete ctx;
+                return CLR_ERR_HIP;
+            }
     *out = ctx;
     return CLR_OK;
 }
@@ -112,8 +152,11 @@ int clr_destroy(clr_ctx* ctx) {
         DeviceScope ds(ctx->device);
         (void)hipDeviceSynchronize();
         if (ctx->pool) (void)hipFree(ctx->pool);
-        for (auto& e : ctx->events)
-            if (e) (void)hipEventDestroy(e);
+        if (ctx->xchg) (void)hipFree(ctx->xchg);
+        if (ctx->devStatus) (void)hipFree(ctx->devStatus);
+        for (auto& row : ctx->events)
+            for (auto& e : row)
+                if (e) (void)hipEventDestroy(e);
     }
     delete ctx;
     return CLR_OK;
@@ -127,27 +170,47 @@ int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes) {
     return CLR_OK;
 }
 
+int clr_device_status(clr_ctx* ctx, uint32_t* flags, int clear) {
+    if (!ctx || !flags) return CLR_ERR_INVALID_ARG;
+    *flags = 0;
+    if (!ctx->devStatus) return CLR_OK;
+    DeviceScope ds(ctx->device);
+    int rc = hipCheck(ctx, hipDeviceSynchronize(), "status sync");
+    if (rc) return rc;
+    rc = hipCheck(ctx, hipMemcpy(flags, ctx->devStatus, 4, hipMemcpyDeviceToHost), "status read");
+    if (rc) return rc;
+    if (clear) {
+        fillZero(ctx->devStatus, 1, nullptr);
+        rc = hipCheck(ctx, hipDeviceSynchronize(), "status clear");
+    }
+    return rc;
+}
+
 int clr_set_profiling(clr_ctx* ctx, int enable) {
     if (!ctx) return CLR_ERR_INVALID_ARG;
     ctx->profiling = enable != 0;
-    ctx->haveProfile = false;
+    ctx->profiledBuilds = 0;
     return CLR_OK;
 }
 
 int clr_lbvh_last_profile(clr_ctx* ctx, float* ms, int maxStages, int* numStages) {
     if (!ctx || !ms || !numStages || maxStages < 0) return CLR_ERR_INVALID_ARG;
-    if (!ctx->haveProfile) return fail(ctx, CLR_ERR_INVALID_ARG, "no profiled build recorded");
+    const int builds = ctx->profiledBuilds;
+    if (builds == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "no profiled build recorded");
     DeviceScope ds(ctx->device);
-    int rc = hipCheck(ctx, hipEventSynchronize(ctx->events[ctx->numEvents - 1]), "profile sync");
-    if (rc) return rc;
-    int stages = ctx->numEvents - 1;
+    const int stages = clr_ctx::kStageEvents - 1;
     *numStages = stages;
-    for (int k = 0; k < stages && k < maxStages; ++k) {
-        float v = 0.0f;
-        rc = hipCheck(ctx, hipEventElapsedTime(&v, ctx->events[k], ctx->events[k + 1]),
-                      "profile elapsed");
+    for (int k = 0; k < stages && k < maxStages; ++k) ms[k] = 0.0f;
+    for (int b = 0; b < builds; ++b) {
+        int rc = hipCheck(ctx, hipEventSynchronize(ctx->events[b][stages]), "profile sync");
         if (rc) return rc;
-        ms[k] = v;
+        for (int k = 0; k < stages && k < maxStages; ++k) {
+            float v = 0.0f;
+            rc = hipCheck(ctx, hipEventElapsedTime(&v, ctx->events[b][k], ctx->events[b][k + 1]),
+                          "profile elapsed");
+            if (rc) return rc;
+            ms[k] += v / (float)builds;
+        }
     }
     return CLR_OK;
 }

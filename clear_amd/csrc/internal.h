@@ -15,12 +15,23 @@ struct clr_ctx {
     // m_bufferGenericPool, LBVHBuilder.cpp:130).
     void* pool = nullptr;
     size_t poolBytes = 0;
-    // Profiling of the LBVH build stages.
+    // Profiling of the LBVH build stages: one event per stage boundary for up
+    // to kMaxProfiledBuilds builds since profiling was (re)enabled.
+    static constexpr int kMaxProfiledBuilds = 64;
+    static constexpr int kStageEvents = 5;
     bool profiling = false;
-    hipEvent_t events[8] = {};
-    int numEvents = 0;
-    bool haveProfile = false;
-    hipStream_t profStream = nullptr;
+    hipEvent_t events[kMaxProfiledBuilds][kStageEvents] = {};
+    int profiledBuilds = 0;
+    // Box hand-off buffer of the bottom-up hierarchy pass: six 64-bit words
+    // (float bits | build epoch) per internal node. Only ever written with
+    // epoch-tagged words, zeroed when (re)allocated, so a stale word can never
+    // carry the current epoch.
+    uint64_t* xchg = nullptr;
+    size_t xchgNodes = 0;
+    uint32_t epoch = 0;
+    // Device-side failure flags (bounded spin exceeded, stack overflow):
+    // one 32-bit word, read back by clr_device_status.
+    uint32_t* devStatus = nullptr;
 };
 
 namespace clr {
@@ -34,6 +45,19 @@ int hipCheck(clr_ctx* ctx, hipError_t e, const char* what);
 // Makes sure the pool holds at least `bytes`; synchronises the device when it
 // has to regrow.
 int ensurePool(clr_ctx* ctx, size_t bytes);
+
+// Bits of the device status word.
+constexpr uint32_t kStatusLookbackSpin = 1u << 0;
+constexpr uint32_t kStatusExchangeSpin = 1u << 1;
+constexpr uint32_t kStatusStackOverflow = 1u << 2;
+// Upper bound on any device-side wait loop (iterations of one atomic load).
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// Grows the box hand-off buffer to `nodes` entries (synchronous when it has
+// to reallocate) and returns the epoch for the next build.
+int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut);
+// Lazily allocates the device status word.
+int ensureStatus(clr_ctx* ctx);
 
 // Bump allocator over the pool (256-B aligned sub-buffers).
 struct Carve {
@@ -58,6 +82,8 @@ struct LbvhInput {
 size_t lbvhScratchBytes(uint32_t n, uint32_t bitsPerDim);
 int lbvhBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bitsPerDim,
               clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream);
+// Fills `count` 32-bit words with zero on `stream` (kernel, not hipMemset).
+void fillZero(uint32_t* words, size_t count, hipStream_t stream);
 size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bitsPerDim);
 int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegOffsets,
                        uint32_t numSegments, uint32_t bitsPerDim, clr_lbvh_inode* inodes,

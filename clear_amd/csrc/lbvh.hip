@@ -88,28 +88,42 @@ __device__ __forceinline__ Box boxOf3(const float* p0, const float* p1, const fl
     return b;
 }
 
-// Primitive source: a 36-B triangle soup or a precomputed [n][6] box array.
+// Primitive source: a 36-B triangle soup, plus a dense box array in input
+// order (32-B records: lo.xyz, hi.x | hi.yz, 0, 0) written by the first pass
+// that sees the triangles. The hierarchy pass gathers boxes by sorted index
+// from that array: one aligned 32-B sector per leaf.
 struct PrimSrc {
-    const float* tris;   // soup
-    const float* boxes;  // box array (drop-in path, written by the bounds kernel)
+    const float* tris;  // soup (null on the drop-in path)
+    float4* boxes;      // box array
 };
+
+__device__ __forceinline__ Box soupBox(const float* tris, uint32_t i) {
+    const float* t = tris + 9 * (size_t)i;
+    float p0[3] = {t[0], t[1], t[2]};
+    float p1[3] = {t[3], t[4], t[5]};
+    float p2[3] = {t[6], t[7], t[8]};
+    return boxOf3(p0, p1, p2);
+}
+
+__device__ __forceinline__ void storeBoxRec(float4* boxes, uint32_t i, const Box& b) {
+    boxes[2 * (size_t)i] = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
+    boxes[2 * (size_t)i + 1] = make_float4(b.hi[1], b.hi[2], 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ Box loadBoxRec(const float4* boxes, uint32_t i) {
+    const float4 a = boxes[2 * (size_t)i], c = boxes[2 * (size_t)i + 1];
+    Box r;
+    r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z;
+    r.hi[0] = a.w; r.hi[1] = c.x; r.hi[2] = c.y;
+    return r;
+}
 
 template <bool SOUP>
 __device__ __forceinline__ Box loadBox(const PrimSrc& s, uint32_t i) {
-    if constexpr (SOUP) {
-        const float* t = s.tris + 9 * (size_t)i;
-        float p0[3] = {t[0], t[1], t[2]};
-        float p1[3] = {t[3], t[4], t[5]};
-        float p2[3] = {t[6], t[7], t[8]};
-        return boxOf3(p0, p1, p2);
-    } else {
-        const float2* b = reinterpret_cast<const float2*>(s.boxes + 6 * (size_t)i);
-        float2 a = b[0], c = b[1], d = b[2];
-        Box r;
-        r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = c.x;
-        r.hi[0] = c.y; r.hi[1] = d.x; r.hi[2] = d.y;
-        return r;
-    }
+    if constexpr (SOUP)
+        return soupBox(s.tris, i);
+    else
+        return loadBoxRec(s.boxes, i);
 }
 
 __device__ __forceinline__ float waveMin(float v) {
@@ -129,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_init(uint32_t* words, size_t count,
     size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     size_t stride = (size_t)gridDim.x * kBlock;
     for (; i < count; i += stride) words[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x < 6)
+    if (sceneOrd && blockIdx.x == 0 && threadIdx.x < 6)
         sceneOrd[threadIdx.x] = threadIdx.x < 3 ? ordOf(INFINITY) : ordOf(-INFINITY);
 }
 
@@ -139,15 +153,14 @@ template <bool SOUP>
 __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris,
                                                    const clr_float3* __restrict__ verts,
                                                    const clr_face* __restrict__ faces, uint32_t n,
-                                                   float* __restrict__ boxesOut,
+                                                   float4* __restrict__ boxesOut,
                                                    uint32_t* __restrict__ sceneOrd) {
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         Box b;
         if constexpr (SOUP) {
-            PrimSrc s{tris, nullptr};
-            b = loadBox<true>(s, i);
+            b = soupBox(tris, i);
         } else {
             const uint32_t* f = reinterpret_cast<const uint32_t*>(faces + i);
             float4 v0 = *reinterpret_cast<const float4*>(verts + f[0]);
@@ -156,10 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
             float p0[3] = {v0.x, v0.y, v0.z}, p1[3] = {v1.x, v1.y, v1.z},
                   p2[3] = {v2.x, v2.y, v2.z};
             b = boxOf3(p0, p1, p2);
-            float2* o = reinterpret_cast<float2*>(boxesOut + 6 * (size_t)i);
-            o[0] = make_float2(b.lo[0], b.lo[1]);
-            o[1] = make_float2(b.lo[2], b.hi[0]);
-            o[2] = make_float2(b.hi[1], b.hi[2]);
+            storeBoxRec(boxesOut, i, b);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -244,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint
     const float fDivs = (float)numDivs;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         Box b = loadBox<SOUP>(src, i);
+        if constexpr (SOUP) storeBoxRec(src.boxes, i, b);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -280,6 +291,18 @@ __device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* 
     return pre + x - v;
 }
 
+__device__ __forceinline__ void writeLeaf(clr_lbvh_leaf* leaves, uint32_t i, const Box& b,
+                                          uint32_t obj) {
+    uint4* o = reinterpret_cast<uint4*>(leaves + i);
+    o[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]),
+                      __float_as_uint(b.lo[2]), 0u);
+    o[1] = make_uint4(__float_as_uint(b.hi[0]), __float_as_uint(b.hi[1]),
+                      __float_as_uint(b.hi[2]), 0u);
+    o[2] = make_uint4(obj, 0u, 0u, 0u);
+}
+
+// One stable LSD pass on the 8-bit digit at `shift`. FIRST generates the
+// payload (original index) instead of reading it.
 template <typename KeyT, bool FIRST>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict__ keysIn,
                                                          const uint32_t* __restrict__ valsIn,
@@ -288,7 +311,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
                                                          uint32_t n, int shift,
                                                          const uint32_t* __restrict__ passHist,
                                                          uint32_t* __restrict__ status,
-                                                         uint32_t* __restrict__ partCounter) {
+                                                         uint32_t* __restrict__ partCounter,
+                                                         uint32_t* __restrict__ devStatus) {
     constexpr int kItems = SortCfg<KeyT>::kItems;
     constexpr int kTile = kSortBlock * kItems;
     __shared__ uint32_t s_part;
@@ -368,12 +392,19 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
     } else {
         __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagAgg | tileCount,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Earlier partitions took their ids before this one, so they are
+        // resident and will publish; the bound only guards against a bug.
         int64_t look = (int64_t)part - 1;
+        uint32_t spins = 0;
         while (true) {
             uint32_t v = __hip_atomic_load(status + (size_t)look * 256 + tid, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
             uint32_t flag = v & kFlagMask;
-            if (flag == 0u) continue;
+            if (flag == 0u) {
+                if (++spins < kSpinLimit) continue;
+                atomicOr(devStatus, kStatusLookbackSpin);
+                break;
+            }
             exclusive += v & kCountMask;
             if (flag == kFlagInc) break;
             --look;
@@ -426,16 +457,6 @@ __device__ __forceinline__ int deltaAdj(const KeyT* __restrict__ k, uint32_t x, 
     return keyBits + __clz((int)(x ^ (x + 1u)));
 }
 
-__device__ __forceinline__ void writeLeaf(clr_lbvh_leaf* leaves, uint32_t i, const Box& b,
-                                          uint32_t obj) {
-    uint4* o = reinterpret_cast<uint4*>(leaves + i);
-    o[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]),
-                      __float_as_uint(b.lo[2]), 0u);
-    o[1] = make_uint4(__float_as_uint(b.hi[0]), __float_as_uint(b.hi[1]),
-                      __float_as_uint(b.hi[2]), 0u);
-    o[2] = make_uint4(obj, 0u, 0u, 0u);
-}
-
 __device__ __forceinline__ void writeNode(clr_lbvh_inode* nodes, uint32_t idx, const Box& b,
                                           uint32_t split, uint32_t l, uint32_t r) {
     uint4* o = reinterpret_cast<uint4*>(nodes + idx);
@@ -447,63 +468,285 @@ __device__ __forceinline__ void writeNode(clr_lbvh_inode* nodes, uint32_t idx, c
     o[2] = make_uint4(flags, split, split + 1u, 0u);
 }
 
-__device__ __forceinline__ Box readBox(const void* rec) {
-    const float4* p = reinterpret_cast<const float4*>(rec);
-    float4 a = p[0], b = p[1];
+__device__ __forceinline__ Box leafBox(const clr_lbvh_leaf* leaves, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(leaves + i);
+    const float4 a = p[0], b = p[1];
     Box r;
     r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z;
     r.hi[0] = b.x; r.hi[1] = b.y; r.hi[2] = b.z;
     return r;
 }
 
-template <bool SOUP, typename KeyT>
-__global__ __launch_bounds__(kBlock) void k_hierarchy(PrimSrc src, const KeyT* __restrict__ skeys,
-                                                      const uint32_t* __restrict__ sidx,
-                                                      uint32_t n, int keyBits,
-                                                      uint32_t* __restrict__ otherEnd,
-                                                      clr_lbvh_inode* __restrict__ inodes,
-                                                      clr_lbvh_leaf* __restrict__ leaves) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t obj = sidx[i];
-    Box box = loadBox<SOUP>(src, obj);
-    writeLeaf(leaves, i, box, obj);
-    if (n == 1) return;
+// Box hand-off between the two children of a node. Every internal node
+// publishes its box before it arrives at its parent, as six 64-bit words
+// (float bits | epoch) written with relaxed agent-scope atomics (plain sc1
+// stores on gfx950; an acq_rel exchange would instead write back and
+// invalidate the whole per-XCD L2 at every step). The second arrival polls the
+// sibling's words until they carry this build's epoch: the sibling issued its
+// stores before its own arrival, which this thread has already observed, so
+// the wait ends; kSpinLimit only guards against a bug.
+__device__ __forceinline__ void publishBox(uint64_t* slot, const Box& b, uint32_t epoch) {
+    const uint64_t tag = (uint64_t)epoch << 32;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        __hip_atomic_store(slot + k, tag | __float_as_uint(b.lo[k]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(slot + 3 + k, tag | __float_as_uint(b.hi[k]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
 
-    uint32_t l = i, r = i, split = 0;
-    bool internal = false;
+__device__ __forceinline__ Box awaitBox(const uint64_t* slot, uint32_t epoch,
+                                        uint32_t* devStatus) {
+    uint64_t w[6];
+    // All six words in flight at once; re-poll only while some are stale.
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        w[k] = __hip_atomic_load(slot + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (true) {
+        bool fresh = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fresh &= (uint32_t)(w[k] >> 32) == epoch;
+        if (fresh) break;
+        if (++spins >= kSpinLimit) {
+            atomicOr(devStatus, kStatusExchangeSpin);
+            break;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if ((uint32_t)(w[k] >> 32) != epoch)
+                w[k] = __hip_atomic_load(slot + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    Box b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = __uint_as_float((uint32_t)w[k]);
+        b.hi[k] = __uint_as_float((uint32_t)w[3 + k]);
+    }
+    return b;
+}
+
+// Bottom-up topology + refit in two phases.
+//
+// Phase 1 (k_hier_local): block b owns leaves [bs, be) (kLocalLeaves per
+// block) and completes every node whose range lies inside them: arrival slots
+// for splits bs..be-2, leaf boxes, internal boxes and boundary deltas all live
+// in LDS, and the hand-off is a workgroup-scope exchange (s_waitcnt around
+// ds_wrxchg, no cache maintenance). A climb whose parent splits at a block
+// boundary (bs-1 or be-1) stops and leaves a pending record (at most two per
+// block). After a barrier the block exports its half-filled slots (one child
+// arrived) to the global arrival words, publishing that child's box.
+// Phase 2 (k_hier_global): one thread per pending record continues with the
+// global epoch-tagged protocol. Only nodes whose range crosses a block
+// boundary (a small fraction) are finished there.
+constexpr int kLocalLeaves = 1024;
+constexpr uint32_t kDone = 0xFFFFFFFEu;
+
+struct Pending {
+    uint32_t valid, l, r, split, internal;
+    int dl, dr;
+    float box[6];
+    uint32_t pad[3];
+};
+static_assert(sizeof(Pending) == 64, "pending record layout");
+
+template <bool SOUP, typename KeyT>
+__global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
+    PrimSrc src, const KeyT* __restrict__ skeys, const uint32_t* __restrict__ sidx, uint32_t n,
+    int keyBits, uint32_t* __restrict__ otherEnd, uint64_t* __restrict__ xchg, uint32_t epoch,
+    clr_lbvh_inode* __restrict__ inodes, clr_lbvh_leaf* __restrict__ leaves,
+    Pending* __restrict__ pend) {
+    constexpr int B = kLocalLeaves;
+    __shared__ uint32_t s_other[B];
+    __shared__ float s_lbox[6][B];
+    __shared__ float s_ibox[6][B];
+    __shared__ int s_delta[B + 1];  // delta(x) for x = bs-1 .. be-1 at [x - bs + 1]
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t bs = blockIdx.x * (uint32_t)B;
+    const uint32_t be = min(bs + (uint32_t)B, n);
+    const uint32_t i = bs + t;
+    s_other[t] = kEmpty;
+    if (i < be) {
+        // Leaf i: gather the sorted primitive, write its LeafNode record.
+        const uint32_t obj = sidx[i];
+        const Box b = loadBoxRec(src.boxes, obj);
+        writeLeaf(leaves, i, b, obj);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_lbox[k][t] = b.lo[k];
+            s_lbox[3 + k][t] = b.hi[k];
+        }
+    }
+    for (uint32_t j = t; j <= (uint32_t)B; j += B) {
+        const int64_t x = (int64_t)bs - 1 + j;
+        s_delta[j] = (x >= 0 && x + 1 < (int64_t)n && x < (int64_t)be)
+                         ? deltaAdj<KeyT>(skeys, (uint32_t)x, keyBits)
+                         : -1;
+    }
+    if (t < 2) pend[2 * blockIdx.x + t].valid = 0u;
+    __syncthreads();
+
+    // delta(x, x+1) for x in [bs-1, be-1]; x = bs-1 may be -1 (then -1).
+    auto D = [&](int64_t x) -> int { return s_delta[x - (int64_t)bs + 1]; };
+    if (i < be && n > 1) {
+        Box box;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            box.lo[k] = s_lbox[k][t];
+            box.hi[k] = s_lbox[3 + k][t];
+        }
+        uint32_t l = i, r = i, split = 0;
+        int dl = D((int64_t)l - 1), dr = D(r);  // delta(l-1, l), delta(r, r+1)
+        bool internal = false;
+        while (true) {
+            if (l == 0 && r == n - 1) {
+                writeNode(inodes, 0, box, split, l, r);
+                break;
+            }
+            const bool isLeft = dr > dl;
+            const uint32_t parentSplit = isLeft ? r : l - 1;
+            if (internal) {
+                const uint32_t self = isLeft ? r : l;
+                writeNode(inodes, self, box, split, l, r);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    s_ibox[k][self - bs] = box.lo[k];
+                    s_ibox[3 + k][self - bs] = box.hi[k];
+                }
+            }
+            if (parentSplit < bs || parentSplit + 1 >= be) {
+                // Split on a block boundary: hand the climb to phase 2.
+                Pending& P = pend[2 * blockIdx.x + (isLeft ? 1 : 0)];
+                P.l = l; P.r = r; P.split = split; P.internal = internal ? 1u : 0u;
+                P.dl = dl; P.dr = dr;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    P.box[k] = box.lo[k];
+                    P.box[3 + k] = box.hi[k];
+                }
+                P.valid = 1u;
+                break;
+            }
+            const uint32_t other =
+                __hip_atomic_exchange(&s_other[parentSplit - bs], isLeft ? l : r,
+                                      __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (other == kEmpty) break;
+            s_other[parentSplit - bs] = kDone;  // both children arrived
+            const uint32_t sIdx = (isLeft ? parentSplit + 1u : parentSplit) - bs;
+            const bool sLeaf = (sIdx + bs) == other;
+            Box sb;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                sb.lo[k] = sLeaf ? s_lbox[k][sIdx] : s_ibox[k][sIdx];
+                sb.hi[k] = sLeaf ? s_lbox[3 + k][sIdx] : s_ibox[3 + k][sIdx];
+            }
+            if (isLeft) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    box.lo[k] = omin(box.lo[k], sb.lo[k]);
+                    box.hi[k] = omax(box.hi[k], sb.hi[k]);
+                }
+                r = other;
+                dr = D(other);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    box.lo[k] = omin(sb.lo[k], box.lo[k]);
+                    box.hi[k] = omax(sb.hi[k], box.hi[k]);
+                }
+                l = other;
+                dl = D((int64_t)other - 1);
+            }
+            split = parentSplit;
+            internal = true;
+        }
+    }
+    __syncthreads();
+    // Export slots where exactly one child arrived: phase 2 brings the other.
+    if (bs + t + 1 < be) {
+        const uint32_t v = s_other[t];
+        if (v != kEmpty && v != kDone) {
+            const uint32_t ps = bs + t;
+            otherEnd[ps] = v;
+            const bool leftArrived = v <= ps;
+            const uint32_t idx = leftArrived ? ps : ps + 1u;
+            const bool isLeaf = leftArrived ? (v == ps) : (v == ps + 1u);
+            if (!isLeaf) {
+                const uint64_t tag = (uint64_t)epoch << 32;
+                uint64_t* slot = xchg + 6 * (size_t)idx;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    slot[k] = tag | __float_as_uint(s_ibox[k][idx - bs]);
+            }
+        }
+    }
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_hier_global(
+    const KeyT* __restrict__ skeys, uint32_t n, int keyBits, uint32_t* __restrict__ otherEnd,
+    uint64_t* __restrict__ xchg, uint32_t epoch, uint32_t* __restrict__ devStatus,
+    clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
+    const Pending* __restrict__ pend, uint32_t numPending) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= numPending) return;
+    const Pending P = pend[p];
+    if (!P.valid) return;
+    uint32_t l = P.l, r = P.r, split = P.split;
+    bool internal = P.internal != 0u;
+    int dl = P.dl, dr = P.dr;
+    Box box;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        box.lo[k] = P.box[k];
+        box.hi[k] = P.box[3 + k];
+    }
+    bool written = true;  // phase 1 already wrote this node's record
     while (true) {
         if (l == 0 && r == n - 1) {
             writeNode(inodes, 0, box, split, l, r);
             return;
         }
-        const int dl = l > 0 ? deltaAdj<KeyT>(skeys, l - 1, keyBits) : -1;
-        const int dr = r < n - 1 ? deltaAdj<KeyT>(skeys, r, keyBits) : -1;
         const bool isLeft = dr > dl;
         const uint32_t parentSplit = isLeft ? r : l - 1;
-        if (internal) writeNode(inodes, isLeft ? r : l, box, split, l, r);
-        const uint32_t other =
-            __hip_atomic_exchange(otherEnd + parentSplit, isLeft ? l : r, __ATOMIC_ACQ_REL,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+        if (internal) {
+            const uint32_t self = isLeft ? r : l;
+            if (!written) writeNode(inodes, self, box, split, l, r);
+            publishBox(xchg + 6 * (size_t)self, box, epoch);
+        }
+        written = false;
+        // Keep the publish ahead of the arrival in program order: a sibling in
+        // the same wave may be the one that waits for it.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const uint32_t other = __hip_atomic_exchange(otherEnd + parentSplit, isLeft ? l : r,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (other == kEmpty) return;  // first arrival: the sibling finishes the parent
+        const uint32_t s = isLeft ? parentSplit + 1u : parentSplit;
+        int dNew;
+        if (isLeft)
+            dNew = other < n - 1 ? deltaAdj<KeyT>(skeys, other, keyBits) : -1;
+        else
+            dNew = other > 0 ? deltaAdj<KeyT>(skeys, other - 1, keyBits) : -1;
+        const Box sb = (s == other) ? leafBox(leaves, s)
+                                    : awaitBox(xchg + 6 * (size_t)s, epoch, devStatus);
         if (isLeft) {
-            const uint32_t s = parentSplit + 1u;
-            const Box sb = (s == other) ? readBox(leaves + s) : readBox(inodes + s);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 box.lo[k] = omin(box.lo[k], sb.lo[k]);
                 box.hi[k] = omax(box.hi[k], sb.hi[k]);
             }
             r = other;
+            dr = dNew;
         } else {
-            const uint32_t s = parentSplit;
-            const Box sb = (s == other) ? readBox(leaves + s) : readBox(inodes + s);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 box.lo[k] = omin(sb.lo[k], box.lo[k]);
                 box.hi[k] = omax(sb.hi[k], box.hi[k]);
             }
             l = other;
+            dl = dNew;
         }
         split = parentSplit;
         internal = true;
@@ -516,12 +759,13 @@ struct Layout {
     uint32_t* partCounters;
     uint32_t* status;
     size_t clearWords;  // hist + partCounters + status are contiguous
-    float* boxes;
+    float4* boxes;
     void* keysA;
     void* keysB;
     uint32_t* valsA;
     uint32_t* valsB;
     uint32_t* otherEnd;
+    Pending* pend;
     size_t bytes;
 };
 
@@ -543,12 +787,13 @@ Layout layoutFor(void* base, uint32_t n, uint32_t bits) {
     L.partCounters = L.hist + (size_t)kMaxPasses * 256;
     L.status = L.partCounters + 64;
     L.clearWords = (size_t)kMaxPasses * 256 + 64 + (size_t)P * parts * 256;
-    L.boxes = c.take<float>(6 * (size_t)n);
+    L.boxes = c.take<float4>(2 * (size_t)n);
     L.keysA = c.take<char>(keyBytes * n);
     L.keysB = c.take<char>(keyBytes * n);
     L.valsA = c.take<uint32_t>(n);
     L.valsB = c.take<uint32_t>(n);
     L.otherEnd = c.take<uint32_t>(n);
+    L.pend = c.take<Pending>(2 * (size_t)((n + kLocalLeaves - 1) / kLocalLeaves));
     L.bytes = c.off + 256;
     return L;
 }
@@ -561,13 +806,19 @@ inline unsigned gridFor(uint32_t n, unsigned cap) {
 template <typename KeyT, bool SOUP>
 int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
              clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
+    uint32_t epoch = 0;
+    int rc = ensureStatus(ctx);
+    if (rc) return rc;
+    rc = ensureExchange(ctx, n, &epoch);
+    if (rc) return rc;
     Layout L = layoutFor(ctx->pool, n, bits);
     const int P = numPassesFor(bits);
     const uint32_t tile = tileFor(bits);
     const uint32_t parts = (n + tile - 1) / tile;
     const int keyBits = (int)(3 * bits);
+    const bool prof = ctx->profiling && ctx->profiledBuilds < clr_ctx::kMaxProfiledBuilds;
     auto ev = [&](int k) {
-        if (ctx->profiling) (void)hipEventRecord(ctx->events[k], stream);
+        if (prof) (void)hipEventRecord(ctx->events[ctx->profiledBuilds][k], stream);
     };
     ev(0);
     k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0,
@@ -587,26 +838,33 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
         uint32_t* st = L.status + (size_t)p * parts * 256;
         if (p == 0)
             k_onesweep<KeyT, true><<<parts, kSortBlock, 0, stream>>>(
-                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p);
+                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p,
+                ctx->devStatus);
         else
             k_onesweep<KeyT, false><<<parts, kSortBlock, 0, stream>>>(
-                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p);
+                kin, vin, kout, vout, n, 8 * p, L.hist + 256 * p, st, L.partCounters + p,
+                ctx->devStatus);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     ev(3);
-    k_hierarchy<SOUP, KeyT><<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, inodes, leaves);
+    const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
+    k_hier_local<SOUP, KeyT><<<localBlocks, kLocalLeaves, 0, stream>>>(
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend);
+    k_hier_global<KeyT><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend,
+        2 * localBlocks);
     ev(4);
-    if (ctx->profiling) {
-        ctx->numEvents = 5;
-        ctx->haveProfile = true;
-        ctx->profStream = stream;
-    }
+    if (prof) ++ctx->profiledBuilds;
     return hipCheck(ctx, hipGetLastError(), "LBVH build launch");
 }
 
 }  // namespace
+
+void fillZero(uint32_t* words, size_t count, hipStream_t stream) {
+    k_init<<<gridFor((uint32_t)std::min<size_t>(count, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
+        words, count, nullptr);
+}
 
 size_t lbvhScratchBytes(uint32_t n, uint32_t bitsPerDim) {
     return layoutFor(nullptr, n, bitsPerDim).bytes;

@@ -92,6 +92,12 @@ const char* clr_error_string(int status);
 const char* clr_last_error(const clr_ctx* ctx);
 /* Bytes currently held by the context's scratch pool. */
 int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes);
+/* Device-side failure flags raised by kernels since the last clear: bit 0 a
+ * radix-sort look-back wait, bit 1 a hierarchy hand-off wait exceeded its
+ * bound (the result is then wrong), bit 2 a traversal stack overflowed.
+ * Synchronises the device. No reference counterpart: the reference has no
+ * failure detection (its atomic refit has no bound, lbvh_construction.cl:465). */
+int clr_device_status(clr_ctx* ctx, uint32_t* flags, int clear);
 
 /* ---- LBVH build ---------------------------------------------------------- */
 
@@ -133,9 +139,13 @@ int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* se
                              clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes,
                              void* stream);
 
-/* Per-stage GPU times (ms) of the most recent build when profiling was
- * enabled with clr_set_profiling(ctx, 1): [bounds, morton, histogram,
- * sort, hierarchy]. Synchronises the stream of that build. */
+/* Stage profiling of LBVH builds. clr_set_profiling(ctx, 1) starts recording
+ * HIP events at the stage boundaries of every following build (up to 64
+ * builds, on the build's own stream); clr_lbvh_last_profile returns the mean
+ * per-stage GPU time in ms over the recorded builds, stages =
+ * [bounds, morton+histograms, radix sort, hierarchy]. It waits for the
+ * recorded builds to finish. (Replaces the profiling prints of perform,
+ * LBVHBuilder.cpp:173-349.) */
 int clr_set_profiling(clr_ctx* ctx, int enable);
 int clr_lbvh_last_profile(clr_ctx* ctx, float* msPerStage, int maxStages, int* numStages);
 

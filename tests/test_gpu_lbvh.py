@@ -44,6 +44,7 @@ def assert_same_as_oracle(ctx, soup, bits, mesh=False):
     np.testing.assert_array_equal(gl, o["leaves"], err_msg=f"leaves n={n} bits={bits}")
     if n > 1:
         np.testing.assert_array_equal(gi, o["inodes"], err_msg=f"inodes n={n} bits={bits}")
+    assert ctx.device_status() == 0
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 63, 64, 65, 4095, 4096, 4097, 12289, 65537])
@@ -125,6 +126,7 @@ def test_full_size_16m_properties(gpu_ctx):
     assert (obj[1:][eq] > obj[:-1][eq]).all()
     np.testing.assert_array_equal(lv["min"], lo[obj])
     np.testing.assert_array_equal(lv["max"], hi[obj])
+    assert gpu_ctx.device_status() == 0
     ino = ca.decode_inodes(gi, n)
     c, leaf = ino["c"].astype(np.int64), ino["isChild"].astype(bool)
     cmin = np.where(leaf[:, :, None], lv["min"][np.minimum(c, n - 1)], ino["min"][np.minimum(c, n - 2)])
