@@ -63,6 +63,13 @@ def lib():
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_gen_cluster_soup": (i32, [u32, i32, u32, f32, f32, f32, vp]),
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
+        "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
+        "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
+        "clr_scene_build": (i32, [i32, vp, ctypes.POINTER(vp)]),
+        "clr_scene_free": (i32, [vp]),
+        "clr_scene_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                 ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "clr_scene_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -256,3 +263,112 @@ def decode_leaves(buf: np.ndarray, n: int):
     f = rec[:, :32].copy().view(np.float32).reshape(-1, 8)
     return {"min": f[:, 0:3], "max": f[:, 4:7],
             "objIdx": rec[:, 32:36].copy().view(np.uint32).reshape(-1)}
+
+
+# ---- scenes and rendering -------------------------------------------------
+
+SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE = 0, 1, 2
+ARRAY_NAMES = ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials",
+               "textures", "others", "bvh_nodes"]
+RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1
+
+
+class SceneParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("detail", ctypes.c_uint32), ("seed", ctypes.c_int),
+                ("lensRadius", ctypes.c_float), ("focusDistance", ctypes.c_float),
+                ("buildCpuBvh", ctypes.c_uint32), ("albedo", ctypes.c_float),
+                ("fovY", ctypes.c_float)]
+
+
+class SceneDev(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materialsData",
+                 "texturesData", "otherResources", "internalNodes", "leafNodes"]] + \
+               [("numFaces", ctypes.c_uint32), ("numLights", ctypes.c_uint32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("samplesPerPixel", ctypes.c_uint32), ("maxDepth", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+class HostScene:
+    """A procedural scene built by the library's host Scene API; `arrays` holds
+    numpy copies of the reference's arrays and blobs (empty attribute arrays
+    become 1-element dummies, as the reference's host does)."""
+
+    def __init__(self, kind: int, width: int = 0, height: int = 0, detail: int = 0, seed: int = 0,
+                 lens_radius: float = 0.0, focus_distance: float = 0.0, cpu_bvh: bool = False,
+                 albedo: float = 0.5, fov_y: float = 0.0):
+        p = SceneParams(width, height, detail, seed, lens_radius, focus_distance, int(cpu_bvh),
+                        albedo, fov_y)
+        h = ctypes.c_void_p()
+        rc = lib().clr_scene_build(kind, ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise ClrError(rc, f"clr_scene_build({kind})")
+        try:
+            nf, nl, w, hh = (ctypes.c_uint32() for _ in range(4))
+            lib().clr_scene_info(h, ctypes.byref(nf), ctypes.byref(nl), ctypes.byref(w), ctypes.byref(hh))
+            self.num_faces, self.num_lights = nf.value, nl.value
+            self.width, self.height = w.value, hh.value
+            self.arrays = {}
+            for i, name in enumerate(ARRAY_NAMES):
+                ptr, nbytes = ctypes.c_void_p(), ctypes.c_uint64()
+                lib().clr_scene_array(h, i, ctypes.byref(ptr), ctypes.byref(nbytes))
+                n = nbytes.value
+                buf = np.zeros(max(n, 16), dtype=np.uint8)
+                if n:
+                    ctypes.memmove(buf.ctypes.data, ptr.value, n)
+                self.arrays[name] = buf
+        finally:
+            lib().clr_scene_free(h)
+        self.faces = self.arrays["faces"][: self.num_faces * 56].view(FACE_DTYPE)
+        self.vertices = self.arrays["vertices"].view(np.float32).reshape(-1, 4)
+
+    def to_device(self, device="cuda"):
+        import torch
+
+        return {k: torch.from_numpy(v).to(device) for k, v in self.arrays.items()}
+
+
+class DeviceScene:
+    """Scene arrays resident on the GPU plus its LBVH (built by clr_lbvh_build)."""
+
+    def __init__(self, ctx: Context, host: HostScene, bits: int = 10, device="cuda", stream=None):
+        import torch
+
+        self.ctx, self.host = ctx, host
+        self.t = host.to_device(device)
+        n = host.num_faces
+        self.inodes, self.leaves = node_buffers(n, device)
+        LBVHBuilder(ctx, n, bits).perform(self.t["vertices"], self.t["faces"], n, bits, self.inodes,
+                                          self.leaves, stream)
+        self.dev = SceneDev(
+            _ptr(self.t["vertices"]).value, _ptr(self.t["normals"]).value,
+            _ptr(self.t["tangents"]).value, _ptr(self.t["uvs"]).value, _ptr(self.t["faces"]).value,
+            _ptr(self.t["lights"]).value, _ptr(self.t["materials"]).value,
+            _ptr(self.t["textures"]).value, _ptr(self.t["others"]).value,
+            _ptr(self.inodes).value, _ptr(self.leaves).value, n, host.num_lights)
+        self.width, self.height = host.width, host.height
+
+
+def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None) -> None:
+    ctx.check(lib().clr_seed_pixel_states(ctx.handle, width, height, _ptr(states), _stream(stream)))
+
+
+def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, pixels,
+           mode: int = RENDER_MEGAKERNEL, stream=None) -> None:
+    rp = RenderParams(scene.width, scene.height, spp, max_depth, 0)
+    ctx.check(lib().clr_render(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp), _ptr(states),
+                               _ptr(pixels), mode, _stream(stream)))
+
+
+def new_image(width: int, height: int, device="cuda"):
+    """(states uint32 [H*W*4] seeded per pixel, pixels float32 [H*W*4] zeroed)."""
+    import torch
+
+    states = torch.empty(width * height * 4, dtype=torch.int32, device=device)
+    pixels = torch.zeros(width * height * 4, dtype=torch.float32, device=device)
+    return states, pixels

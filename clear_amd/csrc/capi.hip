@@ -74,6 +74,33 @@ int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut) {
     return CLR_OK;
 }
 
+int ensureRenderScratch(clr_ctx* ctx, uint32_t numFaces) {
+    if (!ctx->renderWords) {
+        hipError_t e = hipMalloc(&ctx->renderWords, 64 * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->renderWords = nullptr;
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "render words");
+        }
+    }
+    if (ctx->faceLightCap < numFaces || !ctx->faceLight) {
+        int rc = hipCheck(ctx, hipDeviceSynchronize(), "render scratch regrow sync");
+        if (rc) return rc;
+        if (ctx->faceLight) (void)hipFree(ctx->faceLight);
+        ctx->faceLight = nullptr;
+        ctx->faceLightCap = 0;
+        size_t want = std::max<size_t>(numFaces, 1);
+        hipError_t e = hipMalloc(&ctx->faceLight, want * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->faceLight = nullptr;
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "face light map");
+        }
+        ctx->faceLightCap = want;
+    }
+    return CLR_OK;
+}
+
 int ensureStatus(clr_ctx* ctx) {
     if (ctx->devStatus) return CLR_OK;
     hipError_t e = hipMalloc(&ctx->devStatus, 256);
@@ -154,6 +181,9 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->pool) (void)hipFree(ctx->pool);
         if (ctx->xchg) (void)hipFree(ctx->xchg);
         if (ctx->devStatus) (void)hipFree(ctx->devStatus);
+        if (ctx->faceLight) (void)hipFree(ctx->faceLight);
+        if (ctx->renderWords) (void)hipFree(ctx->renderWords);
+        if (ctx->wfPool) (void)hipFree(ctx->wfPool);
         for (auto& row : ctx->events)
             for (auto& e : row)
                 if (e) (void)hipEventDestroy(e);
@@ -312,6 +342,47 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
     return traceClosest(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
                         hitFace, hitT, stats, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+               uint32_t* randStates, clr_float3* pixels, int mode, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!scene || !params || !randStates || !pixels)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
+    const clr_scene_dev& s = *scene;
+    if (s.numFaces == 0 || !s.vertices || !s.faces || !s.leafNodes || (s.numFaces > 1 && !s.internalNodes) ||
+        !s.materialsData || !s.texturesData || !s.otherResources || (s.numLights && !s.lights))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "incomplete scene");
+    if (s.numLights == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "scene has no light");
+    if (params->width == 0 || params->height == 0 || params->maxDepth == 0)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "empty image or zero bounce cap");
+    if ((uint64_t)params->width * params->height >= (1ull << 31))
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "image too large");
+    if (!aligned16(randStates) || !aligned16(pixels) || !aligned16(s.vertices) ||
+        !aligned16(s.leafNodes) || (s.internalNodes && !aligned16(s.internalNodes)) ||
+        !aligned16(s.otherResources) || !aligned16(s.texturesData) || !aligned16(s.materialsData))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
+    if (params->samplesPerPixel == 0) return CLR_OK;
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (mode == CLR_RENDER_MEGAKERNEL) return renderMegakernel(ctx, s, *params, randStates, pixels, st);
+    if (mode == CLR_RENDER_WAVEFRONT) return renderWavefront(ctx, s, *params, randStates, pixels, st);
+    return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
+}
+
+int clr_seed_pixel_states(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* randStates,
+                          void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!randStates || !aligned16(randStates)) return fail(ctx, CLR_ERR_INVALID_ARG, "state buffer");
+    if ((uint64_t)width * height >= (1ull << 31)) return fail(ctx, CLR_ERR_UNSUPPORTED, "image too large");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return seedPixelStates(ctx, width, height, randStates, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -32,6 +32,14 @@ struct clr_ctx {
     // Device-side failure flags (bounded spin exceeded, stack overflow):
     // one 32-bit word, read back by clr_device_status.
     uint32_t* devStatus = nullptr;
+    // Render scratch: face -> first light index, and 64 small device words
+    // (word 0: first light at infinity).
+    uint32_t* faceLight = nullptr;
+    size_t faceLightCap = 0;
+    uint32_t* renderWords = nullptr;
+    // Wavefront path-state pool.
+    void* wfPool = nullptr;
+    size_t wfPoolBytes = 0;
 };
 
 namespace clr {
@@ -88,6 +96,15 @@ size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bi
 int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegOffsets,
                        uint32_t numSegments, uint32_t bitsPerDim, clr_lbvh_inode* inodes,
                        clr_lbvh_leaf* leaves, hipStream_t stream);
+
+// ---- rendering (render.hip, wavefront.hip) ----
+int ensureRenderScratch(clr_ctx* ctx, uint32_t numFaces);
+int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
+                     uint32_t* states, clr_float3* pixels, hipStream_t stream);
+int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
+                    uint32_t* states, clr_float3* pixels, hipStream_t stream);
+int seedPixelStates(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* states,
+                    hipStream_t stream);
 
 // ---- traversal (traverse.hip) ----
 int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,

@@ -167,6 +167,54 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
                       const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
                       uint32_t* stats, void* stream);
 
+/* ---- path tracing ---------------------------------------------------------- */
+
+/* Device-resident scene: the arguments of the reference kernel `pathtracing`
+ * (pathtracer.cl:25-33) plus sizes. Empty normal / tangent / uv arrays may be
+ * 1-element dummies as in the reference (main.cpp:450-464). */
+typedef struct clr_scene_dev {
+    const clr_float3* vertices;
+    const clr_float3* normals;
+    const clr_float3* tangents;
+    const float* uvs;                 /* float2 per uv */
+    const clr_face* faces;
+    const clr_light_info* lights;
+    const uint8_t* materialsData;
+    const uint8_t* texturesData;
+    const uint8_t* otherResources;    /* [0..2]: camera, environment, light-power offsets */
+    const clr_lbvh_inode* internalNodes;
+    const clr_lbvh_leaf* leafNodes;
+    uint32_t numFaces;
+    uint32_t numLights;
+} clr_scene_dev;
+
+typedef struct clr_render_params {
+    uint32_t width, height;   /* image size; must equal the camera record's */
+    uint32_t samplesPerPixel; /* camera paths per pixel in this call */
+    uint32_t maxDepth;        /* bounce cap: a path stops after maxDepth bounces
+                                 (the reference's cap is ray.depth > 99, i.e. 100,
+                                 pathtracer.cl:166) */
+    uint32_t flags;           /* reserved, 0 */
+} clr_render_params;
+
+enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
+
+/* Adds samplesPerPixel camera paths per pixel into `pixels` (clr_float3 per
+ * pixel, row-major, not divided by the sample count: the caller scales, as
+ * post_processing.cl:20-26 does) and advances the per-pixel xorshift128
+ * states `randStates` (4 x uint32 per pixel). Replaces the host loop of
+ * main.cpp:475-494 launching `pathtracing` per tile and iteration. Both
+ * modes consume each pixel's random stream in the reference's order and give
+ * the same image up to floating-point evaluation order. */
+int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+               uint32_t* randStates, clr_float3* pixels, int mode, void* stream);
+
+/* Seeds per-pixel states: seed = pixelIndex ^ 0x9E3779B9 expanded as
+ * s_k = seed = 1812433253 * (seed ^ (seed >> 30)) + k, k = 0..3
+ * (the per-pixel expansion of main.cpp:101-104, unsigned, no discards). */
+int clr_seed_pixel_states(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* randStates,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

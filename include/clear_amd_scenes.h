@@ -32,6 +32,51 @@ int clr_gen_cluster_soup(uint32_t numTris, int seed, uint32_t numClusters, float
  * (ox, oy, oz, dx, dy, dz, tmax). */
 int clr_gen_random_rays(uint32_t numRays, int seed, float* out);
 
+/* ---- render scenes ------------------------------------------------------
+ * Built on the host with the drop-in Scene / MaterialCreator API
+ * (include/clear_amd_scene.hpp) and exposed as the reference's arrays and
+ * byte blobs; upload them to the device and fill a clr_scene_dev. */
+typedef struct clr_host_scene clr_host_scene;
+
+enum {
+    /* Closed Cornell box [-1,1]^3, 0.3-radius icosphere (subdivision 5:
+     * 20,480 triangles) with vertex normals, 2-triangle ceiling light of
+     * emission 15, all Lambert (white 0.75, red/green 0.6), pinhole camera. */
+    CLR_SCENE_CORNELL = 0,
+    /* Coverage scene: every BxDF (Oren-Nayar, specular reflection and
+     * transmission, Ward, Ashikhmin-Shirley, Beckmann coat), checkerboard,
+     * bump, RGBA8, gray8 alpha and FP32 textures, normal maps, alpha
+     * cut-outs, an image-based environment and a thin lens. */
+    CLR_SCENE_SHOWCASE = 1,
+    /* White-furnace check: an icosphere of Lambert albedo `albedo` inside a
+     * uniform environment of radiance 1; the expected image is the camera
+     * weight times 1 (background) or albedo (sphere). */
+    CLR_SCENE_FURNACE = 2
+};
+
+enum {
+    CLR_ARRAY_VERTICES = 0, CLR_ARRAY_NORMALS, CLR_ARRAY_TANGENTS, CLR_ARRAY_UVS, CLR_ARRAY_FACES,
+    CLR_ARRAY_LIGHTS, CLR_ARRAY_MATERIALS, CLR_ARRAY_TEXTURES, CLR_ARRAY_OTHERS, CLR_ARRAY_BVH_NODES
+};
+
+typedef struct clr_scene_params {
+    uint32_t width, height;   /* image size (0 = scene default) */
+    uint32_t detail;          /* sphere subdivision level (0 = scene default) */
+    int seed;                 /* procedural content seed */
+    float lensRadius;         /* thin-lens radius (0 = pinhole) */
+    float focusDistance;      /* thin-lens focus distance (0 = scene default) */
+    uint32_t buildCpuBvh;     /* also build the reference's CPU midpoint BVH */
+    float albedo;             /* furnace sphere albedo */
+    float fovY;               /* vertical field of view in degrees (0 = scene default) */
+} clr_scene_params;
+
+int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** out);
+int clr_scene_free(clr_host_scene* scene);
+int clr_scene_info(const clr_host_scene* scene, uint32_t* numFaces, uint32_t* numLights,
+                   uint32_t* width, uint32_t* height);
+/* Pointer and byte size of one host array (CLR_ARRAY_*); valid until free. */
+int clr_scene_array(const clr_host_scene* scene, int which, const void** data, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

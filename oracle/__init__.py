@@ -126,3 +126,68 @@ def range_boxes(leaf_boxes: np.ndarray, lo: np.ndarray, hi: np.ndarray) -> np.nd
                           _p(np.ascontiguousarray(lo, dtype=np.uint32)),
                           _p(np.ascontiguousarray(hi, dtype=np.uint32)), n, _p(out))
     return out[: n - 1]
+
+
+def seed_states(width, height):
+    st = np.zeros(width * height * 4, dtype=np.uint32)
+    lib().orc_seed_states(ctypes.c_uint32(width), ctypes.c_uint32(height), _p(st))
+    return st
+
+
+def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, states, pixels,
+           pix_begin=0, pix_end=None, threads=0):
+    """Runs the render restatement over pixels [pix_begin, pix_end) (row-major),
+    accumulating into `pixels` (float32 [H*W*4]) and advancing `states`."""
+    L = lib()
+    if not hasattr(L.orc_render, "_typed"):
+        L.orc_render.restype = ctypes.c_int
+        L.orc_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.orc_render._typed = True
+    keep = [np.ascontiguousarray(arrays[k]) for k in
+            ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
+             "others"]]
+    keep.append(np.ascontiguousarray(inodes) if num_faces > 1 else np.zeros((1, 48), np.uint8))
+    keep.append(np.ascontiguousarray(leaves))
+    ptrs = (ctypes.c_void_p * 11)(*[k.ctypes.data for k in keep])
+    end = width * height if pix_end is None else pix_end
+    rc = L.orc_render(ptrs, num_faces, width, height, spp, max_depth, _p(states), _p(pixels),
+                      pix_begin, end, threads)
+    if rc:
+        raise RuntimeError(f"orc_render failed: {rc}")
+
+
+def _probe_fn(name, argtypes):
+    fn = getattr(lib(), name)
+    fn.restype = ctypes.c_int
+    fn.argtypes = argtypes
+    return fn
+
+
+def bxdf_probe(kind, p0, p1, R, wo, wi, u0, u1):
+    fp = ctypes.POINTER(ctypes.c_float)
+    fn = _probe_fn("orc_bxdf_probe", [ctypes.c_int, ctypes.c_float, ctypes.c_float, fp, fp, fp,
+                                      ctypes.c_float, ctypes.c_float, fp])
+    out = (ctypes.c_float * 11)()
+    arr = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])
+    fn(kind, p0, p1, arr(R), arr(wo), arr(wi), u0, u1, out)
+    o = list(out)
+    return {"f": o[0:3], "pdf": o[3], "wi": o[4:7], "spdf": o[7], "sf": o[8:11]}
+
+
+def fresnel_probe(ftype, eta, k, eta_ext, eta_int, cosi):
+    fp = ctypes.POINTER(ctypes.c_float)
+    fn = _probe_fn("orc_fresnel_probe", [ctypes.c_int, fp, fp, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, fp])
+    out = (ctypes.c_float * 3)()
+    arr = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])
+    fn(ftype, arr(eta), arr(k), eta_ext, eta_int, cosi, out)
+    return list(out)
+
+
+def disk_probe(u1, u2):
+    fn = _probe_fn("orc_disk_probe", [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)])
+    out = (ctypes.c_float * 2)()
+    fn(u1, u2, out)
+    return list(out)
