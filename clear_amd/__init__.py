@@ -267,7 +267,7 @@ def decode_leaves(buf: np.ndarray, n: int):
 
 # ---- scenes and rendering -------------------------------------------------
 
-SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE = 0, 1, 2
+SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN = 0, 1, 2, 3
 ARRAY_NAMES = ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials",
                "textures", "others", "bvh_nodes"]
 RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1

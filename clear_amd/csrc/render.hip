@@ -63,7 +63,9 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                 SurfPt lsp;
                 const bool hit = intersect(S, sorg, vinL, stack, stride, lsp);
                 const bool visible = atInf ? !hit : (hit && lsp.face == lp.face);
-                if (visible) {
+                // A zero or infinite light density (an environment sample at a
+                // pole, where sin(theta) = 0) contributes nothing.
+                if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {
                     const V3 vinLrev = -vinL;
                     const float fsPDF = evalFsPdf(B, vout, vinL) * absCosEDF(E, vinL) / dist2;
                     const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);

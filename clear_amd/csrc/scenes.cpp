@@ -389,6 +389,125 @@ void buildFurnace(clr_host_scene* hs, const clr_scene_params& p) {
                    p.fovY > 0 ? p.fovY : 40.0f, 0.0f, 1.0f);
 }
 
+// Value-noise fBm on a wrapped 256^2 lattice of seeded values, smoothstep
+// interpolation, `octaves` octaves of doubling frequency and halving
+// amplitude; result roughly in [-1, 1].
+struct Fbm {
+    std::vector<float> lat;
+    explicit Fbm(int seed) : lat(256 * 256) {
+        Rng r(seed);
+        for (auto& v : lat) v = 2.0f * r.uniform() - 1.0f;
+    }
+    float noise(float x, float y) const {
+        const float fx = std::floor(x), fy = std::floor(y);
+        const int ix = (int)fx, iy = (int)fy;
+        float tx = x - fx, ty = y - fy;
+        tx = tx * tx * (3 - 2 * tx);
+        ty = ty * ty * (3 - 2 * ty);
+        auto at = [&](int a, int b) { return lat[(size_t)(b & 255) * 256 + (a & 255)]; };
+        const float a = at(ix, iy) + tx * (at(ix + 1, iy) - at(ix, iy));
+        const float b = at(ix, iy + 1) + tx * (at(ix + 1, iy + 1) - at(ix, iy + 1));
+        return a + ty * (b - a);
+    }
+    float operator()(float x, float y, int octaves) const {
+        float s = 0.0f, amp = 0.5f, f = 1.0f;
+        for (int o = 0; o < octaves; ++o) {
+            s += amp * noise(x * f, y * f);
+            amp *= 0.5f;
+            f *= 2.0f;
+        }
+        return s;
+    }
+};
+
+// Outdoor scene: a (detail x detail)-vertex fBm heightfield over [-10,10]^2
+// (default 1024 -> 2,093,058 triangles) with a tiled 512^2 RGB8 normal map,
+// 200 icospheres (subdivision 4, 5,120 triangles each) resting on it, every
+// object a Lambert base under a Beckmann coat (alpha U[0.05,0.5]) with a
+// dielectric Fresnel of eta 1.5, lit by a 1024x512 FP32 sky with a Gaussian
+// sun of peak 5e4, seen through a thin lens (default radius 0.02).
+void buildTerrain(clr_host_scene* hs, const clr_scene_params& p) {
+    Scene& s = hs->scene;
+    MaterialCreator& mc = MaterialCreator::sharedInstance();
+    mc.setScene(&s);
+    const int seed = p.seed ? p.seed : 2;
+    Rng rng(seed);
+    const uint32_t ew = 1024, eh = 512;
+    {
+        std::vector<float> sky = skyImage(ew, eh, 5.0e4f, 0.55f, 2.2f, 0.03f);
+        mc.createImageTexture("env", TextureType::ColorImageRGBF32x3, ew, eh, sky.data());
+    }
+    mc.createImageBasedEnvLightPropety("IBL", "env", 1.0f);
+    EnvironmentHead eh0{(uint32_t)s.idxOfLight("IBL")};
+    s.setEnvironment(addDataAligned(&s.otherResouces, eh0, 4));
+
+    std::vector<uint8_t> nm = bumpImage(512, 512, seed + 1);
+    mc.createNormalMapTexture("terrain_nm", 512, 512, nm.data());
+    mc.createFloatConstantTexture("sigma0", 0.0f);
+    mc.createFloat3ConstantTexture("R_coat", 1.0f, 1.0f, 1.0f);
+    auto coat = [&](const std::string& name, const char* bump, float r, float g, float b) {
+        const std::string rd = name + "_Rd", al = name + "_alpha";
+        mc.createFloat3ConstantTexture(rd.c_str(), r, g, b);
+        mc.createFloatConstantTexture(al.c_str(), 0.05f + 0.45f * rng.uniform());
+        mc.createCoatedMaterial(name.c_str(), bump, rd.c_str(), "sigma0", "R_coat", al.c_str(), 1.5f);
+        return (uint16_t)s.idxOfMat(name);
+    };
+    const uint16_t terrainMat = coat("terrain", "terrain_nm", 0.42f, 0.38f, 0.30f);
+
+    const uint32_t G = p.detail >= 2 ? p.detail : 1024;
+    const float ext = 10.0f, amp = 1.6f;
+    Fbm fbm(seed);
+    auto height = [&](float x, float z) { return amp * fbm(x * 0.25f + 37.0f, z * 0.25f + 11.0f, 6); };
+    {
+        Mesh t;
+        t.p.resize(3 * (size_t)G * G);
+        t.n.resize(3 * (size_t)G * G);
+        t.uv.resize(2 * (size_t)G * G);
+        const float step = 2 * ext / (G - 1);
+        for (uint32_t j = 0; j < G; ++j)
+            for (uint32_t i = 0; i < G; ++i) {
+                const float x = -ext + step * i, z = -ext + step * j;
+                const size_t k = (size_t)j * G + i;
+                t.p[3 * k] = x;
+                t.p[3 * k + 1] = height(x, z);
+                t.p[3 * k + 2] = z;
+                const float dx = height(x + step, z) - height(x - step, z);
+                const float dz = height(x, z + step) - height(x, z - step);
+                float n[3] = {-dx, 2 * step, -dz};
+                const float l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                t.n[3 * k] = n[0] / l;
+                t.n[3 * k + 1] = n[1] / l;
+                t.n[3 * k + 2] = n[2] / l;
+                t.uv[2 * k] = 8.0f * i / (G - 1);
+                t.uv[2 * k + 1] = 8.0f * j / (G - 1);
+            }
+        t.tri.reserve(6 * (size_t)(G - 1) * (G - 1));
+        for (uint32_t j = 0; j + 1 < G; ++j)
+            for (uint32_t i = 0; i + 1 < G; ++i) {
+                const uint32_t a = j * G + i, b = a + 1, c = a + G, d = c + 1;
+                t.tri.insert(t.tri.end(), {a, c, b, b, c, d});
+            }
+        addMesh(s, t, true, true, true, [&](size_t) { return terrainMat; });
+    }
+    Mesh ico = icosphere(4);
+    for (int k = 0; k < 200; ++k) {
+        const float x = -8.0f + 16.0f * rng.uniform(), z = -8.0f + 16.0f * rng.uniform();
+        const float r = 0.2f + 0.4f * rng.uniform();
+        const float cr = 0.15f + 0.8f * rng.uniform(), cg = 0.15f + 0.8f * rng.uniform(),
+                    cb = 0.15f + 0.8f * rng.uniform();
+        const uint16_t m = coat("sphere" + std::to_string(k), nullptr, cr, cg, cb);
+        s.localToWorld.push();
+        s.localToWorld *= Mat4::translate(x, height(x, z) + 0.8f * r, z) * Mat4::scale(r, r, r);
+        addMesh(s, ico, true, false, false, [&](size_t) { return m; });
+        s.localToWorld.pop();
+    }
+    const float eye[3] = {0.0f, 3.2f, 11.0f}, at[3] = {0.0f, 0.0f, 0.0f};
+    const float lens = p.lensRadius > 0 ? p.lensRadius : (p.lensRadius < 0 ? 0.0f : 0.02f);
+    const float focus = p.focusDistance > 0 ? p.focusDistance : std::sqrt(3.2f * 3.2f + 11.0f * 11.0f);
+    setPerspective(s, hs, p.width ? p.width : 1920, p.height ? p.height : 1080, eye, at,
+                   p.fovY > 0 ? p.fovY : 45.0f, lens, focus);
+}
+
 }  // namespace
 
 extern "C" {
@@ -406,6 +525,7 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
             case CLR_SCENE_CORNELL: buildCornell(hs, p); break;
             case CLR_SCENE_SHOWCASE: buildShowcase(hs, p); break;
             case CLR_SCENE_FURNACE: buildFurnace(hs, p); break;
+            case CLR_SCENE_TERRAIN: buildTerrain(hs, p); break;
             default: delete hs; return CLR_ERR_UNSUPPORTED;
         }
         hs->scene.build();

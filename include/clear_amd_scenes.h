@@ -51,7 +51,14 @@ enum {
     /* White-furnace check: an icosphere of Lambert albedo `albedo` inside a
      * uniform environment of radiance 1; the expected image is the camera
      * weight times 1 (background) or albedo (sphere). */
-    CLR_SCENE_FURNACE = 2
+    CLR_SCENE_FURNACE = 2,
+    /* Outdoor renderer workload: a detail^2-vertex fBm heightfield (default
+     * 1024^2: 2,093,058 triangles) with a 512^2 RGB8 normal map plus 200
+     * subdivision-4 icospheres (1,024,000 triangles), all Lambert under a
+     * Beckmann coat (alpha U[0.05,0.5], dielectric eta 1.5), a 1024x512 FP32
+     * sky with a sun of peak 5e4, thin lens (default radius 0.02; a negative
+     * lensRadius gives a pinhole), 1920x1080 by default. */
+    CLR_SCENE_TERRAIN = 3
 };
 
 enum {
@@ -63,7 +70,7 @@ typedef struct clr_scene_params {
     uint32_t width, height;   /* image size (0 = scene default) */
     uint32_t detail;          /* sphere subdivision level (0 = scene default) */
     int seed;                 /* procedural content seed */
-    float lensRadius;         /* thin-lens radius (0 = pinhole) */
+    float lensRadius;         /* thin-lens radius (0 = scene default: pinhole except TERRAIN) */
     float focusDistance;      /* thin-lens focus distance (0 = scene default) */
     uint32_t buildCpuBvh;     /* also build the reference's CPU midpoint BVH */
     float albedo;             /* furnace sphere albedo */

@@ -31,7 +31,10 @@
 //     (reflection.cl:210; the Equivalent omits it, sim_reflection.hpp:208-211);
 //  D10 stratum indices of piecewise-constant distributions are clamped to
 //     numValues - 1 (distribution.cl:110-111 indexes one past the end for a
-//     point at the domain's upper end, as its own comment there notes).
+//     point at the domain's upper end, as its own comment there notes);
+//  D11 a light sample whose density is zero or infinite (an environment
+//     sample exactly at a pole, u = 0, where sin(theta) = 0 makes
+//     light.cl:169's pdf infinite and the MIS weight inf/inf) adds nothing.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -1066,7 +1069,7 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
                 SurfacePoint lsp;
                 bool hit = rayIntersection(s, sorg, vinL, &lsp);
                 bool visible = lp.atInfinity ? !hit : (hit && lsp.faceID == lp.faceID);  // D4
-                if (visible) {
+                if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {  // D11
                     V3 vinLrev = -vinL;
                     float fsPDF = fsPdf(B, vout, vinL) * absCosNsEDF(E, vinL) / dist2;
                     MIS = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);

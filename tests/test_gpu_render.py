@@ -114,6 +114,22 @@ def test_white_furnace(gpu_ctx):
     compare("furnace", host, 32, 64, gpu_render(gpu_ctx, host, 32, 64), oracle_render(host, 32, 64))
 
 
+def test_terrain_reduced(gpu_ctx):
+    """Outdoor scene with a 256^2 heightfield: coat BSDF, normal map, HDR sky
+    with a 5e4 sun, thin lens 0.02."""
+    host = ca.HostScene(ca.SCENE_TERRAIN, 192, 108, detail=256)
+    print(f"[render parity] terrain detail=256 faces={host.num_faces} 192x108 spp=8 depth=8",
+          flush=True)
+    compare("terrain256", host, 8, 8, gpu_render(gpu_ctx, host, 8, 8), oracle_render(host, 8, 8))
+
+
+def test_terrain_full_geometry(gpu_ctx):
+    """The full 3,117,058-triangle outdoor scene at a small image size."""
+    host = ca.HostScene(ca.SCENE_TERRAIN, 128, 72)
+    print(f"[render parity] terrain full faces={host.num_faces} 128x72 spp=2 depth=8", flush=True)
+    compare("terrain", host, 2, 8, gpu_render(gpu_ctx, host, 2, 8), oracle_render(host, 2, 8))
+
+
 def test_render_rejects_bad_arguments(gpu_ctx):
     import torch
 
