@@ -65,6 +65,9 @@ def lib():
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
         "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
+        "clr_wavefront_get_config": (i32, [vp, vp]),
+        "clr_wavefront_set_config": (i32, [vp, vp]),
+        "clr_wavefront_autotune": (i32, [vp, vp, vp, vp, vp]),
         "clr_scene_build": (i32, [i32, vp, ctypes.POINTER(vp)]),
         "clr_scene_free": (i32, [vp]),
         "clr_scene_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
@@ -363,6 +366,37 @@ def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, p
     rp = RenderParams(scene.width, scene.height, spp, max_depth, 0)
     ctx.check(lib().clr_render(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp), _ptr(states),
                                _ptr(pixels), mode, _stream(stream)))
+
+
+class WavefrontConfig(ctypes.Structure):
+    """clr_wavefront_config: launch shapes of the wavefront renderer."""
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ["traceBlock", "traceWavesPerCU", "shadeBlock", "shadeWavesPerCU", "itersPerGraph",
+                 "checkEvery", "maxSlots", "useGraph"]]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def wavefront_config(ctx: Context) -> WavefrontConfig:
+    c = WavefrontConfig()
+    ctx.check(lib().clr_wavefront_get_config(ctx.handle, ctypes.byref(c)))
+    return c
+
+
+def set_wavefront_config(ctx: Context, cfg: WavefrontConfig) -> None:
+    ctx.check(lib().clr_wavefront_set_config(ctx.handle, ctypes.byref(cfg)))
+
+
+def wavefront_autotune(ctx: Context, scene: "DeviceScene", spp: int, max_depth: int,
+                       stream=None) -> WavefrontConfig:
+    """Times launch-shape candidates on this scene (scratch image and states)
+    and keeps the fastest in the context."""
+    rp = RenderParams(scene.width, scene.height, spp, max_depth, 0)
+    c = WavefrontConfig()
+    ctx.check(lib().clr_wavefront_autotune(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp),
+                                           ctypes.byref(c), _stream(stream)))
+    return c
 
 
 def new_image(width: int, height: int, device="cuda"):

@@ -184,6 +184,7 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->faceLight) (void)hipFree(ctx->faceLight);
         if (ctx->renderWords) (void)hipFree(ctx->renderWords);
         if (ctx->wfPool) (void)hipFree(ctx->wfPool);
+        destroyWfState(ctx);
         for (auto& row : ctx->events)
             for (auto& e : row)
                 if (e) (void)hipEventDestroy(e);
@@ -351,8 +352,55 @@ extern "C" {
 int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
                uint32_t* randStates, clr_float3* pixels, int mode, void* stream) {
     if (!ctx) return CLR_ERR_INVALID_ARG;
-    if (!scene || !params || !randStates || !pixels)
-        return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
+    if (!randStates || !pixels) return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
+    int rc = checkRenderArgs(ctx, scene, params);
+    if (rc) return rc;
+    if (!aligned16(randStates) || !aligned16(pixels))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
+    if (mode == CLR_RENDER_WAVEFRONT && params->maxDepth > 255)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
+    if (params->samplesPerPixel == 0) return CLR_OK;
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (mode == CLR_RENDER_MEGAKERNEL) return renderMegakernel(ctx, *scene, *params, randStates, pixels, st);
+    if (mode == CLR_RENDER_WAVEFRONT) return renderWavefront(ctx, *scene, *params, randStates, pixels, st);
+    return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
+}
+
+int clr_wavefront_get_config(clr_ctx* ctx, clr_wavefront_config* config) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!config) return fail(ctx, CLR_ERR_INVALID_ARG, "null config");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return wavefrontGetConfig(ctx, config);
+}
+
+int clr_wavefront_set_config(clr_ctx* ctx, const clr_wavefront_config* config) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!config) return fail(ctx, CLR_ERR_INVALID_ARG, "null config");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return wavefrontSetConfig(ctx, config);
+}
+
+int clr_wavefront_autotune(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                           clr_wavefront_config* config, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    int rc = checkRenderArgs(ctx, scene, params);
+    if (rc) return rc;
+    if (params->maxDepth > 255) return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
+    if (params->samplesPerPixel == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "autotune needs samplesPerPixel >= 1");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return wavefrontAutotune(ctx, *scene, *params, config, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+namespace clr {
+int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params) {
+    if (!scene || !params) return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
     const clr_scene_dev& s = *scene;
     if (s.numFaces == 0 || !s.vertices || !s.faces || !s.leafNodes || (s.numFaces > 1 && !s.internalNodes) ||
         !s.materialsData || !s.texturesData || !s.otherResources || (s.numLights && !s.lights))
@@ -362,18 +410,14 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
         return fail(ctx, CLR_ERR_INVALID_ARG, "empty image or zero bounce cap");
     if ((uint64_t)params->width * params->height >= (1ull << 31))
         return fail(ctx, CLR_ERR_UNSUPPORTED, "image too large");
-    if (!aligned16(randStates) || !aligned16(pixels) || !aligned16(s.vertices) ||
-        !aligned16(s.leafNodes) || (s.internalNodes && !aligned16(s.internalNodes)) ||
+    if (!aligned16(s.vertices) || !aligned16(s.leafNodes) || (s.internalNodes && !aligned16(s.internalNodes)) ||
         !aligned16(s.otherResources) || !aligned16(s.texturesData) || !aligned16(s.materialsData))
         return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
-    if (params->samplesPerPixel == 0) return CLR_OK;
-    DeviceScope ds(ctx->device);
-    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (mode == CLR_RENDER_MEGAKERNEL) return renderMegakernel(ctx, s, *params, randStates, pixels, st);
-    if (mode == CLR_RENDER_WAVEFRONT) return renderWavefront(ctx, s, *params, randStates, pixels, st);
-    return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
+    return CLR_OK;
 }
+}  // namespace clr
+
+extern "C" {
 
 int clr_seed_pixel_states(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* randStates,
                           void* stream) {

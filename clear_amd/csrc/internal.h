@@ -8,6 +8,10 @@
 
 #include "../../include/clear_amd.h"
 
+namespace clr {
+struct WfState;
+}
+
 struct clr_ctx {
     int device = 0;
     std::string lastError;
@@ -37,9 +41,10 @@ struct clr_ctx {
     uint32_t* faceLight = nullptr;
     size_t faceLightCap = 0;
     uint32_t* renderWords = nullptr;
-    // Wavefront path-state pool.
+    // Wavefront path-state pool, configuration and captured graph.
     void* wfPool = nullptr;
     size_t wfPoolBytes = 0;
+    clr::WfState* wf = nullptr;
 };
 
 namespace clr {
@@ -105,6 +110,13 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
                     uint32_t* states, clr_float3* pixels, hipStream_t stream);
 int seedPixelStates(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* states,
                     hipStream_t stream);
+int wavefrontGetConfig(clr_ctx* ctx, clr_wavefront_config* cfg);
+int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg);
+int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
+                      clr_wavefront_config* out, hipStream_t stream);
+void destroyWfState(clr_ctx* ctx);
+// Argument checks shared by clr_render and the autotuner.
+int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params);
 
 // ---- traversal (traverse.hip) ----
 int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,

@@ -136,11 +136,12 @@ namespace {
 
 constexpr int kTile = 8;  // 8x8 pixels per wave
 
-__global__ __launch_bounds__(64) void k_render_mega(dev::RScene S, uint32_t width, uint32_t height,
+__global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t width, uint32_t height,
                                                     uint32_t spp, uint32_t maxDepth,
                                                     uint32_t* __restrict__ states,
                                                     clr_float3* __restrict__ pixels) {
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+    dev::RScene S = scene;  // resolved copy; the kernel argument itself stays read-only
     dev::finishScene(S);
     const uint32_t tilesX = (width + kTile - 1) / kTile;
     const uint32_t gx = (blockIdx.x % tilesX) * kTile + (threadIdx.x % kTile);
@@ -196,6 +197,8 @@ __global__ void k_seed_states(uint32_t width, uint32_t height, uint32_t* states)
 int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hipStream_t stream) {
     int rc = ensureRenderScratch(ctx, sd.numFaces);
     if (rc) return rc;
+    rc = ensureStatus(ctx);
+    if (rc) return rc;
     uint32_t* faceLight = ctx->faceLight;
     int* envLight = reinterpret_cast<int*>(ctx->renderWords);
     k_face_lights_clear<<<(sd.numFaces + 255) / 256 + 1, 256, 0, stream>>>(faceLight, sd.numFaces, envLight);
@@ -216,6 +219,7 @@ int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hi
     S.faceLight = faceLight;
     S.envLightPtr = envLight;
     S.envLight = -1;
+    S.status = ctx->devStatus;
     return hipCheck(ctx, hipGetLastError(), "scene prepare launch");
 }
 

@@ -43,6 +43,7 @@ struct RScene {
     const uint32_t* faceLight; // face -> first light index with that face (or kNone)
     const int* envLightPtr;    // first light at infinity (INT_MAX if none), device word
     int envLight;              // resolved by finishScene: index or -1
+    uint32_t* status;          // device status word (stack overflow flag)
 };
 
 template <typename T>
@@ -350,8 +351,10 @@ __device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint3
                                           SurfPt& sp) {
     float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
     TraceStats st{0u, 0u};
-    const uint32_t face = closestHit<false, TexAlpha>(S.g, org, dir, t, stack, stride, st, &b1, &b2,
-                                                       TexAlpha{&S});
+    bool over = false;
+    const uint32_t face = closestHit<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1,
+                                                       &b2, TexAlpha{&S}, &over);
+    if (over) atomicOr(S.status, kStatusStackOverflow);
     if (face == kNone || isinf(t)) {
         sp.face = kNone;
         return false;

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
     uint32_t n, const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces,
     const float* __restrict__ rays, uint32_t numRays, uint32_t* __restrict__ hitFace,
-    float* __restrict__ hitT, uint32_t* __restrict__ stats) {
+    float* __restrict__ hitT, uint32_t* __restrict__ stats, uint32_t* __restrict__ status) {
     __shared__ uint32_t s_stack[dev::kStackDepth * kTraceBlock];
     const uint32_t r = blockIdx.x * kTraceBlock + threadIdx.x;
     if (r >= numRays) return;
@@ -35,7 +35,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     float t = rays[6 * R + r];
     dev::GeomScene g{inodes, leaves, n, verts, faces};
     dev::TraceStats st{0u, 0u};
-    uint32_t best = dev::closestHit<STATS>(g, o, d, t, s_stack + threadIdx.x, kTraceBlock, st);
+    bool over = false;
+    uint32_t best = dev::closestHit<STATS>(g, o, d, t, dev::LdsStack{s_stack + threadIdx.x, kTraceBlock},
+                                           st, nullptr, nullptr, dev::NoAlpha(), &over);
+    if (over) atomicOr(status, kStatusStackOverflow);
     const bool hit = best != kEmpty && !isinf(t);
     hitFace[r] = hit ? best : kEmpty;
     hitT[r] = t;
@@ -52,13 +55,15 @@ int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
                  uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
                  hipStream_t stream) {
     if (numRays == 0) return CLR_OK;
+    int rc = ensureStatus(ctx);
+    if (rc) return rc;
     const unsigned grid = (numRays + kTraceBlock - 1) / kTraceBlock;
     if (stats)
         k_trace_closest<true><<<grid, kTraceBlock, 0, stream>>>(
-            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats);
+            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus);
     else
         k_trace_closest<false><<<grid, kTraceBlock, 0, stream>>>(
-            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats);
+            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus);
     return hipCheck(ctx, hipGetLastError(), "trace launch");
 }
 

@@ -114,16 +114,45 @@ struct NoAlpha {
     __device__ __forceinline__ bool reject(uint32_t, float, float) const { return false; }
 };
 
-// rayIntersection (bvh_traversal.cl:182-215). `stack` points at this lane's
-// first slot; slot k is stack[k * stride]. Returns the face or kNone; t, b1,
-// b2 describe the accepted hit. An alpha-rejected triangle leaves t unchanged
-// (the reference stores t before its alpha test, :87 vs :102-105, which lets
-// a masked-out triangle occlude; that is not reproduced).
-template <bool STATS, class Alpha = NoAlpha>
+// Traversal stacks. The reference keeps a private uint[64] per work item
+// (bvh_traversal.cl:185). Here slot k of a lane lives in LDS at lds[k * stride]
+// ([depth][lane], conflict-free when lanes sit at equal depth); SpillStack
+// keeps only the first L slots in LDS and the rest in a per-lane global
+// region (slot k >= L at spill[(k - L) * spillStride]), which trades a rare
+// global access for 64 / L times more waves per CU.
+struct LdsStack {
+    uint32_t* lds;
+    int stride;
+    __device__ __forceinline__ void put(int k, uint32_t v) const { lds[k * stride] = v; }
+    __device__ __forceinline__ uint32_t get(int k) const { return lds[k * stride]; }
+};
+template <int L>
+struct SpillStack {
+    uint32_t* lds;
+    int stride;
+    uint32_t* spill;
+    size_t spillStride;
+    __device__ __forceinline__ void put(int k, uint32_t v) const {
+        if (k < L) lds[k * stride] = v;
+        else spill[(size_t)(k - L) * spillStride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int k) const {
+        return k < L ? lds[k * stride] : spill[(size_t)(k - L) * spillStride];
+    }
+};
+
+// rayIntersection (bvh_traversal.cl:182-215). Returns the face or kNone; t,
+// b1, b2 describe the accepted hit. An alpha-rejected triangle leaves t
+// unchanged (the reference stores t before its alpha test, :87 vs :102-105,
+// which lets a masked-out triangle occlude; that is not reproduced). A push
+// beyond 64 entries (the reference's array bound) is dropped and reported
+// through *overflow.
+template <bool STATS, class Alpha = NoAlpha, class Stack = LdsStack>
 __device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, float& t,
-                                               uint32_t* stack, int stride, TraceStats& st,
+                                               const Stack& stack, TraceStats& st,
                                                float* b1Out = nullptr, float* b2Out = nullptr,
-                                               const Alpha& alpha = Alpha()) {
+                                               const Alpha& alpha = Alpha(),
+                                               bool* overflow = nullptr) {
     uint32_t best = kNone;
     float bb1 = 0.0f, bb2 = 0.0f;
     auto tryLeaf = [&](uint32_t leafIdx) {
@@ -144,29 +173,71 @@ __device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, f
         tryLeaf(0);
     } else if (g.n > 1) {
         int depth = 0;
-        stack[0] = 0u;
+        stack.put(0, 0u);
         depth = 1;
+        bool over = false;
         while (depth > 0) {
             --depth;
-            const uint32_t idx = stack[depth * stride];
+            const uint32_t idx = stack.get(depth);
             const float4* np = reinterpret_cast<const float4*>(g.inodes + idx);
             const float4 mn = np[0], mx = np[1];
             if (!boxTest(mn, mx, o, d, t)) continue;
             const uint4 c = reinterpret_cast<const uint4*>(np)[2];
             if constexpr (STATS) ++st.visits;
-            if (c.x & 0xFFu)
-                tryLeaf(c.y);
-            else if (depth < kStackDepth)
-                stack[(depth++) * stride] = c.y;
-            if (c.x & 0xFF00u)
-                tryLeaf(c.z);
-            else if (depth < kStackDepth)
-                stack[(depth++) * stride] = c.z;
+            if (c.x & 0xFFu) tryLeaf(c.y);
+            else if (depth < kStackDepth) stack.put(depth++, c.y);
+            else over = true;
+            if (c.x & 0xFF00u) tryLeaf(c.z);
+            else if (depth < kStackDepth) stack.put(depth++, c.z);
+            else over = true;
         }
+        if (overflow && over) *overflow = true;
     }
     if (b1Out) *b1Out = bb1;
     if (b2Out) *b2Out = bb2;
     return best;
+}
+
+// Occlusion query with the same visiting order and tests as closestHit with
+// t = +inf, stopped at the first accepted triangle with a finite distance:
+// true exactly when closestHit would report a hit.
+template <class Alpha = NoAlpha, class Stack = LdsStack>
+__device__ __forceinline__ bool anyHit(const GeomScene& g, V3 o, V3 d, const Stack& stack,
+                                       const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
+    float t = INFINITY;
+    auto tryLeaf = [&](uint32_t leafIdx) -> bool {
+        const float4* lp = reinterpret_cast<const float4*>(g.leaves + leafIdx);
+        const float4 mn = lp[0], mx = lp[1];
+        if (!boxTest(mn, mx, o, d, t)) return false;
+        const uint32_t obj = reinterpret_cast<const uint32_t*>(lp + 2)[0];
+        float tt = t, b1, b2;
+        if (!triTest(g, o, d, obj, tt, b1, b2)) return false;
+        if (alpha.reject(obj, b1, b2)) return false;
+        return !isinf(tt);
+    };
+    if (g.n == 1) return tryLeaf(0);
+    if (g.n == 0) return false;
+    int depth = 0;
+    stack.put(0, 0u);
+    depth = 1;
+    bool over = false, hit = false;
+    while (depth > 0 && !hit) {
+        --depth;
+        const uint32_t idx = stack.get(depth);
+        const float4* np = reinterpret_cast<const float4*>(g.inodes + idx);
+        const float4 mn = np[0], mx = np[1];
+        if (!boxTest(mn, mx, o, d, t)) continue;
+        const uint4 c = reinterpret_cast<const uint4*>(np)[2];
+        if (c.x & 0xFFu) hit = tryLeaf(c.y);
+        else if (depth < kStackDepth) stack.put(depth++, c.y);
+        else over = true;
+        if (hit) break;
+        if (c.x & 0xFF00u) hit = tryLeaf(c.z);
+        else if (depth < kStackDepth) stack.put(depth++, c.z);
+        else over = true;
+    }
+    if (overflow && over) *overflow = true;
+    return hit;
 }
 
 }  // namespace dev

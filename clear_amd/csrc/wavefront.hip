@@ -1,13 +1,755 @@
-// Wavefront path tracer (queues of ray, shadow and material work).
+// Wavefront path tracer for gfx950: the same integrator as the megakernel
+// (pathtracing, OpenCL_PathTracer/OpenCL/pathtracer.cl:25-197) split into
+// queue-driven kernels, each running only the lanes that have that kind of
+// work:
+//
+//   extend  closest hit for every ray of the extension queue; the entry goes
+//           to the hit queue (surface material work) or the miss queue
+//           (environment material work);
+//   shade   hit queue: emission with MIS, bounce cap, Russian roulette, BSDF
+//           set-up, light sample (-> shadow queue, with its contribution
+//           already weighted), BSDF sample (-> next extension queue);
+//   miss    miss queue: environment emission with MIS, end of the sample;
+//   shadow  shadow queue: occlusion (any-hit for environment samples, closest
+//           hit compared with the light's face for area samples), adds the
+//           contribution, ends a sample whose BSDF sample failed.
+//
+// One path slot per pixel of a chunk; a slot runs its pixel's samples one
+// after the other and starts the next camera path as soon as one ends, so the
+// queues stay full until the last samples drain. Per pixel, the random draws
+// and the order of radiance additions are those of the megakernel (emission
+// of a vertex, then its light sample's contribution, then the next vertex),
+// so both renderers produce the same image for the same RNG states.
+//
+// Kernels grid-stride over device-side queue counts, so an iteration can be
+// launched without knowing its size: a render is a fixed sequence of
+// iterations (an upper bound of spp * (maxDepth + 1)), captured in a hipGraph
+// of G iterations and replayed, with an optional early stop when the
+// extension queue runs dry. Block shapes, resident waves per CU, G and the
+// early-stop period come from WfConfig, picked by clr_wavefront_autotune.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "internal.h"
+#include "render_device.h"
 
 namespace clr {
 
-int renderWavefront(clr_ctx* ctx, const clr_scene_dev&, const clr_render_params&, uint32_t*, clr_float3*,
-                    hipStream_t) {
-    return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer not available in this build");
+int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hipStream_t stream);
+
+namespace {
+
+using namespace dev;
+
+constexpr int kLdsDepth = 32;                   // traversal stack slots kept in LDS
+constexpr uint32_t kFlagEndAfterShadow = 1u << 16;
+
+// Device-side counters (one 256-B line).
+enum : uint32_t { kCntExt0 = 0, kCntExt1 = 1, kCntShadow = 2, kCntHit = 3, kCntMiss = 4, kCntChunk = 8 };
+
+struct WfBufs {
+    uint4* rng;      // [P] xorshift128 state
+    float4* alpha;   // [P] xyz path throughput, w initial luminance
+    float4* pix;     // [P] xyz radiance of the current sample, w dirPDF
+    float4* acc;     // [P] xyz sum over finished samples, w samples left (bits)
+    uint32_t* meta;  // [P] depth | sampledType << 8 | flags
+    uint32_t* rqSlot[2];
+    float4* rqOrg[2];
+    float4* rqDir[2];
+    uint4* hit;      // [P] face, t, b1, b2 (extension-queue order)
+    uint32_t* hitQ;  // [P] extension-queue indices that hit
+    uint32_t* missQ; // [P] extension-queue indices that missed
+    uint32_t* sqSlot;
+    float4* sqOrg;   // xyz origin, w target face (bits; kNone = environment)
+    float4* sqDir;
+    float4* sqC;     // weighted contribution
+    uint32_t* counts;
+    uint32_t* spill; // traversal stack overflow region, (64 - kLdsDepth) x lanes
+    uint32_t lanes;
+    uint32_t slots;  // P: capacity of every queue
+};
+
+struct WfArgs {
+    RScene S;
+    WfBufs B;
+    uint32_t width, height, maxDepth;
+    uint32_t* states;
+    float4* pixels;
+};
+
+size_t wfBytes(uint32_t P, uint32_t lanes) {
+    size_t b = 0;
+    auto add = [&](size_t x) { b += (x + 255) & ~size_t(255); };
+    add(16ull * P); add(16ull * P); add(16ull * P); add(16ull * P); add(4ull * P);
+    for (int q = 0; q < 2; ++q) { add(4ull * P); add(16ull * P); add(16ull * P); }
+    add(16ull * P); add(4ull * P); add(4ull * P);
+    add(4ull * P); add(16ull * P); add(16ull * P); add(16ull * P);
+    add(256);
+    add(4ull * (kStackDepth - kLdsDepth) * lanes);
+    return b;
+}
+
+WfBufs carveWf(void* pool, uint32_t P, uint32_t lanes) {
+    Carve c(pool);
+    WfBufs B;
+    B.rng = c.take<uint4>(P);
+    B.alpha = c.take<float4>(P);
+    B.pix = c.take<float4>(P);
+    B.acc = c.take<float4>(P);
+    B.meta = c.take<uint32_t>(P);
+    for (int q = 0; q < 2; ++q) {
+        B.rqSlot[q] = c.take<uint32_t>(P);
+        B.rqOrg[q] = c.take<float4>(P);
+        B.rqDir[q] = c.take<float4>(P);
+    }
+    B.hit = c.take<uint4>(P);
+    B.hitQ = c.take<uint32_t>(P);
+    B.missQ = c.take<uint32_t>(P);
+    B.sqSlot = c.take<uint32_t>(P);
+    B.sqOrg = c.take<float4>(P);
+    B.sqDir = c.take<float4>(P);
+    B.sqC = c.take<float4>(P);
+    B.counts = c.take<uint32_t>(64);
+    B.spill = c.take<uint32_t>((size_t)(kStackDepth - kLdsDepth) * lanes);
+    B.lanes = lanes;
+    B.slots = P;
+    return B;
+}
+
+// Wave-aggregated queue append: one atomic per wave; returns this lane's slot
+// (meaningful where `want` is set). Every active lane must call it.
+__device__ __forceinline__ uint32_t appendIndex(uint32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// Starts the next camera path of `pixel` (cameraRay consumes 4 draws).
+__device__ __forceinline__ void startPath(const RScene& S, uint32_t pixel, uint32_t width, Rng& rng, V3& org,
+                                          V3& dir, V3& alpha, float& initY, float& dirPDF) {
+    float w;
+    cameraRay(S, pixel % width, pixel / width, rng, org, dir, w, dirPDF);
+    alpha = v3s(1.0f) * w;
+    initY = lum(alpha);
+}
+
+// End of one sample of `slot`: acc += pix; start the next sample (returns
+// true with the new ray) or, after the last one, write the pixel and state.
+__device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint32_t slot, Rng& rng, V3 pix, V3& org,
+                                          V3& dir) {
+    const WfBufs& B = A.B;
+    float4 acc = B.acc[slot];
+    acc.x += pix.x;
+    acc.y += pix.y;
+    acc.z += pix.z;
+    const uint32_t left = __float_as_uint(acc.w) - 1u;
+    acc.w = __uint_as_float(left);
+    B.acc[slot] = acc;
+    const uint32_t pixel = B.counts[kCntChunk] + slot;
+    if (left > 0) {
+        V3 alpha;
+        float initY, dirPDF;
+        startPath(S, pixel, A.width, rng, org, dir, alpha, initY, dirPDF);
+        B.alpha[slot] = f4(alpha, initY);
+        B.pix[slot] = make_float4(0.0f, 0.0f, 0.0f, dirPDF);
+        B.meta[slot] = 0u;
+        B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+        return true;
+    }
+    float4 p = A.pixels[pixel];
+    p.x += acc.x;
+    p.y += acc.y;
+    p.z += acc.z;
+    A.pixels[pixel] = p;
+    reinterpret_cast<uint4*>(A.states)[pixel] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+    return false;
+}
+
+__device__ __forceinline__ void pushExt(const WfBufs& B, uint32_t q, bool want, uint32_t slot, V3 org, V3 dir) {
+    const uint32_t k = appendIndex(B.counts + q, want);
+    if (want) {
+        B.rqSlot[q][k] = slot;
+        B.rqOrg[q][k] = f4(org, 0.0f);
+        B.rqDir[q][k] = f4(dir, 0.0f);
+    }
+}
+
+// First camera path of every slot of the chunk [chunk, chunk + count).
+__global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp) {
+    const WfBufs& B = A.B;
+    RScene S = A.S;
+    finishScene(S);
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot == 0) {
+        B.counts[kCntExt0] = count;
+        B.counts[kCntExt1] = 0u;
+        B.counts[kCntShadow] = 0u;
+        B.counts[kCntHit] = 0u;
+        B.counts[kCntMiss] = 0u;
+        B.counts[kCntChunk] = chunk;
+    }
+    if (slot >= count) return;
+    const uint32_t pixel = chunk + slot;
+    const uint4 s = reinterpret_cast<const uint4*>(A.states)[pixel];
+    Rng rng{s.x, s.y, s.z, s.w};
+    V3 org, dir, alpha;
+    float initY, dirPDF;
+    startPath(S, pixel, A.width, rng, org, dir, alpha, initY, dirPDF);
+    B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+    B.alpha[slot] = f4(alpha, initY);
+    B.pix[slot] = make_float4(0.0f, 0.0f, 0.0f, dirPDF);
+    B.acc[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(spp));
+    B.meta[slot] = 0u;
+    B.rqSlot[0][slot] = slot;
+    B.rqOrg[0][slot] = f4(org, 0.0f);
+    B.rqDir[0][slot] = f4(dir, 0.0f);
+}
+
+// Closest hit of extension queue `in`; resets the counters the shade / miss /
+// shadow kernels of this iteration append to.
+__global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
+    extern __shared__ uint32_t s_stack[];
+    const WfBufs& B = A.B;
+    const RScene S = A.S;
+    const uint32_t out = in ^ 1u;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    if (gid == 0) {
+        B.counts[out] = 0u;
+        B.counts[kCntShadow] = 0u;
+    }
+    const uint32_t n = min(B.counts[in], B.slots);
+    const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
+    bool over = false;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t j = base + threadIdx.x;
+        const bool active = j < n;
+        bool isHit = false;
+        if (active) {
+            const V3 o = xyz(B.rqOrg[in][j]), d = xyz(B.rqDir[in][j]);
+            float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
+            TraceStats st{0u, 0u};
+            const uint32_t face = closestHit<false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2,
+                                                               TexAlpha{&S}, &over);
+            isHit = face != kNone && !isinf(t);
+            B.hit[j] = make_uint4(isHit ? face : kNone, __float_as_uint(t), __float_as_uint(b1),
+                                  __float_as_uint(b2));
+        }
+        const uint32_t kh = appendIndex(B.counts + kCntHit, active && isHit);
+        const uint32_t km = appendIndex(B.counts + kCntMiss, active && !isHit);
+        if (active) {
+            if (isHit) B.hitQ[kh] = j;
+            else B.missQ[km] = j;
+        }
+    }
+    if (over) atomicOr(S.status, kStatusStackOverflow);
+}
+
+// Surface vertices (hit queue of extension queue `in`).
+__global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
+    RScene S = A.S;
+    finishScene(S);
+    const WfBufs& B = A.B;
+    const uint32_t out = in ^ 1u;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t n = min(B.counts[kCntHit], B.slots);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        bool wantRay = false, wantShadow = false;
+        uint32_t slot = 0, target = kNone;
+        V3 org = v3z(), dir = v3z(), sorg = v3z(), sdir = v3z(), contrib = v3z();
+        if (active) {
+            const uint32_t j = B.hitQ[i];
+            slot = B.rqSlot[in][j];
+            org = xyz(B.rqOrg[in][j]);
+            dir = xyz(B.rqDir[in][j]);
+            const uint4 h = B.hit[j];
+            const uint4 rs = B.rng[slot];
+            Rng rng{rs.x, rs.y, rs.z, rs.w};
+            const float4 a4 = B.alpha[slot];
+            const float4 p4 = B.pix[slot];
+            V3 alpha = xyz(a4), pix = xyz(p4);
+            const float initY = a4.w;
+            float dirPDF = p4.w;
+            uint32_t meta = B.meta[slot];
+            uint32_t depth = meta & 0xFFu, sampledType = (meta >> 8) & 0xFFu;
+
+            SurfPt sp;
+            hitParams(S, org, dir, __uint_as_float(h.y), h.x, __uint_as_float(h.z), __uint_as_float(h.w), sp);
+            const clr_face F = S.g.faces[h.x];
+            const V3 vout = -dir;
+            EDF E;
+            if (F.lightPtr != 0xFFFFu) {
+                edfArea(S, F.lightPtr, sp, sp.u, sp.v, E);
+                float mis = 1.0f;
+                if (depth > 0 && (sampledType & kNonSingular)) {
+                    const V3 dd = sp.p - org;
+                    const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
+                    const float lightPDF = areaPdfOfFace(S, h.x) * d2 / absCosEDF(E, vout);
+                    mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+                }
+                if (depth == 0) pix += alpha * evalLe(E, vout);
+                else pix += (mis * alpha) * evalLe(E, vout);
+            }
+            bool live = true;
+            if (depth > 0) {
+                if (depth >= A.maxDepth) {
+                    live = false;
+                } else {
+                    const float cp = fminf(lum(alpha) * (1.0f / initY), 1.0f);
+                    if (rng.uniform() < cp) {
+                        alpha = alpha * (1.0f / cp);
+                        dirPDF *= cp;
+                    } else {
+                        live = false;
+                    }
+                }
+            }
+            if (live) {
+                BSDF Bs;
+                bsdfAlloc(S, F.matPtr, sp, Bs);
+                if (hasNonSpecular(Bs)) {
+                    const float lu = rng.uniform(), l0 = rng.uniform(), l1 = rng.uniform();
+                    SurfPt lp;
+                    bool atInf = false;
+                    float lightPDF;
+                    sampleLight(S, lu, l0, l1, lp, atInf, E, lightPDF);
+                    V3 vinL = lp.p - sp.p;
+                    float dist2 = dot(vinL, vinL);
+                    vinL = vinL * (1.0f / sqrtf(dist2));
+                    if (atInf) {
+                        vinL = lp.p;
+                        dist2 = 1.0f;
+                    }
+                    if (lightPDF > 0.0f && lightPDF < INFINITY) {
+                        const float fsPDF = evalFsPdf(Bs, vout, vinL) * absCosEDF(E, vinL) / dist2;
+                        const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
+                        const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(Bs.N, vinL)) / dist2 / lightPDF;
+                        contrib = wgt * ((alpha * evalLe(E, -vinL)) * evalFs(Bs, vout, vinL));
+                        sorg = sp.p + vinL * kRayEps;
+                        sdir = vinL;
+                        target = atInf ? kNone : lp.face;
+                        wantShadow = true;
+                    }
+                }
+                const float bc = rng.uniform(), b0 = rng.uniform(), b1 = rng.uniform();
+                V3 nd = v3z();
+                const V3 f = sampleFs(Bs, vout, bc, b0, b1, nd, dirPDF, sampledType);
+                if ((f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) || dirPDF == 0.0f) {
+                    live = false;
+                } else {
+                    alpha = alpha * (f * (fabsf(dot(Bs.N, nd)) / dirPDF));
+                    dir = nd;
+                    org = sp.p + dir * kRayEps;
+                    ++depth;
+                    wantRay = true;
+                }
+            }
+            meta = depth | (sampledType << 8);
+            if (!live && wantShadow) {
+                // The light sample's contribution comes first; the shadow
+                // kernel ends the sample.
+                meta |= kFlagEndAfterShadow;
+                B.pix[slot] = f4(pix, dirPDF);
+                B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+            } else if (!live) {
+                wantRay = endSample(A, S, slot, rng, pix, org, dir);
+                meta = 0u;
+            } else {
+                B.alpha[slot] = f4(alpha, initY);
+                B.pix[slot] = f4(pix, dirPDF);
+                B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+            }
+            if (live || wantShadow) B.meta[slot] = meta;
+        }
+        pushExt(B, out, active && wantRay, slot, org, dir);
+        const uint32_t ks = appendIndex(B.counts + kCntShadow, active && wantShadow);
+        if (active && wantShadow) {
+            B.sqSlot[ks] = slot;
+            B.sqOrg[ks] = f4(sorg, __uint_as_float(target));
+            B.sqDir[ks] = f4(sdir, 0.0f);
+            B.sqC[ks] = f4(contrib, 0.0f);
+        }
+    }
+}
+
+// Environment vertices (miss queue of extension queue `in`).
+__global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
+    RScene S = A.S;
+    finishScene(S);
+    const WfBufs& B = A.B;
+    const uint32_t out = in ^ 1u;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t n = min(B.counts[kCntMiss], B.slots);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        bool wantRay = false;
+        uint32_t slot = 0;
+        V3 org = v3z(), dir = v3z();
+        if (active) {
+            const uint32_t j = B.missQ[i];
+            slot = B.rqSlot[in][j];
+            dir = xyz(B.rqDir[in][j]);
+            const uint4 rs = B.rng[slot];
+            Rng rng{rs.x, rs.y, rs.z, rs.w};
+            const V3 alpha = xyz(B.alpha[slot]);
+            const float4 p4 = B.pix[slot];
+            V3 pix = xyz(p4);
+            const float dirPDF = p4.w;
+            const uint32_t meta = B.meta[slot];
+            const uint32_t depth = meta & 0xFFu, sampledType = (meta >> 8) & 0xFFu;
+            const V3 vout = -dir;
+            const float theta = acosf(clampf(dir.y, -1.0f, 1.0f));
+            const float phi = fmodf(atan2f(-dir.x, dir.z) + 2.0f * kPiF, 2 * kPiF);
+            const float u = phi / (2.0f * kPiF), v = theta / kPiF;
+            EDF E;
+            edfEnv(S, u, v, E);
+            float mis = 1.0f;
+            if ((sampledType & kNonSingular) != 0 && depth > 0) {
+                const float lightPDF = areaPdfEnv(S, u, v) * 1.0f / absCosEDF(E, vout);
+                mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+            }
+            pix += (mis * alpha) * evalLe(E, vout);
+            wantRay = endSample(A, S, slot, rng, pix, org, dir);
+        }
+        pushExt(B, out, active && wantRay, slot, org, dir);
+    }
+}
+
+// Shadow rays of this iteration; resets the hit / miss counters for the next
+// iteration's extend kernel.
+__global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
+    RScene S = A.S;
+    finishScene(S);
+    extern __shared__ uint32_t s_stack[];
+    const WfBufs& B = A.B;
+    const uint32_t out = in ^ 1u;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    if (gid == 0) {
+        B.counts[kCntHit] = 0u;
+        B.counts[kCntMiss] = 0u;
+    }
+    const uint32_t n = min(B.counts[kCntShadow], B.slots);
+    const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
+    bool over = false;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        bool wantRay = false;
+        uint32_t slot = 0;
+        V3 org = v3z(), dir = v3z();
+        if (active) {
+            slot = B.sqSlot[i];
+            const float4 o4 = B.sqOrg[i];
+            const uint32_t target = __float_as_uint(o4.w);
+            const V3 so = xyz(o4), sd = xyz(B.sqDir[i]);
+            bool visible;
+            if (target == kNone) {
+                visible = !anyHit<TexAlpha>(S.g, so, sd, stk, TexAlpha{&S}, &over);
+            } else {
+                float t = INFINITY, b1, b2;
+                TraceStats st{0u, 0u};
+                const uint32_t face = closestHit<false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2,
+                                                                   TexAlpha{&S}, &over);
+                visible = face != kNone && !isinf(t) && face == target;
+            }
+            float4 p4 = B.pix[slot];
+            if (visible) {
+                const float4 c = B.sqC[i];
+                p4.x += c.x;
+                p4.y += c.y;
+                p4.z += c.z;
+            }
+            if (B.meta[slot] & kFlagEndAfterShadow) {
+                const uint4 rs = B.rng[slot];
+                Rng rng{rs.x, rs.y, rs.z, rs.w};
+                wantRay = endSample(A, S, slot, rng, xyz(p4), org, dir);
+            } else {
+                B.pix[slot] = p4;
+            }
+        }
+        pushExt(B, out, active && wantRay, slot, org, dir);
+    }
+    if (over) atomicOr(S.status, kStatusStackOverflow);
+}
+
+struct Launch {
+    uint32_t grid, block;
+};
+
+}  // namespace
+
+// ---- host side --------------------------------------------------------------
+
+struct WfState {
+    clr_wavefront_config cfg{};
+    bool tuned = false;
+    hipStream_t capture = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<uint8_t> key;  // launch arguments the captured graph was made for
+    uint32_t* hostCount = nullptr;  // pinned
+    int numCUs = 0;
+};
+
+namespace {
+
+clr_wavefront_config defaultConfig() {
+    clr_wavefront_config c{};
+    c.traceBlock = 128;
+    c.traceWavesPerCU = 16;
+    c.shadeBlock = 128;
+    c.shadeWavesPerCU = 16;
+    c.itersPerGraph = 8;
+    c.checkEvery = 4;
+    c.maxSlots = 1u << 22;
+    c.useGraph = 1;
+    return c;
+}
+
+int validConfig(const clr_wavefront_config& c) {
+    auto okBlock = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    return okBlock(c.traceBlock) && okBlock(c.shadeBlock) && c.traceWavesPerCU >= 1 &&
+           c.traceWavesPerCU <= 32 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 32 &&
+           c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64;
+}
+
+int ensureWfState(clr_ctx* ctx) {
+    if (ctx->wf) return CLR_OK;
+    WfState* w = new (std::nothrow) WfState();
+    if (!w) return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "wavefront state");
+    w->cfg = defaultConfig();
+    int rc = hipCheck(ctx, hipDeviceGetAttribute(&w->numCUs, hipDeviceAttributeMultiprocessorCount, ctx->device),
+                      "CU count");
+    if (!rc) rc = hipCheck(ctx, hipStreamCreateWithFlags(&w->capture, hipStreamNonBlocking), "capture stream");
+    if (!rc) rc = hipCheck(ctx, hipHostMalloc(reinterpret_cast<void**>(&w->hostCount), 64, hipHostMallocDefault),
+                           "pinned counter");
+    ctx->wf = w;
+    return rc;
+}
+
+Launch traceLaunch(const WfState& w) {
+    const uint32_t b = w.cfg.traceBlock;
+    return {std::max(1u, (uint32_t)w.numCUs * w.cfg.traceWavesPerCU * 64u / b), b};
+}
+Launch shadeLaunch(const WfState& w) {
+    const uint32_t b = w.cfg.shadeBlock;
+    return {std::max(1u, (uint32_t)w.numCUs * w.cfg.shadeWavesPerCU * 64u / b), b};
+}
+
+void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStream_t st) {
+    const Launch tl = traceLaunch(w), sl = shadeLaunch(w);
+    const size_t lds = sizeof(uint32_t) * kLdsDepth * tl.block;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t in = it & 1u;
+        k_wf_extend<<<tl.grid, tl.block, lds, st>>>(A, in);
+        k_wf_shade<<<sl.grid, sl.block, 0, st>>>(A, in);
+        k_wf_miss<<<sl.grid, sl.block, 0, st>>>(A, in);
+        k_wf_shadow<<<tl.grid, tl.block, lds, st>>>(A, in);
+    }
+}
+
+void dropGraph(WfState& w) {
+    if (w.exec) (void)hipGraphExecDestroy(w.exec);
+    if (w.graph) (void)hipGraphDestroy(w.graph);
+    w.exec = nullptr;
+    w.graph = nullptr;
+    w.key.clear();
+}
+
+// Captures G iterations for these arguments unless the cached graph matches.
+int ensureGraph(clr_ctx* ctx, WfState& w, const WfArgs& A) {
+    std::vector<uint8_t> key(sizeof(WfArgs) + sizeof(clr_wavefront_config));
+    std::memcpy(key.data(), &A, sizeof(WfArgs));
+    std::memcpy(key.data() + sizeof(WfArgs), &w.cfg, sizeof(clr_wavefront_config));
+    if (w.exec && key == w.key) return CLR_OK;
+    dropGraph(w);
+    int rc = hipCheck(ctx, hipStreamBeginCapture(w.capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    enqueueIterations(A, w, w.cfg.itersPerGraph, w.capture);
+    hipGraph_t g = nullptr;
+    rc = hipCheck(ctx, hipStreamEndCapture(w.capture, &g), "end capture");
+    if (rc) return rc;
+    w.graph = g;
+    rc = hipCheck(ctx, hipGraphInstantiate(&w.exec, g, nullptr, nullptr, 0), "graph instantiate");
+    if (rc) {
+        dropGraph(w);
+        return rc;
+    }
+    w.key = key;
+    return CLR_OK;
+}
+
+}  // namespace
+
+void destroyWfState(clr_ctx* ctx) {
+    if (!ctx->wf) return;
+    WfState* w = ctx->wf;
+    dropGraph(*w);
+    if (w->capture) (void)hipStreamDestroy(w->capture);
+    if (w->hostCount) (void)hipHostFree(w->hostCount);
+    delete w;
+    ctx->wf = nullptr;
+}
+
+int wavefrontGetConfig(clr_ctx* ctx, clr_wavefront_config* cfg) {
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    *cfg = ctx->wf->cfg;
+    return CLR_OK;
+}
+
+int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg) {
+    if (!validConfig(*cfg)) return fail(ctx, CLR_ERR_INVALID_ARG, "invalid wavefront configuration");
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    ctx->wf->cfg = *cfg;
+    ctx->wf->tuned = true;
+    return CLR_OK;
+}
+
+int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, uint32_t* states,
+                    clr_float3* pixels, hipStream_t stream) {
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    WfState& w = *ctx->wf;
+    WfArgs A;
+    std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
+    rc = prepareRenderScene(ctx, sd, A.S, stream);
+    if (rc) return rc;
+    const uint64_t numPix = (uint64_t)rp.width * rp.height;
+    const uint32_t P = (uint32_t)std::min<uint64_t>(numPix, w.cfg.maxSlots);
+    const Launch tl = traceLaunch(w);
+    const uint32_t lanes = tl.grid * tl.block;
+    const size_t need = wfBytes(P, lanes);
+    if (ctx->wfPoolBytes < need) {
+        rc = hipCheck(ctx, hipDeviceSynchronize(), "wavefront pool regrow sync");
+        if (rc) return rc;
+        if (ctx->wfPool) (void)hipFree(ctx->wfPool);
+        ctx->wfPool = nullptr;
+        ctx->wfPoolBytes = 0;
+        dropGraph(w);
+        hipError_t e = hipMalloc(&ctx->wfPool, need);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "wavefront pool of " + std::to_string(need) + " bytes");
+        }
+        ctx->wfPoolBytes = need;
+    }
+    A.B = carveWf(ctx->wfPool, P, lanes);
+    A.width = rp.width;
+    A.height = rp.height;
+    A.maxDepth = rp.maxDepth;
+    A.states = states;
+    A.pixels = reinterpret_cast<float4*>(pixels);
+    // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).
+    const uint64_t bound = (uint64_t)rp.samplesPerPixel * (rp.maxDepth + 1ull) + 1ull;
+    const uint32_t G = w.cfg.itersPerGraph;
+    if (w.cfg.useGraph) {
+        rc = ensureGraph(ctx, w, A);
+        if (rc) return rc;
+    }
+    for (uint64_t chunk = 0; chunk < numPix; chunk += P) {
+        const uint32_t count = (uint32_t)std::min<uint64_t>(P, numPix - chunk);
+        k_wf_init<<<(count + 255) / 256, 256, 0, stream>>>(A, (uint32_t)chunk, count, rp.samplesPerPixel);
+        rc = hipCheck(ctx, hipGetLastError(), "wavefront init launch");
+        if (rc) return rc;
+        uint64_t launched = 0, batches = 0;
+        while (launched < bound) {
+            if (w.cfg.useGraph) {
+                rc = hipCheck(ctx, hipGraphLaunch(w.exec, stream), "graph launch");
+                if (rc) return rc;
+            } else {
+                enqueueIterations(A, w, G, stream);
+                rc = hipCheck(ctx, hipGetLastError(), "wavefront launch");
+                if (rc) return rc;
+            }
+            launched += G;
+            ++batches;
+            if (w.cfg.checkEvery && batches % w.cfg.checkEvery == 0 && launched < bound) {
+                // Early stop: G is even, so the next iteration reads queue 0.
+                rc = hipCheck(ctx, hipMemcpyAsync(w.hostCount, A.B.counts + kCntExt0, 4, hipMemcpyDeviceToHost, stream),
+                              "count readback");
+                if (!rc) rc = hipCheck(ctx, hipStreamSynchronize(stream), "count sync");
+                if (rc) return rc;
+                if (*w.hostCount == 0) break;
+            }
+        }
+    }
+    return CLR_OK;
+}
+
+int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
+                      clr_wavefront_config* out, hipStream_t stream) {
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    WfState& w = *ctx->wf;
+    const uint64_t numPix = (uint64_t)rp.width * rp.height;
+    uint32_t* states = nullptr;
+    clr_float3* pixels = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = hipCheck(ctx, hipMalloc(&states, 16 * numPix), "autotune states");
+    if (!rc) rc = hipCheck(ctx, hipMalloc(&pixels, 16 * numPix), "autotune pixels");
+    if (!rc) rc = hipCheck(ctx, hipEventCreate(&e0), "event");
+    if (!rc) rc = hipCheck(ctx, hipEventCreate(&e1), "event");
+    auto timeOne = [&](const clr_wavefront_config& c, float* ms) -> int {
+        w.cfg = c;
+        int r = seedPixelStates(ctx, rp.width, rp.height, states, stream);
+        // One untimed render: pool growth and graph capture happen here.
+        if (!r) r = renderWavefront(ctx, sd, rp, states, pixels, stream);
+        if (!r) r = hipCheck(ctx, hipEventRecord(e0, stream), "event record");
+        if (!r) r = renderWavefront(ctx, sd, rp, states, pixels, stream);
+        if (!r) r = hipCheck(ctx, hipEventRecord(e1, stream), "event record");
+        if (!r) r = hipCheck(ctx, hipEventSynchronize(e1), "event sync");
+        if (!r) r = hipCheck(ctx, hipEventElapsedTime(ms, e0, e1), "event time");
+        return r;
+    };
+    clr_wavefront_config best = w.tuned ? w.cfg : defaultConfig();
+    float bestMs = 0.0f;
+    if (!rc) rc = timeOne(best, &bestMs);
+    // Coordinate descent over the trace kernels' shape, the shade kernels'
+    // shape, then the graph length and the early-stop period.
+    auto tryVariants = [&](auto mutate, int count) {
+        for (int v = 0; v < count && !rc; ++v) {
+            clr_wavefront_config c = best;
+            mutate(c, v);
+            if (!validConfig(c) || std::memcmp(&c, &best, sizeof c) == 0) continue;
+            float ms = 0.0f;
+            rc = timeOne(c, &ms);
+            if (!rc && ms < bestMs) {
+                bestMs = ms;
+                best = c;
+            }
+        }
+    };
+    const uint32_t blocks[3] = {64, 128, 256}, waves[4] = {8, 12, 16, 24};
+    tryVariants([&](clr_wavefront_config& c, int v) { c.traceBlock = blocks[v / 4]; c.traceWavesPerCU = waves[v % 4]; }, 12);
+    tryVariants([&](clr_wavefront_config& c, int v) { c.shadeBlock = blocks[v / 4]; c.shadeWavesPerCU = waves[v % 4]; }, 12);
+    const uint32_t iters[3] = {4, 8, 16}, checks[3] = {0, 2, 8};
+    tryVariants([&](clr_wavefront_config& c, int v) { c.itersPerGraph = iters[v / 3]; c.checkEvery = checks[v % 3]; }, 9);
+    w.cfg = best;
+    w.tuned = true;
+    if (out) *out = best;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(stream);
+    if (states) (void)hipFree(states);
+    if (pixels) (void)hipFree(pixels);
+    return rc;
 }
 
 }  // namespace clr

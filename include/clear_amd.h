@@ -215,6 +215,32 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
 int clr_seed_pixel_states(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* randStates,
                           void* stream);
 
+/* ---- wavefront renderer tuning -------------------------------------------
+ * No counterpart in the reference, which launches one megakernel per tile
+ * (main.cpp:475-494). CLR_RENDER_WAVEFRONT runs the integrator as queue-driven
+ * kernels (extend, shade, miss, shadow) over one path slot per pixel in
+ * flight, replayed from a captured hipGraph of itersPerGraph iterations. */
+typedef struct clr_wavefront_config {
+    uint32_t traceBlock;      /* threads per block of the extend / shadow kernels: 64, 128, 256 */
+    uint32_t traceWavesPerCU; /* their grid: waves per CU, 1..32 */
+    uint32_t shadeBlock;      /* threads per block of the shade / miss kernels: 64, 128, 256 */
+    uint32_t shadeWavesPerCU; /* their grid: waves per CU, 1..32 */
+    uint32_t itersPerGraph;   /* iterations per captured graph: even, 2..64 */
+    uint32_t checkEvery;      /* graph launches between early-stop checks (a host sync);
+                                 0: never, the render stays fully asynchronous */
+    uint32_t maxSlots;        /* path slots (pixels in flight); larger images run in chunks */
+    uint32_t useGraph;        /* 1: replay the captured graph, 0: launch kernels directly */
+} clr_wavefront_config;
+
+int clr_wavefront_get_config(clr_ctx* ctx, clr_wavefront_config* config);
+int clr_wavefront_set_config(clr_ctx* ctx, const clr_wavefront_config* config);
+/* Times candidate configurations on this scene and image size (renders of
+ * `params->samplesPerPixel` samples into scratch copies; the caller's image
+ * and states are not touched), keeps the fastest in the context and returns
+ * it in `config` (may be NULL). Synchronises `stream`. */
+int clr_wavefront_autotune(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                           clr_wavefront_config* config, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
