@@ -61,6 +61,7 @@ def lib():
         "clr_lbvh_reserve": (i32, [vp, u32, u32]),
         "clr_lbvh_build_segmented": (i32, [vp, vp, ctypes.POINTER(u32), u32, u32, vp, vp, vp]),
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
+        "clr_trace_closest_ordered": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_gen_cluster_soup": (i32, [u32, i32, u32, f32, f32, f32, vp]),
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
         "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
@@ -199,11 +200,14 @@ class LBVHBuilder:
 
 
 def trace_closest(ctx: Context, internal_nodes, leaf_nodes, num_faces: int, vertices, faces,
-                  rays, num_rays: int, hit_face, hit_t, stats=None, stream=None) -> None:
-    ctx.check(lib().clr_trace_closest(
-        ctx.handle, _ptr(internal_nodes), _ptr(leaf_nodes), num_faces, _ptr(vertices),
-        _ptr(faces), _ptr(rays), num_rays, _ptr(hit_face), _ptr(hit_t), _ptr(stats),
-        _stream(stream)))
+                  rays, num_rays: int, hit_face, hit_t, stats=None, stream=None,
+                  ordered: bool = False) -> None:
+    """Closest hit per ray: the reference's walk (clr_trace_closest) or, with
+    ordered=True, the ordered traversal (clr_trace_closest_ordered)."""
+    fn = lib().clr_trace_closest_ordered if ordered else lib().clr_trace_closest
+    ctx.check(fn(ctx.handle, _ptr(internal_nodes), _ptr(leaf_nodes), num_faces, _ptr(vertices),
+                 _ptr(faces), _ptr(rays), num_rays, _ptr(hit_face), _ptr(hit_t), _ptr(stats),
+                 _stream(stream)))
 
 
 # ---- host scene generators ------------------------------------------------
@@ -361,9 +365,12 @@ def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None
     ctx.check(lib().clr_seed_pixel_states(ctx.handle, width, height, _ptr(states), _stream(stream)))
 
 
+RENDER_REFERENCE_TRAVERSAL = 1
+
+
 def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, pixels,
-           mode: int = RENDER_MEGAKERNEL, stream=None) -> None:
-    rp = RenderParams(scene.width, scene.height, spp, max_depth, 0)
+           mode: int = RENDER_MEGAKERNEL, stream=None, flags: int = 0) -> None:
+    rp = RenderParams(scene.width, scene.height, spp, max_depth, flags)
     ctx.check(lib().clr_render(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp), _ptr(states),
                                _ptr(pixels), mode, _stream(stream)))
 
@@ -389,10 +396,10 @@ def set_wavefront_config(ctx: Context, cfg: WavefrontConfig) -> None:
 
 
 def wavefront_autotune(ctx: Context, scene: "DeviceScene", spp: int, max_depth: int,
-                       stream=None) -> WavefrontConfig:
+                       stream=None, flags: int = 0) -> WavefrontConfig:
     """Times launch-shape candidates on this scene (scratch image and states)
     and keeps the fastest in the context."""
-    rp = RenderParams(scene.width, scene.height, spp, max_depth, 0)
+    rp = RenderParams(scene.width, scene.height, spp, max_depth, flags)
     c = WavefrontConfig()
     ctx.check(lib().clr_wavefront_autotune(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp),
                                            ctypes.byref(c), _stream(stream)))

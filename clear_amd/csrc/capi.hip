@@ -101,6 +101,23 @@ int ensureRenderScratch(clr_ctx* ctx, uint32_t numFaces) {
     return CLR_OK;
 }
 
+int ensureTraceSpill(clr_ctx* ctx, size_t bytes) {
+    if (ctx->traceSpillBytes >= bytes) return CLR_OK;
+    int rc = hipCheck(ctx, hipDeviceSynchronize(), "trace spill regrow sync");
+    if (rc) return rc;
+    if (ctx->traceSpill) (void)hipFree(ctx->traceSpill);
+    ctx->traceSpill = nullptr;
+    ctx->traceSpillBytes = 0;
+    hipError_t e = hipMalloc(&ctx->traceSpill, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->traceSpill = nullptr;
+        return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "trace spill");
+    }
+    ctx->traceSpillBytes = bytes;
+    return CLR_OK;
+}
+
 int ensureStatus(clr_ctx* ctx) {
     if (ctx->devStatus) return CLR_OK;
     hipError_t e = hipMalloc(&ctx->devStatus, 256);
@@ -184,6 +201,7 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->faceLight) (void)hipFree(ctx->faceLight);
         if (ctx->renderWords) (void)hipFree(ctx->renderWords);
         if (ctx->wfPool) (void)hipFree(ctx->wfPool);
+        if (ctx->traceSpill) (void)hipFree(ctx->traceSpill);
         destroyWfState(ctx);
         for (auto& row : ctx->events)
             for (auto& e : row)
@@ -342,6 +360,26 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
     return traceClosest(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
+                        hitFace, hitT, stats, static_cast<hipStream_t>(stream));
+}
+
+int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
+                              const clr_lbvh_leaf* leafNodes, uint32_t numFaces,
+                              const clr_float3* vertices, const clr_face* faces, const float* rays,
+                              uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
+                              void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (numFaces == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "numFaces must be >= 1");
+    if (!leafNodes || (numFaces > 1 && !internalNodes) || !vertices || !faces)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null scene buffer");
+    if (numRays > 0 && (!rays || !hitFace || !hitT))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null ray buffer");
+    if (!aligned16(leafNodes) || (internalNodes && !aligned16(internalNodes)) ||
+        !aligned16(vertices))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return traceOrdered(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
                         hitFace, hitT, stats, static_cast<hipStream_t>(stream));
 }
 

@@ -45,6 +45,9 @@ struct clr_ctx {
     void* wfPool = nullptr;
     size_t wfPoolBytes = 0;
     clr::WfState* wf = nullptr;
+    // Traversal-stack overflow region of the ordered trace kernel.
+    uint32_t* traceSpill = nullptr;
+    size_t traceSpillBytes = 0;
 };
 
 namespace clr {
@@ -119,6 +122,10 @@ void destroyWfState(clr_ctx* ctx);
 int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params);
 
 // ---- traversal (traverse.hip) ----
+int ensureTraceSpill(clr_ctx* ctx, size_t bytes);
+int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+                 const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
+                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream);
 int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays,
                  uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,

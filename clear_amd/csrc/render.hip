@@ -21,6 +21,7 @@ struct PathOut {
 };
 
 // One camera path for pixel (gx, gy) (pathtracer.cl:48-196).
+template <bool ORDERED>
 __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, Rng& rng,
                         uint32_t* stack, int stride) {
     V3 pix = v3z();
@@ -36,7 +37,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
     bool traceContinue = true;
     uint32_t sampledType = 0, depth = 0;
     V3 vout = v3z();
-    if (intersect(S, org, dir, stack, stride, sp)) {
+    if (intersect<ORDERED>(S, org, dir, stack, stride, sp)) {
         uint32_t face = sp.face;
         clr_face F = S.g.faces[face];
         vout = -dir;
@@ -61,7 +62,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                 }
                 const V3 sorg = sp.p + vinL * kRayEps;
                 SurfPt lsp;
-                const bool hit = intersect(S, sorg, vinL, stack, stride, lsp);
+                const bool hit = intersect<ORDERED>(S, sorg, vinL, stack, stride, lsp);
                 const bool visible = atInf ? !hit : (hit && lsp.face == lp.face);
                 // A zero or infinite light density (an environment sample at a
                 // pole, where sin(theta) = 0) contributes nothing.
@@ -85,7 +86,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
             dir = nd;
             org = sp.p + dir * kRayEps;
             ++depth;
-            if (!intersect(S, org, dir, stack, stride, sp)) break;
+            if (!intersect<ORDERED>(S, org, dir, stack, stride, sp)) break;
             face = sp.face;
             F = S.g.faces[face];
             vout = -dir;
@@ -136,6 +137,7 @@ namespace {
 
 constexpr int kTile = 8;  // 8x8 pixels per wave
 
+template <bool ORDERED>
 __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t width, uint32_t height,
                                                     uint32_t spp, uint32_t maxDepth,
                                                     uint32_t* __restrict__ states,
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
     dev::Rng rng{st.x, st.y, st.z, st.w};
     dev::V3 acc = dev::v3z();
     for (uint32_t k = 0; k < spp; ++k)
-        acc += dev::tracePath(S, gx, gy, maxDepth, rng, s_stack + threadIdx.x, 64);
+        acc += dev::tracePath<ORDERED>(S, gx, gy, maxDepth, rng, s_stack + threadIdx.x, 64);
     reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     float4 p = reinterpret_cast<float4*>(pixels)[pi];
     p.x += acc.x;
@@ -229,8 +231,12 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     int rc = prepareRenderScene(ctx, sd, S, stream);
     if (rc) return rc;
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
-    k_render_mega<<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, states,
-                                            pixels);
+    if (rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)
+        k_render_mega<false><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
+                                                       states, pixels);
+    else
+        k_render_mega<true><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
+                                                      states, pixels);
     return hipCheck(ctx, hipGetLastError(), "render launch");
 }
 

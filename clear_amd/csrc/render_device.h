@@ -346,14 +346,21 @@ struct TexAlpha {
     }
 };
 
-// rayIntersection with hit parameters. Returns false (face = kNone) on a miss.
+// rayIntersection with hit parameters (ORDERED: the ordered traversal).
+// Returns false (face = kNone) on a miss.
+template <bool ORDERED>
 __device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint32_t* stack, int stride,
                                           SurfPt& sp) {
     float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
     TraceStats st{0u, 0u};
     bool over = false;
-    const uint32_t face = closestHit<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1,
-                                                       &b2, TexAlpha{&S}, &over);
+    uint32_t face;
+    if constexpr (ORDERED)
+        face = orderedHit<false, false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
+                                                  TexAlpha{&S}, &over);
+    else
+        face = closestHit<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
+                                           TexAlpha{&S}, &over);
     if (over) atomicOr(S.status, kStatusStackOverflow);
     if (face == kNone || isinf(t)) {
         sp.face = kNone;

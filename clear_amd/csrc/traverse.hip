@@ -12,6 +12,8 @@
 // vertex records are fetched as 16-B vectors.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "internal.h"
 #include "traverse_device.h"
 
@@ -48,7 +50,67 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     }
 }
 
+// Ordered traversal, grid-stride over the rays with a fixed grid (waves per
+// CU x CUs) so that the traversal stack's overflow region is per lane:
+// kOrdLds slots in LDS ([slot][lane]), the rest in `spill`.
+constexpr int kOrdLds = 32;
+constexpr int kOrdBlock = 128;
+
+template <bool STATS>
+__global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
+    const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves, uint32_t n,
+    const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces, const float* __restrict__ rays,
+    uint32_t numRays, uint32_t* __restrict__ hitFace, float* __restrict__ hitT, uint32_t* __restrict__ stats,
+    uint32_t* __restrict__ status, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t s_stack[kOrdLds * kOrdBlock];
+    const uint32_t gid = blockIdx.x * kOrdBlock + threadIdx.x;
+    const uint32_t lanes = gridDim.x * kOrdBlock;
+    const dev::SpillStack<kOrdLds> stk{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
+    const dev::GeomScene g{inodes, leaves, n, verts, faces};
+    const size_t R = numRays;
+    bool over = false;
+    for (uint32_t r = gid; r < numRays; r += lanes) {
+        dev::V3 o = {rays[r], rays[R + r], rays[2 * R + r]};
+        dev::V3 d = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+        float t = rays[6 * R + r];
+        dev::TraceStats st{0u, 0u};
+        const uint32_t best = dev::orderedHit<STATS, false>(g, o, d, t, stk, st, nullptr, nullptr, dev::NoAlpha(),
+                                                            &over);
+        const bool hit = best != kEmpty && !isinf(t);
+        hitFace[r] = hit ? best : kEmpty;
+        hitT[r] = t;
+        if constexpr (STATS) {
+            stats[2 * (size_t)r] = st.visits;
+            stats[2 * (size_t)r + 1] = st.tests;
+        }
+    }
+    if (over) atomicOr(status, kStatusStackOverflow);
+}
+
 }  // namespace
+
+int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+                 const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
+                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream) {
+    if (numRays == 0) return CLR_OK;
+    int rc = ensureStatus(ctx);
+    if (rc) return rc;
+    int cus = 0;
+    rc = hipCheck(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
+    if (rc) return rc;
+    // 16 waves per CU (LDS: 16 KiB per 128-thread block).
+    const uint32_t grid = std::max(1u, std::min((uint32_t)cus * 8u, (numRays + kOrdBlock - 1) / kOrdBlock));
+    const size_t lanes = (size_t)grid * kOrdBlock;
+    rc = ensureTraceSpill(ctx, lanes * (dev::kStackDepth - kOrdLds) * sizeof(uint32_t));
+    if (rc) return rc;
+    if (stats)
+        k_trace_ordered<true><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
+                                                              hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill);
+    else
+        k_trace_ordered<false><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
+                                                               hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill);
+    return hipCheck(ctx, hipGetLastError(), "trace launch");
+}
 
 int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays,

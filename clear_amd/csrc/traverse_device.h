@@ -240,5 +240,122 @@ __device__ __forceinline__ bool anyHit(const GeomScene& g, V3 o, V3 d, const Sta
     return hit;
 }
 
+// ---- ordered traversal -------------------------------------------------------
+// Nearer child first with a padded slab test; ties in t go to the smaller face
+// index, so the hit is the lexicographic minimum of (t, face) over accepted
+// triangles whatever the visiting order. Per ray: reciprocal direction with
+// components below 1e-20 in magnitude replaced by +-1e-20; per axis
+// t0 = (min - o) * inv, t1 = (max - o) * inv; tmin = max(min(t0, t1) over axes,
+// 0), tmax = min(max(t0, t1) over axes); accepted when e = tmin * (1 - 2^-16)
+// <= min(tmax * (1 + 2^-16), t), with e kept as the entry distance and a miss
+// returned as NaN (every later "e <= t" fails, also for t = +inf). At a node
+// whose box passed, leaf children are tested nearer first (child 0 on equal
+// entries), then the nearer internal child is entered and the farther pushed;
+// a popped node's box is tested again against the current t.
+constexpr float kPadDown = 1.0f - 1.0f / 65536.0f;
+constexpr float kPadUp = 1.0f + 1.0f / 65536.0f;
+
+__device__ __forceinline__ float safeInv(float d) {
+    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
+}
+__device__ __forceinline__ float slabEnter(float4 mn, float4 mx, V3 o, V3 inv, float tBound) {
+    const float x0 = (mn.x - o.x) * inv.x, x1 = (mx.x - o.x) * inv.x;
+    const float y0 = (mn.y - o.y) * inv.y, y1 = (mx.y - o.y) * inv.y;
+    const float z0 = (mn.z - o.z) * inv.z, z1 = (mx.z - o.z) * inv.z;
+    const float tmin = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float tmax = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * kPadUp;
+    const float e = tmin * kPadDown;
+    return e <= fminf(tmax, tBound) ? e : __builtin_nanf("");
+}
+
+// ANY = true: stop at the first accepted triangle with a finite distance
+// (occlusion); the return value then only tells hit / miss.
+template <bool STATS, bool ANY, class Alpha = NoAlpha, class Stack = LdsStack>
+__device__ __forceinline__ uint32_t orderedHit(const GeomScene& g, V3 o, V3 d, float& t, const Stack& stack,
+                                               TraceStats& st, float* b1Out = nullptr, float* b2Out = nullptr,
+                                               const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
+    const V3 inv = {safeInv(d.x), safeInv(d.y), safeInv(d.z)};
+    uint32_t best = kNone;
+    float bb1 = 0.0f, bb2 = 0.0f;
+    bool done = false;
+    auto leaf = [&](uint32_t li) {
+        const uint32_t obj = reinterpret_cast<const uint32_t*>(g.leaves + li)[8];
+        if constexpr (STATS) ++st.tests;
+        float tt = t, b1, b2;
+        if (!triTest(g, o, d, obj, tt, b1, b2)) return;
+        if (tt == t && obj > best) return;
+        if (alpha.reject(obj, b1, b2)) return;
+        t = tt;
+        best = obj;
+        bb1 = b1;
+        bb2 = b2;
+        if (ANY && !isinf(tt)) done = true;
+    };
+    auto box = [&](bool isLeaf, uint32_t idx, float4& mn, float4& mx) {
+        const float4* p = isLeaf ? reinterpret_cast<const float4*>(g.leaves + idx)
+                                 : reinterpret_cast<const float4*>(g.inodes + idx);
+        mn = p[0];
+        mx = p[1];
+    };
+    if (g.n == 1) {
+        float4 mn, mx;
+        box(true, 0, mn, mx);
+        if (slabEnter(mn, mx, o, inv, t) <= t) leaf(0);
+    } else if (g.n > 1) {
+        float4 mn, mx;
+        box(false, 0, mn, mx);
+        if (slabEnter(mn, mx, o, inv, t) <= t) {
+            int depth = 0;
+            uint32_t node = 0;
+            bool over = false;
+            while (true) {
+                if constexpr (STATS) ++st.visits;
+                const uint4 c = reinterpret_cast<const uint4*>(g.inodes + node)[2];
+                const bool l0 = (c.x & 0xFFu) != 0, l1 = (c.x & 0xFF00u) != 0;
+                float4 mn0, mx0, mn1, mx1;
+                box(l0, c.y, mn0, mx0);
+                box(l1, c.z, mn1, mx1);
+                const float e0 = slabEnter(mn0, mx0, o, inv, t), e1 = slabEnter(mn1, mx1, o, inv, t);
+                const bool first1 = e1 < e0;
+                if (l0 || l1) {
+                    const uint32_t ca = first1 ? c.z : c.y, cb = first1 ? c.y : c.z;
+                    const bool la = first1 ? l1 : l0, lb = first1 ? l0 : l1;
+                    const float ea = first1 ? e1 : e0, eb = first1 ? e0 : e1;
+                    if (la && ea <= t) leaf(ca);
+                    if (!done && lb && eb <= t) leaf(cb);
+                    if (done) break;
+                }
+                const bool go0 = !l0 && e0 <= t, go1 = !l1 && e1 <= t;
+                if (go0 && go1) {
+                    const uint32_t nearC = first1 ? c.z : c.y, farC = first1 ? c.y : c.z;
+                    if (depth < kStackDepth) stack.put(depth++, farC);
+                    else over = true;
+                    node = nearC;
+                    continue;
+                }
+                if (go0 || go1) {
+                    node = go0 ? c.y : c.z;
+                    continue;
+                }
+                bool found = false;
+                while (depth > 0) {
+                    const uint32_t cand = stack.get(--depth);
+                    box(false, cand, mn, mx);
+                    if (slabEnter(mn, mx, o, inv, t) <= t) {
+                        node = cand;
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found) break;
+            }
+            if (overflow && over) *overflow = true;
+        }
+    }
+    if (b1Out) *b1Out = bb1;
+    if (b2Out) *b2Out = bb2;
+    return best;
+}
+
 }  // namespace dev
 }  // namespace clr

@@ -75,7 +75,7 @@ struct WfBufs {
 struct WfArgs {
     RScene S;
     WfBufs B;
-    uint32_t width, height, maxDepth;
+    uint32_t width, height, maxDepth, flags;
     uint32_t* states;
     float4* pixels;
 };
@@ -218,6 +218,7 @@ __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp
 
 // Closest hit of extension queue `in`; resets the counters the shade / miss /
 // shadow kernels of this iteration append to.
+template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     extern __shared__ uint32_t s_stack[];
     const WfBufs& B = A.B;
@@ -240,8 +241,11 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
             const V3 o = xyz(B.rqOrg[in][j]), d = xyz(B.rqDir[in][j]);
             float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
             TraceStats st{0u, 0u};
-            const uint32_t face = closestHit<false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2,
-                                                               TexAlpha{&S}, &over);
+            uint32_t face;
+            if constexpr (ORDERED)
+                face = orderedHit<false, false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
+            else
+                face = closestHit<false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
             isHit = face != kNone && !isinf(t);
             B.hit[j] = make_uint4(isHit ? face : kNone, __float_as_uint(t), __float_as_uint(b1),
                                   __float_as_uint(b2));
@@ -431,6 +435,7 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
 
 // Shadow rays of this iteration; resets the hit / miss counters for the next
 // iteration's extend kernel.
+template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
     RScene S = A.S;
     finishScene(S);
@@ -458,13 +463,22 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
             const uint32_t target = __float_as_uint(o4.w);
             const V3 so = xyz(o4), sd = xyz(B.sqDir[i]);
             bool visible;
+            float t = INFINITY, b1, b2;
+            TraceStats st{0u, 0u};
             if (target == kNone) {
-                visible = !anyHit<TexAlpha>(S.g, so, sd, stk, TexAlpha{&S}, &over);
+                if constexpr (ORDERED)
+                    visible = orderedHit<false, true, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S},
+                                                                &over) == kNone ||
+                              isinf(t);
+                else
+                    visible = !anyHit<TexAlpha>(S.g, so, sd, stk, TexAlpha{&S}, &over);
             } else {
-                float t = INFINITY, b1, b2;
-                TraceStats st{0u, 0u};
-                const uint32_t face = closestHit<false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2,
-                                                                   TexAlpha{&S}, &over);
+                uint32_t face;
+                if constexpr (ORDERED)
+                    face = orderedHit<false, false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S},
+                                                              &over);
+                else
+                    face = closestHit<false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
                 visible = face != kNone && !isinf(t) && face == target;
             }
             float4 p4 = B.pix[slot];
@@ -554,12 +568,15 @@ Launch shadeLaunch(const WfState& w) {
 void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStream_t st) {
     const Launch tl = traceLaunch(w), sl = shadeLaunch(w);
     const size_t lds = sizeof(uint32_t) * kLdsDepth * tl.block;
+    const bool ordered = (A.flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t in = it & 1u;
-        k_wf_extend<<<tl.grid, tl.block, lds, st>>>(A, in);
+        if (ordered) k_wf_extend<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_extend<false><<<tl.grid, tl.block, lds, st>>>(A, in);
         k_wf_shade<<<sl.grid, sl.block, 0, st>>>(A, in);
         k_wf_miss<<<sl.grid, sl.block, 0, st>>>(A, in);
-        k_wf_shadow<<<tl.grid, tl.block, lds, st>>>(A, in);
+        if (ordered) k_wf_shadow<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_shadow<false><<<tl.grid, tl.block, lds, st>>>(A, in);
     }
 }
 
@@ -654,6 +671,7 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     A.width = rp.width;
     A.height = rp.height;
     A.maxDepth = rp.maxDepth;
+    A.flags = rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL;
     A.states = states;
     A.pixels = reinterpret_cast<float4*>(pixels);
     // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).

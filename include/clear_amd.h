@@ -167,6 +167,24 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
                       const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
                       uint32_t* stats, void* stream);
 
+/* Same arguments and outputs as clr_trace_closest, with the ordered
+ * traversal: at each node both children's boxes are slab-tested (reciprocal
+ * direction, entry padded by 1 - 2^-16, exit by 1 + 2^-16, so no box holding
+ * an accepted intersection is rejected), the nearer child is entered first,
+ * and a triangle replaces the hit when its t is smaller, or equal with a
+ * smaller face index. The hit is therefore the lexicographic minimum of
+ * (t, face) over all faces the reference's triangle test accepts,
+ * independent of visiting order; it differs from clr_trace_closest only when
+ * two faces tie in t exactly (the reference keeps the last one its walk
+ * tests) or a ray grazes a box within rounding. stats count entered internal
+ * nodes and triangle tests of this traversal. No counterpart in the
+ * reference; it is the traversal the renderers use by default. */
+int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
+                              const clr_lbvh_leaf* leafNodes, uint32_t numFaces,
+                              const clr_float3* vertices, const clr_face* faces,
+                              const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
+                              uint32_t* stats, void* stream);
+
 /* ---- path tracing ---------------------------------------------------------- */
 
 /* Device-resident scene: the arguments of the reference kernel `pathtracing`
@@ -194,10 +212,18 @@ typedef struct clr_render_params {
     uint32_t maxDepth;        /* bounce cap: a path stops after maxDepth bounces
                                  (the reference's cap is ray.depth > 99, i.e. 100,
                                  pathtracer.cl:166) */
-    uint32_t flags;           /* reserved, 0 */
+    uint32_t flags;           /* CLR_RENDER_* flags below */
 } clr_render_params;
 
 enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
+
+/* clr_render_params.flags. By default rays use the ordered traversal of
+ * clr_trace_closest_ordered (nearer child first, ties to the smaller face
+ * index); this flag selects the reference's fixed-order walk and box test
+ * instead (bvh_traversal.cl:160-215). The two differ only where two
+ * triangles are hit at exactly the same distance or a ray grazes a box
+ * within rounding. */
+#define CLR_RENDER_REFERENCE_TRAVERSAL 1u
 
 /* Adds samplesPerPixel camera paths per pixel into `pixels` (clr_float3 per
  * pixel, row-major, not divided by the sample count: the caller scales, as

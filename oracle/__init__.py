@@ -78,6 +78,42 @@ def trace_closest(inodes, leaves, n, verts, faces, rays, stats=False):
     return (hit, t, st) if stats else (hit, t)
 
 
+def trace_ordered(inodes, leaves, n, verts, faces, rays, any_hit=False):
+    """Ordered traversal restatement: (face, t, stats[R, 2])."""
+    L = lib()
+    fn = L.orc_trace_ordered
+    fn.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint32, ctypes.c_int, vp, vp, vp]
+    R = rays.shape[1]
+    hit = np.zeros(R, dtype=np.uint32)
+    t = np.zeros(R, dtype=np.float32)
+    st = np.zeros((R, 2), dtype=np.uint32)
+    inodes = np.ascontiguousarray(inodes) if n > 1 else np.zeros((1, 48), np.uint8)
+    rc = fn(_p(inodes), _p(np.ascontiguousarray(leaves)), n, _p(np.ascontiguousarray(verts)),
+            _p(np.ascontiguousarray(faces)), _p(np.ascontiguousarray(rays, dtype=np.float32)), R,
+            int(any_hit), _p(hit), _p(t), _p(st))
+    if rc:
+        raise RuntimeError(f"orc_trace_ordered failed: {rc}")
+    return hit, t, st
+
+
+def trace_bruteforce_minid(verts, faces, rays):
+    """Brute force with the (t, smaller face) rule of the ordered traversal."""
+    L = lib()
+    fn = L.orc_trace_bruteforce_minid
+    fn.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, vp]
+    R = rays.shape[1]
+    n = faces.shape[0]
+    hit = np.zeros(R, dtype=np.uint32)
+    t = np.zeros(R, dtype=np.float32)
+    fn(_p(np.ascontiguousarray(verts)), _p(np.ascontiguousarray(faces)), n,
+       _p(np.ascontiguousarray(rays, dtype=np.float32)), R, _p(hit), _p(t))
+    return hit, t
+
+
 def trace_bruteforce(verts, faces, rays):
     R = rays.shape[1]
     n = faces.shape[0]
@@ -135,7 +171,7 @@ def seed_states(width, height):
 
 
 def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, states, pixels,
-           pix_begin=0, pix_end=None, threads=0):
+           pix_begin=0, pix_end=None, threads=0, flags=0):
     """Runs the render restatement over pixels [pix_begin, pix_end) (row-major),
     accumulating into `pixels` (float32 [H*W*4]) and advancing `states`."""
     L = lib()
@@ -143,7 +179,7 @@ def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, sta
         L.orc_render.restype = ctypes.c_int
         L.orc_render.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
-                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]
         L.orc_render._typed = True
     keep = [np.ascontiguousarray(arrays[k]) for k in
             ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
@@ -153,7 +189,7 @@ def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, sta
     ptrs = (ctypes.c_void_p * 11)(*[k.ctypes.data for k in keep])
     end = width * height if pix_end is None else pix_end
     rc = L.orc_render(ptrs, num_faces, width, height, spp, max_depth, _p(states), _p(pixels),
-                      pix_begin, end, threads)
+                      pix_begin, end, threads, flags)
     if rc:
         raise RuntimeError(f"orc_render failed: {rc}")
 

@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "../include/clear_amd.h"
+#include "ordered_traversal.h"
 
 namespace {
 
@@ -99,6 +100,7 @@ struct SceneC {
     const uint8_t* camera;
     const uint8_t* environment;
     const uint8_t* lightDist;
+    bool ordered;  // ordered traversal (ordered_traversal.h) instead of the reference's walk
 };
 
 inline V3 vtx(const clr_float3* a, uint32_t i) { return {a[i].x, a[i].y, a[i].z}; }
@@ -418,10 +420,32 @@ void hitParams(const SceneC& s, V3 org, V3 dir, float t, const Isect& is, Surfac
             sp->uDir = normalize(uDir);
     }
 }
-// rayIntersection (bvh_traversal.cl:182-215).
+// rayIntersection (bvh_traversal.cl:182-215), or the ordered traversal.
 bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
     float t = INFINITY;
     Isect best{kNoFace, 0, 0}, cur{};
+    if (s.ordered) {
+        const float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z};
+        auto tri = [&](uint32_t face, float bound, float* tt) {
+            float x = bound;
+            Isect tmp{};
+            if (!triTest(s, org, dir, face, &x, &tmp)) return false;
+            *tt = x;
+            return true;
+        };
+        orc_ordered::Counts cnt;
+        const uint32_t f = orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr);
+        const bool hit = f != kNoFace && !std::isinf(t);
+        if (hit) {
+            // The accepted triangle's barycentrics: the same test again.
+            float x = INFINITY;
+            triTest(s, org, dir, f, &x, &best);
+            hitParams(s, org, dir, t, best, sp);
+        } else {
+            sp->faceID = kNoFace;  // D4
+        }
+        return hit;
+    }
     auto tryLeaf = [&](uint32_t li) {
         const clr_lbvh_leaf& L = s.leaves[li];
         if (!boxTest(L.min, L.max, org, dir, t)) return;
@@ -1152,8 +1176,9 @@ int orc_seed_states(uint32_t width, uint32_t height, uint32_t* states) {
 // others, inodes, leaves. threads <= 0 uses all hardware threads.
 int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uint32_t height, uint32_t spp,
                uint32_t maxDepth, uint32_t* states, clr_float3* pixels, uint32_t pixBegin, uint32_t pixEnd,
-               int threads) {
+               int threads, uint32_t flags) {
     SceneC s{};
+    s.ordered = (flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
     s.vertices = (const clr_float3*)arrays[0];
     s.normals = (const clr_float3*)arrays[1];
     s.tangents = (const clr_float3*)arrays[2];

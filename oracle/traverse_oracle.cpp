@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "../include/clear_amd.h"
+#include "ordered_traversal.h"
 
 namespace {
 
@@ -136,6 +137,63 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
             stats[2 * (size_t)r + 0] = visits;
             stats[2 * (size_t)r + 1] = tests;
         }
+    }
+    return 0;
+}
+
+// Ordered traversal (ordered_traversal.h) with the geometric triangle test:
+// face, t and per-ray (visits, tests); anyHit != 0 stops at the first
+// accepted triangle (hitFace then only tells hit / miss).
+int orc_trace_ordered(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+                      const clr_float3* verts, const clr_face* faces, const float* rays, uint32_t numRays,
+                      int anyHit, uint32_t* hitFace, float* hitT, uint32_t* stats) {
+    for (uint32_t r = 0; r < numRays; ++r) {
+        const size_t R = numRays;
+        const float o[3] = {rays[r], rays[R + r], rays[2 * R + r]};
+        const float d[3] = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+        float t = rays[6 * R + r];
+        const V3 org = {o[0], o[1], o[2]}, dir = {d[0], d[1], d[2]};
+        auto tri = [&](uint32_t face, float bound, float* tt) {
+            float x = bound;
+            if (!triTest(verts, faces, org, dir, face, &x)) return false;
+            *tt = x;
+            return true;
+        };
+        orc_ordered::Counts cnt;
+        bool over = false;
+        const uint32_t best = orc_ordered::traverse(inodes, leaves, n, o, d, t, tri, cnt, anyHit != 0, &over);
+        if (over) return 2;
+        const bool hit = best != 0xFFFFFFFFu && !std::isinf(t);
+        hitFace[r] = hit ? best : 0xFFFFFFFFu;
+        hitT[r] = t;
+        if (stats) {
+            stats[2 * (size_t)r + 0] = cnt.visits;
+            stats[2 * (size_t)r + 1] = cnt.tests;
+        }
+    }
+    return 0;
+}
+
+// Brute force with the ordered traversal's rule: the lexicographic minimum of
+// (t, face) over all faces the triangle test accepts.
+int orc_trace_bruteforce_minid(const clr_float3* verts, const clr_face* faces, uint32_t n, const float* rays,
+                               uint32_t numRays, uint32_t* hitFace, float* hitT) {
+    for (uint32_t r = 0; r < numRays; ++r) {
+        const size_t R = numRays;
+        const V3 org = {rays[r], rays[R + r], rays[2 * R + r]};
+        const V3 dir = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+        float t = rays[6 * R + r];
+        uint32_t best = 0xFFFFFFFFu;
+        for (uint32_t f = 0; f < n; ++f) {
+            float x = t;
+            if (!triTest(verts, faces, org, dir, f, &x)) continue;
+            if (x == t && f > best) continue;
+            t = x;
+            best = f;
+        }
+        const bool hit = best != 0xFFFFFFFFu && !std::isinf(t);
+        hitFace[r] = hit ? best : 0xFFFFFFFFu;
+        hitT[r] = t;
     }
     return 0;
 }

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
 
 
-def gpu_render(ctx, host, spp, depth, calls=1, mode=ca.RENDER_MEGAKERNEL):
+def gpu_render(ctx, host, spp, depth, calls=1, mode=ca.RENDER_MEGAKERNEL, flags=0):
     import torch
 
     W, H = host.width, host.height
@@ -29,7 +29,7 @@ def gpu_render(ctx, host, spp, depth, calls=1, mode=ca.RENDER_MEGAKERNEL):
     states, pixels = ca.new_image(W, H)
     ca.seed_pixel_states(ctx, W, H, states)
     for _ in range(calls):
-        ca.render(ctx, ds, spp, depth, states, pixels, mode=mode)
+        ca.render(ctx, ds, spp, depth, states, pixels, mode=mode, flags=flags)
     torch.cuda.synchronize()
     assert ctx.device_status() == 0
     leaves = ds.leaves.cpu().numpy().reshape(-1, 48)
@@ -37,14 +37,14 @@ def gpu_render(ctx, host, spp, depth, calls=1, mode=ca.RENDER_MEGAKERNEL):
     return (pixels.cpu().numpy(), states.cpu().numpy().view(np.uint32), inodes, leaves)
 
 
-def oracle_render(host, spp, depth, calls=1):
+def oracle_render(host, spp, depth, calls=1, flags=0):
     W, H = host.width, host.height
     o = orc.lbvh_build(host.vertices, host.faces, 10)
     st = orc.seed_states(W, H)
     px = np.zeros(W * H * 4, np.float32)
     for _ in range(calls):
         orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, spp, depth, st, px,
-                   threads=THREADS)
+                   threads=THREADS, flags=flags)
     return px, st, o
 
 
@@ -128,6 +128,17 @@ def test_terrain_full_geometry(gpu_ctx):
     host = ca.HostScene(ca.SCENE_TERRAIN, 128, 72)
     print(f"[render parity] terrain full faces={host.num_faces} 128x72 spp=2 depth=8", flush=True)
     compare("terrain", host, 2, 8, gpu_render(gpu_ctx, host, 2, 8), oracle_render(host, 2, 8))
+
+
+@pytest.mark.parametrize("kind,w,h,spp,depth", [(ca.SCENE_CORNELL, 128, 128, 8, 5),
+                                                  (ca.SCENE_SHOWCASE, 96, 96, 4, 8)])
+def test_reference_traversal(gpu_ctx, kind, w, h, spp, depth):
+    """CLR_RENDER_REFERENCE_TRAVERSAL: the reference's walk and box test."""
+    host = ca.HostScene(kind, w, h)
+    f = ca.RENDER_REFERENCE_TRAVERSAL
+    print(f"[render parity] reference traversal scene={kind} {w}x{h}", flush=True)
+    compare(f"reftrav{kind}", host, spp, depth, gpu_render(gpu_ctx, host, spp, depth, flags=f),
+            oracle_render(host, spp, depth, flags=f))
 
 
 def test_render_rejects_bad_arguments(gpu_ctx):

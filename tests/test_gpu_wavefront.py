@@ -13,24 +13,24 @@ from test_gpu_render import compare, oracle_render
 pytestmark = pytest.mark.gpu
 
 
-def run(ctx, ds, spp, depth, mode, calls=1):
+def run(ctx, ds, spp, depth, mode, calls=1, flags=0):
     import torch
 
     states, pixels = ca.new_image(ds.width, ds.height)
     ca.seed_pixel_states(ctx, ds.width, ds.height, states)
     for _ in range(calls):
-        ca.render(ctx, ds, spp, depth, states, pixels, mode=mode)
+        ca.render(ctx, ds, spp, depth, states, pixels, mode=mode, flags=flags)
     torch.cuda.synchronize()
     assert ctx.device_status() == 0
     return pixels.cpu().numpy(), states.cpu().numpy().view(np.uint32)
 
 
-def same_as_megakernel(ctx, host, spp, depth, calls=1, cfg=None):
+def same_as_megakernel(ctx, host, spp, depth, calls=1, cfg=None, flags=0):
     ds = ca.DeviceScene(ctx, host)
     if cfg is not None:
         ca.set_wavefront_config(ctx, cfg)
-    wp, ws = run(ctx, ds, spp, depth, ca.RENDER_WAVEFRONT, calls)
-    mp, ms = run(ctx, ds, spp, depth, ca.RENDER_MEGAKERNEL, calls)
+    wp, ws = run(ctx, ds, spp, depth, ca.RENDER_WAVEFRONT, calls, flags)
+    mp, ms = run(ctx, ds, spp, depth, ca.RENDER_MEGAKERNEL, calls, flags)
     ok = (wp.view(np.uint32) == mp.view(np.uint32)).reshape(-1, 4)[:, :3].all(axis=1)
     print(f"[wavefront] {host.width}x{host.height} spp={spp}x{calls}: identical pixels {ok.mean():.6f}",
           flush=True)
@@ -63,6 +63,11 @@ def test_showcase(gpu_ctx, lens):
 def test_terrain_reduced(gpu_ctx):
     host = ca.HostScene(ca.SCENE_TERRAIN, 192, 108, detail=256)
     same_as_megakernel(gpu_ctx, host, 4, 8, cfg=default_cfg(gpu_ctx))
+
+
+def test_reference_traversal_showcase(gpu_ctx):
+    host = ca.HostScene(ca.SCENE_SHOWCASE, 96, 96)
+    same_as_megakernel(gpu_ctx, host, 4, 8, cfg=default_cfg(gpu_ctx), flags=ca.RENDER_REFERENCE_TRAVERSAL)
 
 
 def test_furnace_deep(gpu_ctx):

@@ -69,3 +69,56 @@ def test_stats_count_visits():
     rays = ca.gen_random_rays(500, seed=10)
     h, t, st = orc.trace_closest(o["inodes"], o["leaves"], 2000, v, f, rays, stats=True)
     assert (st[:, 0] >= 1).any() and (st[:, 1] >= (h != 0xFFFFFFFF)).all()
+
+
+# ---- ordered traversal (oracle/ordered_traversal.h) -------------------------
+
+def axis_and_bounded_rays(n_rays, seed):
+    rays = ca.gen_random_rays(n_rays, seed=seed)
+    k = n_rays // 4
+    rays[3:6, :k] = 0.0
+    rays[3 + (np.arange(k) % 3), np.arange(k)] = np.where(np.arange(k) % 2, 1.0, -1.0)
+    rays[6, k:2 * k] = np.float32(0.05)
+    return rays
+
+
+@pytest.mark.parametrize("n,seed", [(1, 3), (2, 4), (3, 9), (50, 5), (3000, 6), (20000, 7)])
+def test_ordered_equals_bruteforce_min_t_then_face(n, seed):
+    """The ordered walk never culls a box holding an accepted intersection:
+    faces and t equal the brute-force lexicographic minimum of (t, face)."""
+    soup, v, f = scene(n, seed)
+    o = orc.lbvh_build(v, f, 10)
+    rays = axis_and_bounded_rays(4000, seed + 100)
+    h, t, st = orc.trace_ordered(o["inodes"], o["leaves"], n, v, f, rays)
+    hb, tb = orc.trace_bruteforce_minid(v, f, rays)
+    np.testing.assert_array_equal(h, hb)
+    np.testing.assert_array_equal(t.view(np.uint32), tb.view(np.uint32))
+    if n > 1:
+        assert (st[:, 0] > 0).any()
+    # any-hit mode agrees on hit / miss
+    ha, _, _ = orc.trace_ordered(o["inodes"], o["leaves"], n, v, f, rays, any_hit=True)
+    np.testing.assert_array_equal(ha != 0xFFFFFFFF, h != 0xFFFFFFFF)
+
+
+def test_ordered_tie_goes_to_smaller_face():
+    """Two coincident triangles (same vertices, faces 0 and 1): the reference
+    walk keeps the last one it tests, the ordered walk the smaller index."""
+    tri = [0, 0, 1, 1, 0, 1, 0, 1, 1]
+    soup = np.float32([tri, tri, [5, 5, 5, 6, 5, 5, 5, 6, 5]])
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, 10)
+    rays = np.zeros((7, 1), np.float32)
+    rays[:, 0] = [0.2, 0.2, 0.0, 0.0, 0.0, 1.0, np.inf]
+    h, t, _ = orc.trace_ordered(o["inodes"], o["leaves"], 3, v, f, rays)
+    assert h[0] == 0 and t[0] == 1.0
+    hr, tr = orc.trace_closest(o["inodes"], o["leaves"], 3, v, f, rays)
+    assert tr[0] == 1.0 and hr[0] in (0, 1)
+
+
+def test_ordered_visits_fewer_nodes_on_clusters():
+    soup, v, f = scene(20000, 11)
+    o = orc.lbvh_build(v, f, 10)
+    rays = ca.gen_random_rays(4000, seed=12)
+    _, _, st_ref = orc.trace_closest(o["inodes"], o["leaves"], 20000, v, f, rays, stats=True)
+    _, _, st_ord = orc.trace_ordered(o["inodes"], o["leaves"], 20000, v, f, rays)
+    assert st_ord[:, 1].sum() < st_ref[:, 1].sum()  # fewer triangle tests
