@@ -202,6 +202,7 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->renderWords) (void)hipFree(ctx->renderWords);
         if (ctx->wfPool) (void)hipFree(ctx->wfPool);
         if (ctx->traceSpill) (void)hipFree(ctx->traceSpill);
+        if (ctx->accel) (void)hipFree(ctx->accel);
         destroyWfState(ctx);
         for (auto& row : ctx->events)
             for (auto& e : row)

@@ -45,6 +45,9 @@ struct clr_ctx {
     void* wfPool = nullptr;
     size_t wfPoolBytes = 0;
     clr::WfState* wf = nullptr;
+    // Traversal layout of the ordered walk (rebuilt by every call that uses it).
+    void* accel = nullptr;
+    size_t accelBytes = 0;
     // Traversal-stack overflow region of the ordered trace kernel.
     uint32_t* traceSpill = nullptr;
     size_t traceSpillBytes = 0;
@@ -123,6 +126,11 @@ int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_p
 
 // ---- traversal (traverse.hip) ----
 int ensureTraceSpill(clr_ctx* ctx, size_t bytes);
+// Builds the ordered walk's traversal layout from the drop-in nodes on `stream`
+// into context memory and returns its two arrays.
+int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+               const clr_float3* vertices, const clr_face* faces, const float4** anodes, const float4** atris,
+               hipStream_t stream);
 int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
                  const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
                  uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream);

@@ -207,7 +207,11 @@ int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hi
     if (sd.numLights)
         k_face_lights_fill<<<(sd.numLights + 255) / 256, 256, 0, stream>>>(sd.lights, sd.numLights,
                                                                            sd.numFaces, faceLight, envLight);
-    S.g = dev::GeomScene{sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces};
+    const float4 *anodes = nullptr, *atris = nullptr;
+    rc = buildAccel(ctx, sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, &anodes, &atris,
+                    stream);
+    if (rc) return rc;
+    S.g = dev::GeomScene{sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, anodes, atris};
     S.normals = sd.normals;
     S.tangents = sd.tangents;
     S.uvs = sd.uvs;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 
 #include "internal.h"
 #include "traverse_device.h"
@@ -50,6 +51,41 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     }
 }
 
+// Traversal layout (see GeomScene): node i gets both children's boxes and
+// references, face f its three vertices.
+__global__ void k_accel_build(const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
+                              uint32_t n, const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces,
+                              float4* __restrict__ anodes, float4* __restrict__ atris) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 < n) {
+        const uint4 c = reinterpret_cast<const uint4*>(inodes + i)[2];
+        float4 box[2][2];
+        uint32_t ref[2];
+        for (int k = 0; k < 2; ++k) {
+            const bool leaf = ((c.x >> (8 * k)) & 0xFFu) != 0;
+            const uint32_t ci = k ? c.z : c.y;
+            const float4* p = leaf ? reinterpret_cast<const float4*>(leaves + ci)
+                                   : reinterpret_cast<const float4*>(inodes + ci);
+            box[k][0] = p[0];
+            box[k][1] = p[1];
+            ref[k] = leaf ? (leaves[ci].objIdx | dev::kLeafRef) : ci;
+        }
+        float4* o = anodes + 4 * (size_t)i;
+        o[0] = make_float4(box[0][0].x, box[0][0].y, box[0][0].z, __uint_as_float(ref[0]));
+        o[1] = make_float4(box[0][1].x, box[0][1].y, box[0][1].z, __uint_as_float(ref[1]));
+        o[2] = make_float4(box[1][0].x, box[1][0].y, box[1][0].z, 0.0f);
+        o[3] = make_float4(box[1][1].x, box[1][1].y, box[1][1].z, 0.0f);
+    }
+    if (i < n) {
+        const uint32_t* f = reinterpret_cast<const uint32_t*>(faces + i);
+        float4* o = atris + 3 * (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            const float4 v = *reinterpret_cast<const float4*>(verts + f[k]);
+            o[k] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
+    }
+}
+
 // Ordered traversal, grid-stride over the rays with a fixed grid (waves per
 // CU x CUs) so that the traversal stack's overflow region is per lane:
 // kOrdLds slots in LDS ([slot][lane]), the rest in `spill`.
@@ -61,12 +97,13 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
     const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves, uint32_t n,
     const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces, const float* __restrict__ rays,
     uint32_t numRays, uint32_t* __restrict__ hitFace, float* __restrict__ hitT, uint32_t* __restrict__ stats,
-    uint32_t* __restrict__ status, uint32_t* __restrict__ spill) {
+    uint32_t* __restrict__ status, uint32_t* __restrict__ spill, const float4* __restrict__ anodes,
+    const float4* __restrict__ atris) {
     __shared__ uint32_t s_stack[kOrdLds * kOrdBlock];
     const uint32_t gid = blockIdx.x * kOrdBlock + threadIdx.x;
     const uint32_t lanes = gridDim.x * kOrdBlock;
     const dev::SpillStack<kOrdLds> stk{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
-    const dev::GeomScene g{inodes, leaves, n, verts, faces};
+    const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris};
     const size_t R = numRays;
     bool over = false;
     for (uint32_t r = gid; r < numRays; r += lanes) {
@@ -89,6 +126,32 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
 
 }  // namespace
 
+int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+               const clr_float3* vertices, const clr_face* faces, const float4** anodes, const float4** atris,
+               hipStream_t stream) {
+    const size_t nodeBytes = (((size_t)std::max(n, 2u) - 1) * 64 + 255) & ~size_t(255);
+    const size_t need = nodeBytes + (size_t)n * 48;
+    if (ctx->accelBytes < need) {
+        int rc = hipCheck(ctx, hipDeviceSynchronize(), "accel regrow sync");
+        if (rc) return rc;
+        if (ctx->accel) (void)hipFree(ctx->accel);
+        ctx->accel = nullptr;
+        ctx->accelBytes = 0;
+        hipError_t e = hipMalloc(&ctx->accel, need);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "traversal layout of " + std::to_string(need) + " bytes");
+        }
+        ctx->accelBytes = need;
+    }
+    float4* an = static_cast<float4*>(ctx->accel);
+    float4* at = reinterpret_cast<float4*>(static_cast<char*>(ctx->accel) + nodeBytes);
+    k_accel_build<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, vertices, faces, an, at);
+    *anodes = an;
+    *atris = at;
+    return hipCheck(ctx, hipGetLastError(), "traversal layout launch");
+}
+
 int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
                  const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
                  uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream) {
@@ -103,12 +166,17 @@ int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     const size_t lanes = (size_t)grid * kOrdBlock;
     rc = ensureTraceSpill(ctx, lanes * (dev::kStackDepth - kOrdLds) * sizeof(uint32_t));
     if (rc) return rc;
+    const float4 *anodes = nullptr, *atris = nullptr;
+    rc = buildAccel(ctx, inodes, leaves, n, vertices, faces, &anodes, &atris, stream);
+    if (rc) return rc;
     if (stats)
         k_trace_ordered<true><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
-                                                              hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill);
+                                                              hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
+                                                              anodes, atris);
     else
         k_trace_ordered<false><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
-                                                               hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill);
+                                                               hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
+                                                               anodes, atris);
     return hipCheck(ctx, hipGetLastError(), "trace launch");
 }
 

@@ -41,13 +41,23 @@ __device__ __forceinline__ V3 loadV3(const clr_float3* p, uint32_t i) {
     return {v.x, v.y, v.z};
 }
 
+// Traversal layout derived from the drop-in nodes for the ordered walk:
+// per internal node i, 4 x float4 = {min of child 0 | ref0, max of child 0 |
+// ref1, min of child 1, max of child 1} (refs in the .w lanes as bits: an
+// internal child's index, or a leaf child's face index | 0x80000000), and per
+// face its three vertices as float4s. One 64-B load per visited node and one
+// 48-B load per triangle test, where the drop-in layout needs two dependent
+// round trips for each.
 struct GeomScene {
     const clr_lbvh_inode* inodes;
     const clr_lbvh_leaf* leaves;
     uint32_t n;
     const clr_float3* verts;
     const clr_face* faces;
+    const float4* anodes = nullptr;  // [n - 1][4]
+    const float4* atris = nullptr;   // [n][3]
 };
+constexpr uint32_t kLeafRef = 0x80000000u;
 
 struct TraceStats {
     uint32_t visits, tests;
@@ -176,7 +186,12 @@ __device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, f
         stack.put(0, 0u);
         depth = 1;
         bool over = false;
+        uint32_t iters = 0;
         while (depth > 0) {
+            if (++iters > 2u * g.n + 2u) {  // malformed nodes: stop and flag
+                over = true;
+                break;
+            }
             --depth;
             const uint32_t idx = stack.get(depth);
             const float4* np = reinterpret_cast<const float4*>(g.inodes + idx);
@@ -221,7 +236,12 @@ __device__ __forceinline__ bool anyHit(const GeomScene& g, V3 o, V3 d, const Sta
     stack.put(0, 0u);
     depth = 1;
     bool over = false, hit = false;
+    uint32_t iters = 0;
     while (depth > 0 && !hit) {
+        if (++iters > 2u * g.n + 2u) {  // malformed nodes: stop and flag
+            over = true;
+            break;
+        }
         --depth;
         const uint32_t idx = stack.get(depth);
         const float4* np = reinterpret_cast<const float4*>(g.inodes + idx);
@@ -268,93 +288,154 @@ __device__ __forceinline__ float slabEnter(float4 mn, float4 mx, V3 o, V3 inv, f
     return e <= fminf(tmax, tBound) ? e : __builtin_nanf("");
 }
 
-// ANY = true: stop at the first accepted triangle with a finite distance
-// (occlusion); the return value then only tells hit / miss.
+// Moller-Trumbore on the packed triangle copy (same operations as triTest).
+__device__ __forceinline__ bool triTestPacked(const float4* tris, V3 o, V3 d, uint32_t faceIdx, float& t,
+                                              float& b1Out, float& b2Out) {
+    const float4* tp = tris + 3 * (size_t)faceIdx;
+    const float4 a = tp[0], b = tp[1], c = tp[2];
+    const V3 p0 = {a.x, a.y, a.z}, p1 = {b.x, b.y, b.z}, p2 = {c.x, c.y, c.z};
+    const V3 edge01 = p1 - p0;
+    const V3 edge02 = p2 - p0;
+    const V3 p = cross(d, edge02);
+    const float det = dot(edge01, p);
+    if (det > -0.000001f && det < 0.000001f) return false;
+    const float invDet = 1.0f / det;
+    const V3 dd = o - p0;
+    const float b1 = dot(dd, p) * invDet;
+    if (b1 < 0.0f || b1 > 1.0f) return false;
+    const V3 q = cross(dd, edge01);
+    const float b2 = dot(d, q) * invDet;
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
+    const float tt = dot(edge02, q) * invDet;
+    if (tt < 0.0f || tt > t) return false;
+    t = tt;
+    b1Out = b1;
+    b2Out = b2;
+    return true;
+}
+
+// Resumable ordered walk over the traversal layout: begin() sets up one ray,
+// each step() either visits one internal node (both child boxes, its leaf
+// children's triangles nearer first, then enter the nearer internal child and
+// push the farther) or pops one candidate and re-tests its box against the
+// current t. Same decisions, hits and counts as a walk run to completion, so
+// callers can interleave rays (refill idle lanes) without changing results.
+// any (set by begin): stop at the first accepted triangle with a finite
+// distance (occlusion queries).
+template <bool STATS, class Alpha, class Stack>
+struct Ordered {
+    const GeomScene& g;
+    const Stack& stack;
+    const Alpha& alpha;
+    V3 o, d, inv;
+    float t, b1, b2;
+    uint32_t best, node, steps;
+    int depth;
+    bool done, over, any;
+    TraceStats st;
+
+    __device__ __forceinline__ Ordered(const GeomScene& g_, const Stack& s_, const Alpha& a_)
+        : g(g_), stack(s_), alpha(a_) {}
+
+    __device__ __forceinline__ void leaf(uint32_t face) {
+        if constexpr (STATS) ++st.tests;
+        float tt = t, x1, x2;
+        if (!triTestPacked(g.atris, o, d, face, tt, x1, x2)) return;
+        if (tt == t && face > best) return;
+        if (alpha.reject(face, x1, x2)) return;
+        t = tt;
+        best = face;
+        b1 = x1;
+        b2 = x2;
+        if (any && !isinf(tt)) done = true;
+    }
+
+    __device__ __forceinline__ void begin(V3 o_, V3 d_, float tmax, bool anyHit = false) {
+        o = o_;
+        d = d_;
+        inv = {safeInv(d.x), safeInv(d.y), safeInv(d.z)};
+        t = tmax;
+        b1 = b2 = 0.0f;
+        best = kNone;
+        node = kNone;
+        depth = 0;
+        done = false;
+        over = false;
+        any = anyHit;
+        steps = 0;
+        st = {0u, 0u};
+        if (g.n == 0) {
+            done = true;
+        } else if (g.n == 1) {
+            const float4* lp = reinterpret_cast<const float4*>(g.leaves);
+            if (slabEnter(lp[0], lp[1], o, inv, t) <= t) leaf(reinterpret_cast<const uint32_t*>(lp + 2)[0]);
+            done = true;
+        } else {
+            const float4* np = reinterpret_cast<const float4*>(g.inodes);
+            if (slabEnter(np[0], np[1], o, inv, t) <= t) node = 0;
+            else done = true;
+        }
+    }
+
+    __device__ __forceinline__ void step() {
+        // A tree visits each node at most once and pops at most as often as
+        // it pushes: more than 2n steps means malformed nodes; stop and flag.
+        if (++steps > 2u * g.n + 2u) {
+            done = true;
+            over = true;
+            return;
+        }
+        if (node != kNone) {
+            if constexpr (STATS) ++st.visits;
+            const float4* np = g.anodes + 4 * (size_t)node;
+            const float4 lo0 = np[0], hi0 = np[1], lo1 = np[2], hi1 = np[3];
+            const uint32_t r0 = __float_as_uint(lo0.w), r1 = __float_as_uint(hi0.w);
+            const float e0 = slabEnter(lo0, hi0, o, inv, t), e1 = slabEnter(lo1, hi1, o, inv, t);
+            const bool l0 = (r0 & kLeafRef) != 0, l1 = (r1 & kLeafRef) != 0;
+            const bool first1 = e1 < e0;
+            if (l0 || l1) {
+                const uint32_t ra = first1 ? r1 : r0, rb = first1 ? r0 : r1;
+                const bool la = first1 ? l1 : l0, lb = first1 ? l0 : l1;
+                const float ea = first1 ? e1 : e0, eb = first1 ? e0 : e1;
+                if (la && ea <= t) leaf(ra & ~kLeafRef);
+                if (!done && lb && eb <= t) leaf(rb & ~kLeafRef);
+                if (done) return;
+            }
+            const bool go0 = !l0 && e0 <= t, go1 = !l1 && e1 <= t;
+            if (go0 && go1) {
+                if (depth < kStackDepth) stack.put(depth++, first1 ? r0 : r1);
+                else over = true;
+                node = first1 ? r1 : r0;
+            } else if (go0 || go1) {
+                node = go0 ? r0 : r1;
+            } else {
+                node = kNone;
+                if (depth == 0) done = true;
+            }
+            return;
+        }
+        const uint32_t cand = stack.get(--depth);
+        const float4* cp = reinterpret_cast<const float4*>(g.inodes + cand);
+        if (slabEnter(cp[0], cp[1], o, inv, t) <= t) node = cand;
+        else if (depth == 0) done = true;
+    }
+};
+
+// The ordered walk run to completion for one ray. Returns the face (kNone
+// when nothing was accepted) and updates t, b1, b2.
 template <bool STATS, bool ANY, class Alpha = NoAlpha, class Stack = LdsStack>
 __device__ __forceinline__ uint32_t orderedHit(const GeomScene& g, V3 o, V3 d, float& t, const Stack& stack,
                                                TraceStats& st, float* b1Out = nullptr, float* b2Out = nullptr,
                                                const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
-    const V3 inv = {safeInv(d.x), safeInv(d.y), safeInv(d.z)};
-    uint32_t best = kNone;
-    float bb1 = 0.0f, bb2 = 0.0f;
-    bool done = false;
-    auto leaf = [&](uint32_t li) {
-        const uint32_t obj = reinterpret_cast<const uint32_t*>(g.leaves + li)[8];
-        if constexpr (STATS) ++st.tests;
-        float tt = t, b1, b2;
-        if (!triTest(g, o, d, obj, tt, b1, b2)) return;
-        if (tt == t && obj > best) return;
-        if (alpha.reject(obj, b1, b2)) return;
-        t = tt;
-        best = obj;
-        bb1 = b1;
-        bb2 = b2;
-        if (ANY && !isinf(tt)) done = true;
-    };
-    auto box = [&](bool isLeaf, uint32_t idx, float4& mn, float4& mx) {
-        const float4* p = isLeaf ? reinterpret_cast<const float4*>(g.leaves + idx)
-                                 : reinterpret_cast<const float4*>(g.inodes + idx);
-        mn = p[0];
-        mx = p[1];
-    };
-    if (g.n == 1) {
-        float4 mn, mx;
-        box(true, 0, mn, mx);
-        if (slabEnter(mn, mx, o, inv, t) <= t) leaf(0);
-    } else if (g.n > 1) {
-        float4 mn, mx;
-        box(false, 0, mn, mx);
-        if (slabEnter(mn, mx, o, inv, t) <= t) {
-            int depth = 0;
-            uint32_t node = 0;
-            bool over = false;
-            while (true) {
-                if constexpr (STATS) ++st.visits;
-                const uint4 c = reinterpret_cast<const uint4*>(g.inodes + node)[2];
-                const bool l0 = (c.x & 0xFFu) != 0, l1 = (c.x & 0xFF00u) != 0;
-                float4 mn0, mx0, mn1, mx1;
-                box(l0, c.y, mn0, mx0);
-                box(l1, c.z, mn1, mx1);
-                const float e0 = slabEnter(mn0, mx0, o, inv, t), e1 = slabEnter(mn1, mx1, o, inv, t);
-                const bool first1 = e1 < e0;
-                if (l0 || l1) {
-                    const uint32_t ca = first1 ? c.z : c.y, cb = first1 ? c.y : c.z;
-                    const bool la = first1 ? l1 : l0, lb = first1 ? l0 : l1;
-                    const float ea = first1 ? e1 : e0, eb = first1 ? e0 : e1;
-                    if (la && ea <= t) leaf(ca);
-                    if (!done && lb && eb <= t) leaf(cb);
-                    if (done) break;
-                }
-                const bool go0 = !l0 && e0 <= t, go1 = !l1 && e1 <= t;
-                if (go0 && go1) {
-                    const uint32_t nearC = first1 ? c.z : c.y, farC = first1 ? c.y : c.z;
-                    if (depth < kStackDepth) stack.put(depth++, farC);
-                    else over = true;
-                    node = nearC;
-                    continue;
-                }
-                if (go0 || go1) {
-                    node = go0 ? c.y : c.z;
-                    continue;
-                }
-                bool found = false;
-                while (depth > 0) {
-                    const uint32_t cand = stack.get(--depth);
-                    box(false, cand, mn, mx);
-                    if (slabEnter(mn, mx, o, inv, t) <= t) {
-                        node = cand;
-                        found = true;
-                        break;
-                    }
-                }
-                if (!found) break;
-            }
-            if (overflow && over) *overflow = true;
-        }
-    }
-    if (b1Out) *b1Out = bb1;
-    if (b2Out) *b2Out = bb2;
-    return best;
+    Ordered<STATS, Alpha, Stack> w(g, stack, alpha);
+    w.begin(o, d, t, ANY);
+    while (!w.done) w.step();
+    t = w.t;
+    if (b1Out) *b1Out = w.b1;
+    if (b2Out) *b2Out = w.b2;
+    if (overflow && w.over) *overflow = true;
+    if constexpr (STATS) st = w.st;
+    return w.best;
 }
 
 }  // namespace dev

@@ -48,7 +48,10 @@ constexpr int kLdsDepth = 32;                   // traversal stack slots kept in
 constexpr uint32_t kFlagEndAfterShadow = 1u << 16;
 
 // Device-side counters (one 256-B line).
-enum : uint32_t { kCntExt0 = 0, kCntExt1 = 1, kCntShadow = 2, kCntHit = 3, kCntMiss = 4, kCntChunk = 8 };
+enum : uint32_t {
+    kCntExt0 = 0, kCntExt1 = 1, kCntShadow = 2, kCntHit = 3, kCntMiss = 4,
+    kCntChunk = 8,
+};
 
 struct WfBufs {
     uint4* rng;      // [P] xorshift128 state
@@ -75,7 +78,7 @@ struct WfBufs {
 struct WfArgs {
     RScene S;
     WfBufs B;
-    uint32_t width, height, maxDepth, flags;
+    uint32_t width, height, maxDepth, flags, refillMin;
     uint32_t* states;
     float4* pixels;
 };
@@ -132,8 +135,60 @@ __device__ __forceinline__ uint32_t appendIndex(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Work distribution of the persistent traversal kernels: each wave claims
+// chunks of kPoolChunk queue entries with one atomic and hands them to its
+// idle lanes (ballot + popcount ranks), refilling only once at least
+// kRefillMin lanes are idle (or none is busy) so that ray set-up runs with
+// most lanes together. All state is wave-uniform.
+// Work distribution of the persistent traversal kernels, without atomics:
+// each wave owns one contiguous range of the queue and hands its entries to
+// idle lanes (ballot + popcount ranks) once at least refillMin lanes are idle
+// (or none is busy), so that ray set-up runs with most lanes together.
+// Ranges are grouped by XCD (workgroups are dispatched to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8): the waves of one XCD own one
+// contiguous eighth of the queue, keeping neighbouring pixels' rays in one
+// L2. The grid is a multiple of 8 blocks.
+struct WavePool {
+    uint32_t next, end;
+    __device__ __forceinline__ explicit WavePool(uint32_t n) {
+        const uint32_t wavesPerBlock = blockDim.x / 64u;
+        const uint32_t blocksPerXcd = gridDim.x / 8u;
+        const uint32_t logicalBlock = (blockIdx.x % 8u) * blocksPerXcd + blockIdx.x / 8u;
+        const uint32_t wave = logicalBlock * wavesPerBlock + threadIdx.x / 64u;
+        const uint64_t waves = (uint64_t)gridDim.x * wavesPerBlock;
+        next = (uint32_t)((uint64_t)n * wave / waves);
+        end = (uint32_t)((uint64_t)n * (wave + 1u) / waves);
+    }
+    // Returns true in lanes that got an entry (index in `j`). Every active
+    // lane of the wave must call it.
+    __device__ __forceinline__ bool refill(uint32_t refillMin, bool busy, uint32_t& j) {
+        if (next >= end) return false;
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t nIdle = (uint32_t)__popcll(idle);
+        if (nIdle == 0 || (nIdle < refillMin && __ballot(busy) != 0)) return false;
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
+        const uint32_t take = min(nIdle, end - next);
+        const bool got = !busy && rank < take;
+        if (got) j = next + rank;
+        next += take;
+        return got;
+    }
+    __device__ __forceinline__ bool drained() const { return next >= end; }
+};
+
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// Path slots walk the image in 8x8 tiles (tile-major, row-major inside a
+// tile), as the megakernel's waves do, so that a wave's rays start coherent.
+// Returns false for positions of partial edge tiles outside the image.
+__device__ __forceinline__ bool tilePixel(uint32_t v, uint32_t width, uint32_t height, uint32_t& pixel) {
+    const uint32_t tilesX = (width + 7u) / 8u;
+    const uint32_t tile = v / 64u, in = v % 64u;
+    const uint32_t x = (tile % tilesX) * 8u + in % 8u, y = (tile / tilesX) * 8u + in / 8u;
+    pixel = y * width + x;
+    return x < width && y < height;
+}
 
 // Starts the next camera path of `pixel` (cameraRay consumes 4 draws).
 __device__ __forceinline__ void startPath(const RScene& S, uint32_t pixel, uint32_t width, Rng& rng, V3& org,
@@ -156,7 +211,8 @@ __device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint
     const uint32_t left = __float_as_uint(acc.w) - 1u;
     acc.w = __uint_as_float(left);
     B.acc[slot] = acc;
-    const uint32_t pixel = B.counts[kCntChunk] + slot;
+    uint32_t pixel;
+    tilePixel(B.counts[kCntChunk] + slot, A.width, A.height, pixel);
     if (left > 0) {
         V3 alpha;
         float initY, dirPDF;
@@ -186,34 +242,36 @@ __device__ __forceinline__ void pushExt(const WfBufs& B, uint32_t q, bool want, 
 }
 
 // First camera path of every slot of the chunk [chunk, chunk + count).
+// Clears the device counters before a chunk (separate launch: k_wf_init
+// appends to the first extension queue from every block).
+__global__ void k_wf_reset(WfArgs A, uint32_t chunk) {
+    const uint32_t i = threadIdx.x;
+    if (i < 64) A.B.counts[i] = i == kCntChunk ? chunk : 0u;
+}
+
+// First camera path of every in-image position of the chunk
+// [chunk, chunk + count) of tile-major positions.
 __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp) {
     const WfBufs& B = A.B;
     RScene S = A.S;
     finishScene(S);
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot == 0) {
-        B.counts[kCntExt0] = count;
-        B.counts[kCntExt1] = 0u;
-        B.counts[kCntShadow] = 0u;
-        B.counts[kCntHit] = 0u;
-        B.counts[kCntMiss] = 0u;
-        B.counts[kCntChunk] = chunk;
+    uint32_t pixel = 0;
+    const bool valid = slot < count && tilePixel(chunk + slot, A.width, A.height, pixel);
+    V3 org = v3z(), dir = v3z();
+    if (valid) {
+        const uint4 s = reinterpret_cast<const uint4*>(A.states)[pixel];
+        Rng rng{s.x, s.y, s.z, s.w};
+        V3 alpha;
+        float initY, dirPDF;
+        startPath(S, pixel, A.width, rng, org, dir, alpha, initY, dirPDF);
+        B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+        B.alpha[slot] = f4(alpha, initY);
+        B.pix[slot] = make_float4(0.0f, 0.0f, 0.0f, dirPDF);
+        B.acc[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(spp));
+        B.meta[slot] = 0u;
     }
-    if (slot >= count) return;
-    const uint32_t pixel = chunk + slot;
-    const uint4 s = reinterpret_cast<const uint4*>(A.states)[pixel];
-    Rng rng{s.x, s.y, s.z, s.w};
-    V3 org, dir, alpha;
-    float initY, dirPDF;
-    startPath(S, pixel, A.width, rng, org, dir, alpha, initY, dirPDF);
-    B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
-    B.alpha[slot] = f4(alpha, initY);
-    B.pix[slot] = make_float4(0.0f, 0.0f, 0.0f, dirPDF);
-    B.acc[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(spp));
-    B.meta[slot] = 0u;
-    B.rqSlot[0][slot] = slot;
-    B.rqOrg[0][slot] = f4(org, 0.0f);
-    B.rqDir[0][slot] = f4(dir, 0.0f);
+    pushExt(B, 0, valid, slot, org, dir);
 }
 
 // Closest hit of extension queue `in`; resets the counters the shade / miss /
@@ -233,6 +291,34 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     const uint32_t n = min(B.counts[in], B.slots);
     const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
     bool over = false;
+    if constexpr (ORDERED) {
+        // Persistent waves refilling idle lanes from a wave pool; hit / miss
+        // queues are built by k_wf_compact.
+        const TexAlpha ta{&S};
+        Ordered<false, TexAlpha, SpillStack<kLdsDepth>> w(S.g, stk, ta);
+        WavePool pool(n);
+        bool busy = false;
+        uint32_t j = 0;
+        while (true) {
+            if (pool.refill(A.refillMin, busy, j)) {
+                w.begin(xyz(B.rqOrg[in][j]), xyz(B.rqDir[in][j]), INFINITY);
+                busy = true;
+            }
+            if (__ballot(busy) == 0 && pool.drained()) break;
+            if (busy) {
+                if (!w.done) w.step();
+                if (w.done) {
+                    over |= w.over;
+                    const bool isHit = w.best != kNone && !isinf(w.t);
+                    B.hit[j] = make_uint4(isHit ? w.best : kNone, __float_as_uint(w.t), __float_as_uint(w.b1),
+                                          __float_as_uint(w.b2));
+                    busy = false;
+                }
+            }
+        }
+        if (over) atomicOr(S.status, kStatusStackOverflow);
+        return;
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
         const uint32_t j = base + threadIdx.x;
         const bool active = j < n;
@@ -241,11 +327,8 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
             const V3 o = xyz(B.rqOrg[in][j]), d = xyz(B.rqDir[in][j]);
             float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
             TraceStats st{0u, 0u};
-            uint32_t face;
-            if constexpr (ORDERED)
-                face = orderedHit<false, false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
-            else
-                face = closestHit<false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
+            const uint32_t face =
+                closestHit<false, TexAlpha>(S.g, o, d, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
             isHit = face != kNone && !isinf(t);
             B.hit[j] = make_uint4(isHit ? face : kNone, __float_as_uint(t), __float_as_uint(b1),
                                   __float_as_uint(b2));
@@ -451,6 +534,45 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
     const uint32_t n = min(B.counts[kCntShadow], B.slots);
     const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
     bool over = false;
+    if constexpr (ORDERED) {
+        // Persistent waves as in k_wf_extend; environment samples use the
+        // occlusion walk, area samples the closest hit compared with the
+        // light's face. Samples ending here are finished by k_wf_shadow_end.
+        const TexAlpha ta{&S};
+        Ordered<false, TexAlpha, SpillStack<kLdsDepth>> w(S.g, stk, ta);
+        WavePool pool(n);
+        bool busy = false;
+        uint32_t i = 0, target = kNone;
+        while (true) {
+            if (pool.refill(A.refillMin, busy, i)) {
+                const float4 o4 = B.sqOrg[i];
+                target = __float_as_uint(o4.w);
+                w.begin(xyz(o4), xyz(B.sqDir[i]), INFINITY, target == kNone);
+                busy = true;
+            }
+            if (__ballot(busy) == 0 && pool.drained()) break;
+            if (busy) {
+                if (!w.done) w.step();
+                if (w.done) {
+                    over |= w.over;
+                    const bool hit = w.best != kNone && !isinf(w.t);
+                    const bool visible = target == kNone ? !hit : (hit && w.best == target);
+                    if (visible) {
+                        const uint32_t slot = B.sqSlot[i];
+                        const float4 c = B.sqC[i];
+                        float4 p4 = B.pix[slot];
+                        p4.x += c.x;
+                        p4.y += c.y;
+                        p4.z += c.z;
+                        B.pix[slot] = p4;
+                    }
+                    busy = false;
+                }
+            }
+        }
+        if (over) atomicOr(S.status, kStatusStackOverflow);
+        return;
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
@@ -466,19 +588,10 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
             float t = INFINITY, b1, b2;
             TraceStats st{0u, 0u};
             if (target == kNone) {
-                if constexpr (ORDERED)
-                    visible = orderedHit<false, true, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S},
-                                                                &over) == kNone ||
-                              isinf(t);
-                else
-                    visible = !anyHit<TexAlpha>(S.g, so, sd, stk, TexAlpha{&S}, &over);
+                visible = !anyHit<TexAlpha>(S.g, so, sd, stk, TexAlpha{&S}, &over);
             } else {
-                uint32_t face;
-                if constexpr (ORDERED)
-                    face = orderedHit<false, false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S},
-                                                              &over);
-                else
-                    face = closestHit<false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
+                const uint32_t face =
+                    closestHit<false, TexAlpha>(S.g, so, sd, t, stk, st, &b1, &b2, TexAlpha{&S}, &over);
                 visible = face != kNone && !isinf(t) && face == target;
             }
             float4 p4 = B.pix[slot];
@@ -499,6 +612,51 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         pushExt(B, out, active && wantRay, slot, org, dir);
     }
     if (over) atomicOr(S.status, kStatusStackOverflow);
+}
+
+// Hit / miss queues of extension queue `in` from the ordered extend kernel's
+// records.
+__global__ __launch_bounds__(256) void k_wf_compact(WfArgs A, uint32_t in) {
+    const WfBufs& B = A.B;
+    const uint32_t n = min(B.counts[in], B.slots);
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t j = base + threadIdx.x;
+        const bool active = j < n;
+        const bool isHit = active && B.hit[j].x != kNone;
+        const uint32_t kh = appendIndex(B.counts + kCntHit, isHit);
+        const uint32_t km = appendIndex(B.counts + kCntMiss, active && !isHit);
+        if (active) {
+            if (isHit) B.hitQ[kh] = j;
+            else B.missQ[km] = j;
+        }
+    }
+}
+
+// Ends the samples whose last vertex had a light sample but no continuation
+// (their radiance is complete once the shadow kernel ran).
+__global__ __launch_bounds__(256) void k_wf_shadow_end(WfArgs A, uint32_t in) {
+    RScene S = A.S;
+    finishScene(S);
+    const WfBufs& B = A.B;
+    const uint32_t out = in ^ 1u;
+    const uint32_t n = min(B.counts[kCntShadow], B.slots);
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
+        const uint32_t i = base + threadIdx.x;
+        bool wantRay = false;
+        uint32_t slot = 0;
+        V3 org = v3z(), dir = v3z();
+        if (i < n) {
+            slot = B.sqSlot[i];
+            if (B.meta[slot] & kFlagEndAfterShadow) {
+                const uint4 rs = B.rng[slot];
+                Rng rng{rs.x, rs.y, rs.z, rs.w};
+                wantRay = endSample(A, S, slot, rng, xyz(B.pix[slot]), org, dir);
+            }
+        }
+        pushExt(B, out, wantRay, slot, org, dir);
+    }
 }
 
 struct Launch {
@@ -532,6 +690,7 @@ clr_wavefront_config defaultConfig() {
     c.checkEvery = 4;
     c.maxSlots = 1u << 22;
     c.useGraph = 1;
+    c.refillMin = 32;
     return c;
 }
 
@@ -539,7 +698,8 @@ int validConfig(const clr_wavefront_config& c) {
     auto okBlock = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     return okBlock(c.traceBlock) && okBlock(c.shadeBlock) && c.traceWavesPerCU >= 1 &&
            c.traceWavesPerCU <= 32 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 32 &&
-           c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64;
+           c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64 &&
+           c.refillMin >= 1 && c.refillMin <= 64;
 }
 
 int ensureWfState(clr_ctx* ctx) {
@@ -558,7 +718,8 @@ int ensureWfState(clr_ctx* ctx) {
 
 Launch traceLaunch(const WfState& w) {
     const uint32_t b = w.cfg.traceBlock;
-    return {std::max(1u, (uint32_t)w.numCUs * w.cfg.traceWavesPerCU * 64u / b), b};
+    const uint32_t g = std::max(1u, (uint32_t)w.numCUs * w.cfg.traceWavesPerCU * 64u / b);
+    return {(g + 7u) / 8u * 8u, b};  // whole XCD groups (WavePool)
 }
 Launch shadeLaunch(const WfState& w) {
     const uint32_t b = w.cfg.shadeBlock;
@@ -571,12 +732,20 @@ void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStr
     const bool ordered = (A.flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t in = it & 1u;
-        if (ordered) k_wf_extend<true><<<tl.grid, tl.block, lds, st>>>(A, in);
-        else k_wf_extend<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        if (ordered) {
+            k_wf_extend<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+            k_wf_compact<<<sl.grid, sl.block, 0, st>>>(A, in);
+        } else {
+            k_wf_extend<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        }
         k_wf_shade<<<sl.grid, sl.block, 0, st>>>(A, in);
         k_wf_miss<<<sl.grid, sl.block, 0, st>>>(A, in);
-        if (ordered) k_wf_shadow<true><<<tl.grid, tl.block, lds, st>>>(A, in);
-        else k_wf_shadow<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        if (ordered) {
+            k_wf_shadow<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+            k_wf_shadow_end<<<sl.grid, sl.block, 0, st>>>(A, in);
+        } else {
+            k_wf_shadow<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        }
     }
 }
 
@@ -648,7 +817,9 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
     rc = prepareRenderScene(ctx, sd, A.S, stream);
     if (rc) return rc;
-    const uint64_t numPix = (uint64_t)rp.width * rp.height;
+    // Tile-major positions: whole 8x8 tiles, edge-tile positions outside the
+    // image are skipped by k_wf_init.
+    const uint64_t numPix = (uint64_t)((rp.width + 7) / 8) * ((rp.height + 7) / 8) * 64;
     const uint32_t P = (uint32_t)std::min<uint64_t>(numPix, w.cfg.maxSlots);
     const Launch tl = traceLaunch(w);
     const uint32_t lanes = tl.grid * tl.block;
@@ -672,6 +843,7 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     A.height = rp.height;
     A.maxDepth = rp.maxDepth;
     A.flags = rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL;
+    A.refillMin = w.cfg.refillMin;
     A.states = states;
     A.pixels = reinterpret_cast<float4*>(pixels);
     // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).
@@ -683,6 +855,7 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     }
     for (uint64_t chunk = 0; chunk < numPix; chunk += P) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(P, numPix - chunk);
+        k_wf_reset<<<1, 64, 0, stream>>>(A, (uint32_t)chunk);
         k_wf_init<<<(count + 255) / 256, 256, 0, stream>>>(A, (uint32_t)chunk, count, rp.samplesPerPixel);
         rc = hipCheck(ctx, hipGetLastError(), "wavefront init launch");
         if (rc) return rc;
@@ -757,6 +930,8 @@ int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_pa
     const uint32_t blocks[3] = {64, 128, 256}, waves[4] = {8, 12, 16, 24};
     tryVariants([&](clr_wavefront_config& c, int v) { c.traceBlock = blocks[v / 4]; c.traceWavesPerCU = waves[v % 4]; }, 12);
     tryVariants([&](clr_wavefront_config& c, int v) { c.shadeBlock = blocks[v / 4]; c.shadeWavesPerCU = waves[v % 4]; }, 12);
+    const uint32_t refills[4] = {8, 16, 32, 64};
+    tryVariants([&](clr_wavefront_config& c, int v) { c.refillMin = refills[v]; }, 4);
     const uint32_t iters[3] = {4, 8, 16}, checks[3] = {0, 2, 8};
     tryVariants([&](clr_wavefront_config& c, int v) { c.itersPerGraph = iters[v / 3]; c.checkEvery = checks[v % 3]; }, 9);
     w.cfg = best;

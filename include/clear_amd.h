@@ -256,6 +256,8 @@ typedef struct clr_wavefront_config {
                                  0: never, the render stays fully asynchronous */
     uint32_t maxSlots;        /* path slots (pixels in flight); larger images run in chunks */
     uint32_t useGraph;        /* 1: replay the captured graph, 0: launch kernels directly */
+    uint32_t refillMin;       /* traversal kernels hand new rays to a wave's idle lanes once at
+                                 least this many are idle (1..64; 64: only when all are) */
 } clr_wavefront_config;
 
 int clr_wavefront_get_config(clr_ctx* ctx, clr_wavefront_config* config);
