@@ -44,7 +44,7 @@ namespace {
 
 using namespace dev;
 
-constexpr int kLdsDepth = 32;                   // traversal stack slots kept in LDS
+constexpr int kMinLdsDepth = 16;  // fewest traversal stack slots kept in LDS (spill sized for it)
 constexpr uint32_t kFlagEndAfterShadow = 1u << 16;
 
 // Device-side counters (one 256-B line).
@@ -70,7 +70,7 @@ struct WfBufs {
     float4* sqDir;
     float4* sqC;     // weighted contribution
     uint32_t* counts;
-    uint32_t* spill; // traversal stack overflow region, (64 - kLdsDepth) x lanes
+    uint32_t* spill; // traversal stack overflow region, (64 - kMinLdsDepth) x lanes
     uint32_t lanes;
     uint32_t slots;  // P: capacity of every queue
 };
@@ -91,7 +91,7 @@ size_t wfBytes(uint32_t P, uint32_t lanes) {
     add(16ull * P); add(4ull * P); add(4ull * P);
     add(4ull * P); add(16ull * P); add(16ull * P); add(16ull * P);
     add(256);
-    add(4ull * (kStackDepth - kLdsDepth) * lanes);
+    add(4ull * (kStackDepth - kMinLdsDepth) * lanes);
     return b;
 }
 
@@ -116,23 +116,35 @@ WfBufs carveWf(void* pool, uint32_t P, uint32_t lanes) {
     B.sqDir = c.take<float4>(P);
     B.sqC = c.take<float4>(P);
     B.counts = c.take<uint32_t>(64);
-    B.spill = c.take<uint32_t>((size_t)(kStackDepth - kLdsDepth) * lanes);
+    B.spill = c.take<uint32_t>((size_t)(kStackDepth - kMinLdsDepth) * lanes);
     B.lanes = lanes;
     B.slots = P;
     return B;
 }
 
-// Wave-aggregated queue append: one atomic per wave; returns this lane's slot
-// (meaningful where `want` is set). Every active lane must call it.
+// Block-aggregated queue append: one atomic per block and call (appends to
+// one address serialise in the memory system, so their number matters more
+// than their size); returns this lane's position (meaningful where `want` is
+// set). Every thread of the block must call it (blocks of at most 256).
 __device__ __forceinline__ uint32_t appendIndex(uint32_t* counter, bool want) {
+    __shared__ uint32_t s_part[5];
     const uint64_t mask = __ballot(want);
-    if (mask == 0) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t wave = threadIdx.x / 64u, lane = __lane_id();
+    if (lane == 0) s_part[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+            const uint32_t c = s_part[w];
+            s_part[w] = total;
+            total += c;
+        }
+        s_part[4] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = s_part[4] + s_part[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    __syncthreads();  // s_part is reused by the next call
+    return r;
 }
 
 // Work distribution of the persistent traversal kernels: each wave claims
@@ -276,7 +288,7 @@ __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp
 
 // Closest hit of extension queue `in`; resets the counters the shade / miss /
 // shadow kernels of this iteration append to.
-template <bool ORDERED>
+template <bool ORDERED, int LD>
 __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     extern __shared__ uint32_t s_stack[];
     const WfBufs& B = A.B;
@@ -289,13 +301,13 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
         B.counts[kCntShadow] = 0u;
     }
     const uint32_t n = min(B.counts[in], B.slots);
-    const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
+    const SpillStack<LD> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
     bool over = false;
     if constexpr (ORDERED) {
         // Persistent waves refilling idle lanes from a wave pool; hit / miss
         // queues are built by k_wf_compact.
         const TexAlpha ta{&S};
-        Ordered<false, TexAlpha, SpillStack<kLdsDepth>> w(S.g, stk, ta);
+        Ordered<false, TexAlpha, SpillStack<LD>> w(S.g, stk, ta);
         WavePool pool(n);
         bool busy = false;
         uint32_t j = 0;
@@ -518,7 +530,7 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
 
 // Shadow rays of this iteration; resets the hit / miss counters for the next
 // iteration's extend kernel.
-template <bool ORDERED>
+template <bool ORDERED, int LD>
 __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
     RScene S = A.S;
     finishScene(S);
@@ -532,14 +544,14 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         B.counts[kCntMiss] = 0u;
     }
     const uint32_t n = min(B.counts[kCntShadow], B.slots);
-    const SpillStack<kLdsDepth> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
+    const SpillStack<LD> stk{s_stack + threadIdx.x, (int)blockDim.x, B.spill + gid, lanes};
     bool over = false;
     if constexpr (ORDERED) {
         // Persistent waves as in k_wf_extend; environment samples use the
         // occlusion walk, area samples the closest hit compared with the
         // light's face. Samples ending here are finished by k_wf_shadow_end.
         const TexAlpha ta{&S};
-        Ordered<false, TexAlpha, SpillStack<kLdsDepth>> w(S.g, stk, ta);
+        Ordered<false, TexAlpha, SpillStack<LD>> w(S.g, stk, ta);
         WavePool pool(n);
         bool busy = false;
         uint32_t i = 0, target = kNone;
@@ -690,7 +702,8 @@ clr_wavefront_config defaultConfig() {
     c.checkEvery = 4;
     c.maxSlots = 1u << 22;
     c.useGraph = 1;
-    c.refillMin = 32;
+    c.refillMin = 16;
+    c.ldsDepth = 32;
     return c;
 }
 
@@ -699,7 +712,7 @@ int validConfig(const clr_wavefront_config& c) {
     return okBlock(c.traceBlock) && okBlock(c.shadeBlock) && c.traceWavesPerCU >= 1 &&
            c.traceWavesPerCU <= 32 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 32 &&
            c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64 &&
-           c.refillMin >= 1 && c.refillMin <= 64;
+           c.refillMin >= 1 && c.refillMin <= 64 && (c.ldsDepth == 16 || c.ldsDepth == 32);
 }
 
 int ensureWfState(clr_ctx* ctx) {
@@ -728,23 +741,28 @@ Launch shadeLaunch(const WfState& w) {
 
 void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStream_t st) {
     const Launch tl = traceLaunch(w), sl = shadeLaunch(w);
-    const size_t lds = sizeof(uint32_t) * kLdsDepth * tl.block;
     const bool ordered = (A.flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
+    const bool ld16 = w.cfg.ldsDepth == 16;
+    const size_t lds = sizeof(uint32_t) * (ld16 ? 16 : 32) * tl.block;
     for (uint32_t it = 0; it < iters; ++it) {
         const uint32_t in = it & 1u;
         if (ordered) {
-            k_wf_extend<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+            if (ld16) k_wf_extend<true, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
+            else k_wf_extend<true, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
             k_wf_compact<<<sl.grid, sl.block, 0, st>>>(A, in);
         } else {
-            k_wf_extend<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+            if (ld16) k_wf_extend<false, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
+            else k_wf_extend<false, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
         }
         k_wf_shade<<<sl.grid, sl.block, 0, st>>>(A, in);
         k_wf_miss<<<sl.grid, sl.block, 0, st>>>(A, in);
         if (ordered) {
-            k_wf_shadow<true><<<tl.grid, tl.block, lds, st>>>(A, in);
+            if (ld16) k_wf_shadow<true, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
+            else k_wf_shadow<true, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
             k_wf_shadow_end<<<sl.grid, sl.block, 0, st>>>(A, in);
         } else {
-            k_wf_shadow<false><<<tl.grid, tl.block, lds, st>>>(A, in);
+            if (ld16) k_wf_shadow<false, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
+            else k_wf_shadow<false, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
         }
     }
 }
@@ -932,6 +950,10 @@ int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_pa
     tryVariants([&](clr_wavefront_config& c, int v) { c.shadeBlock = blocks[v / 4]; c.shadeWavesPerCU = waves[v % 4]; }, 12);
     const uint32_t refills[4] = {8, 16, 32, 64};
     tryVariants([&](clr_wavefront_config& c, int v) { c.refillMin = refills[v]; }, 4);
+    tryVariants([&](clr_wavefront_config& c, int v) {
+        c.ldsDepth = v < 4 ? 16u : 32u;
+        c.traceWavesPerCU = (v % 4 + 1) * 8u;
+    }, 8);
     const uint32_t iters[3] = {4, 8, 16}, checks[3] = {0, 2, 8};
     tryVariants([&](clr_wavefront_config& c, int v) { c.itersPerGraph = iters[v / 3]; c.checkEvery = checks[v % 3]; }, 9);
     w.cfg = best;

@@ -85,9 +85,10 @@ def test_chunks_direct_launches_and_calls(gpu_ctx):
 
 def test_graph_shapes(gpu_ctx):
     host = ca.HostScene(ca.SCENE_CORNELL, 61, 13, detail=2)
-    for tb, sb, g, chk in [(64, 256, 2, 1), (256, 64, 16, 0), (128, 128, 64, 3)]:
+    for tb, sb, g, chk, ld, rf in [(64, 256, 2, 1, 16, 1), (256, 64, 16, 0, 32, 64),
+                                   (128, 128, 64, 3, 16, 24)]:
         cfg = default_cfg(gpu_ctx, traceBlock=tb, shadeBlock=sb, itersPerGraph=g, checkEvery=chk,
-                          traceWavesPerCU=4, shadeWavesPerCU=4)
+                          traceWavesPerCU=4, shadeWavesPerCU=4, ldsDepth=ld, refillMin=rf)
         same_as_megakernel(gpu_ctx, host, 4, 5, cfg=cfg)
 
 
@@ -114,5 +115,8 @@ def test_rejects_bad_config(gpu_ctx):
     with pytest.raises(ca.ClrError):
         ca.set_wavefront_config(gpu_ctx, cfg)
     cfg = default_cfg(gpu_ctx, traceBlock=96)
+    with pytest.raises(ca.ClrError):
+        ca.set_wavefront_config(gpu_ctx, cfg)
+    cfg = default_cfg(gpu_ctx, ldsDepth=24)
     with pytest.raises(ca.ClrError):
         ca.set_wavefront_config(gpu_ctx, cfg)
