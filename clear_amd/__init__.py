@@ -66,6 +66,9 @@ def lib():
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
         "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
+        "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "clr_scene_diff_params": (i32, [vp, ctypes.POINTER(u32), vp, vp, vp, ctypes.POINTER(u32),
+                                        ctypes.POINTER(u32)]),
         "clr_wavefront_get_config": (i32, [vp, vp]),
         "clr_wavefront_set_config": (i32, [vp, vp]),
         "clr_wavefront_autotune": (i32, [vp, vp, vp, vp, vp]),
@@ -274,7 +277,7 @@ def decode_leaves(buf: np.ndarray, n: int):
 
 # ---- scenes and rendering -------------------------------------------------
 
-SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN = 0, 1, 2, 3
+SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN, SCENE_ADJOINT = 0, 1, 2, 3, 4
 ARRAY_NAMES = ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials",
                "textures", "others", "bvh_nodes"]
 RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1
@@ -329,6 +332,15 @@ class HostScene:
                 if n:
                     ctypes.memmove(buf.ctypes.data, ptr.value, n)
                 self.arrays[name] = buf
+            nm = ctypes.c_uint32(0)
+            lib().clr_scene_diff_params(h, ctypes.byref(nm), None, None, None, None, None)
+            m = nm.value
+            mo, ao, so = (np.zeros(max(m, 1), np.uint32) for _ in range(3))
+            lp, eo = ctypes.c_uint32(), ctypes.c_uint32()
+            lib().clr_scene_diff_params(h, ctypes.byref(nm), mo.ctypes.data, ao.ctypes.data,
+                                        so.ctypes.data, ctypes.byref(lp), ctypes.byref(eo))
+            self.diff = {"mat_offsets": mo[:m], "albedo_offsets": ao[:m], "sigma_offsets": so[:m],
+                         "light_prop": lp.value, "emission_offset": eo.value}
         finally:
             lib().clr_scene_free(h)
         self.faces = self.arrays["faces"][: self.num_faces * 56].view(FACE_DTYPE)
@@ -366,6 +378,25 @@ def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None
 
 
 RENDER_REFERENCE_TRAVERSAL = 1
+RENDER_NO_RR = 2
+
+
+class DiffParams(ctypes.Structure):
+    _fields_ = [("numMaterials", ctypes.c_uint32), ("matOffsets", ctypes.c_void_p),
+                ("lightProp", ctypes.c_uint32)]
+
+
+def render_adjoint(ctx: Context, scene: "DeviceScene", spp: int, max_depth: int, states, d_image,
+                   mat_offsets: np.ndarray, light_prop: int, g_albedo, g_sigma, g_emission,
+                   stream=None, flags: int = 0) -> None:
+    """clr_render_adjoint: gradients of sum(d_image * image) with image = the
+    radiance a clr_render call from `states` adds, divided by spp."""
+    rp = RenderParams(scene.width, scene.height, spp, max_depth, flags)
+    mo = np.ascontiguousarray(mat_offsets, dtype=np.uint32)
+    dp = DiffParams(len(mo), mo.ctypes.data if len(mo) else None, light_prop)
+    ctx.check(lib().clr_render_adjoint(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp),
+                                       ctypes.byref(dp), _ptr(states), _ptr(d_image), _ptr(g_albedo),
+                                       _ptr(g_sigma), _ptr(g_emission), _stream(stream)))
 
 
 def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, pixels,

@@ -203,6 +203,7 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->wfPool) (void)hipFree(ctx->wfPool);
         if (ctx->traceSpill) (void)hipFree(ctx->traceSpill);
         if (ctx->accel) (void)hipFree(ctx->accel);
+        if (ctx->adjTable) (void)hipFree(ctx->adjTable);
         destroyWfState(ctx);
         for (auto& row : ctx->events)
             for (auto& e : row)
@@ -405,6 +406,30 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
     if (mode == CLR_RENDER_MEGAKERNEL) return renderMegakernel(ctx, *scene, *params, randStates, pixels, st);
     if (mode == CLR_RENDER_WAVEFRONT) return renderWavefront(ctx, *scene, *params, randStates, pixels, st);
     return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
+}
+
+int clr_render_adjoint(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                       const clr_diff_params* diff, const uint32_t* randStates, const clr_float3* dL_dimage,
+                       float* gAlbedo, float* gRoughness, float* gEmission, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    int rc = checkRenderArgs(ctx, scene, params);
+    if (rc) return rc;
+    if (!diff || !randStates || !dL_dimage) return fail(ctx, CLR_ERR_INVALID_ARG, "null adjoint argument");
+    if (!aligned16(randStates) || !aligned16(dL_dimage))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
+    if (diff->numMaterials > 256) return fail(ctx, CLR_ERR_UNSUPPORTED, "at most 256 differentiable materials");
+    if (diff->numMaterials && (!diff->matOffsets || !gAlbedo || !gRoughness))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null material list or gradient output");
+    for (uint32_t k = 0; k < diff->numMaterials; ++k)
+        if (diff->matOffsets[k] > 0xFFFFu) return fail(ctx, CLR_ERR_INVALID_ARG, "material offset beyond 16 bits");
+    if (diff->lightProp != 0xFFFFFFFFu && (diff->lightProp > 0xFFFFu || !gEmission))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "emitter offset beyond 16 bits or null emission gradient");
+    if (params->maxDepth > 15) return fail(ctx, CLR_ERR_UNSUPPORTED, "adjoint supports maxDepth <= 15");
+    if (params->samplesPerPixel == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "samplesPerPixel must be >= 1");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return renderAdjoint(ctx, *scene, *params, *diff, randStates, dL_dimage, gAlbedo, gRoughness,
+                         diff->lightProp != 0xFFFFFFFFu ? gEmission : nullptr, static_cast<hipStream_t>(stream));
 }
 
 int clr_wavefront_get_config(clr_ctx* ctx, clr_wavefront_config* config) {

@@ -45,6 +45,8 @@ struct clr_ctx {
     void* wfPool = nullptr;
     size_t wfPoolBytes = 0;
     clr::WfState* wf = nullptr;
+    // Adjoint: matPtr -> parameter table (65536 x uint16) + 256 offsets.
+    void* adjTable = nullptr;
     // Traversal layout of the ordered walk (rebuilt by every call that uses it).
     void* accel = nullptr;
     size_t accelBytes = 0;
@@ -116,6 +118,9 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
                     uint32_t* states, clr_float3* pixels, hipStream_t stream);
 int seedPixelStates(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* states,
                     hipStream_t stream);
+int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, const clr_diff_params& dp,
+                  const uint32_t* states, const clr_float3* dLdI, float* gAlb, float* gSig, float* gEm,
+                  hipStream_t stream);
 int wavefrontGetConfig(clr_ctx* ctx, clr_wavefront_config* cfg);
 int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg);
 int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,

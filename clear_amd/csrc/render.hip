@@ -16,14 +16,19 @@
 namespace clr {
 namespace dev {
 
-struct PathOut {
-    V3 L;
+// Observer of a path's radiance terms; the forward renderer's is empty (and
+// compiled away), the adjoint's differentiates them.
+struct NoHook {
+    __device__ __forceinline__ void emission(V3, uint32_t) {}
+    __device__ __forceinline__ void lightSample(V3, uint32_t, uint32_t, const BSDF&, V3, V3, const SurfPt&) {}
+    __device__ __forceinline__ void bounce(uint32_t, const BSDF&, V3, V3, const SurfPt&, V3) {}
 };
 
-// One camera path for pixel (gx, gy) (pathtracer.cl:48-196).
-template <bool ORDERED>
-__device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, Rng& rng,
-                        uint32_t* stack, int stride) {
+// One camera path for pixel (gx, gy) (pathtracer.cl:48-196). noRR disables
+// Russian roulette (the bounce cap still applies).
+template <bool ORDERED, class Hook = NoHook>
+__device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, bool noRR, Rng& rng,
+                        uint32_t* stack, int stride, Hook& hook) {
     V3 pix = v3z();
     V3 org, dir;
     float w, dirPDF;
@@ -43,7 +48,9 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
         vout = -dir;
         if (F.lightPtr != 0xFFFFu) {
             edfArea(S, F.lightPtr, sp, sp.u, sp.v, E);
-            pix += alpha * evalLe(E, vout);
+            const V3 term = alpha * evalLe(E, vout);
+            pix += term;
+            hook.emission(term, F.lightPtr);
         }
         while (true) {
             bsdfAlloc(S, F.matPtr, sp, B);
@@ -71,7 +78,10 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                     const float fsPDF = evalFsPdf(B, vout, vinL) * absCosEDF(E, vinL) / dist2;
                     const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
                     const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(B.N, vinL)) / dist2 / lightPDF;
-                    pix += wgt * ((alpha * evalLe(E, vinLrev)) * evalFs(B, vout, vinL));
+                    const V3 term = wgt * ((alpha * evalLe(E, vinLrev)) * evalFs(B, vout, vinL));
+                    pix += term;
+                    hook.lightSample(term, atInf ? 0xFFFFFFFFu : (uint32_t)S.g.faces[lp.face].lightPtr, F.matPtr, B,
+                                     vout, vinL, sp);
                 }
             }
             traceContinue = true;
@@ -82,6 +92,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                 traceContinue = false;
                 break;
             }
+            hook.bounce(F.matPtr, B, vout, nd, sp, pix);
             alpha = alpha * (f * (fabsf(dot(B.N, nd)) / dirPDF));
             dir = nd;
             org = sp.p + dir * kRayEps;
@@ -99,12 +110,15 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                     const float lightPDF = areaPdfOfFace(S, face) * d2 / absCosEDF(E, vout);
                     mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
                 }
-                pix += (mis * alpha) * evalLe(E, vout);
+                const V3 term = (mis * alpha) * evalLe(E, vout);
+                pix += term;
+                hook.emission(term, F.lightPtr);
             }
             if (depth >= maxDepth) {
                 traceContinue = false;
                 break;
             }
+            if (noRR) continue;
             const float cp = fminf(lum(alpha) * (1.0f / initY), 1.0f);
             if (rng.uniform() < cp) {
                 alpha = alpha * (1.0f / cp);
@@ -131,6 +145,117 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
     return pix;
 }
 
+// ---- path-replay adjoint ----------------------------------------------------
+// Parameters: per differentiable material (a single Oren-Nayar / Lambert
+// element, reflection.cl:239-255) its albedo rho (RGB) and roughness sigma,
+// and the emission M (RGB) of one emitter (Le = M / pi, light.cl:382-393).
+// Sampling, light selection and MIS weights do not depend on them (cosine
+// sampling, power-proportional light choice), and Russian roulette
+// probabilities are held fixed (replayed), so with the random states fixed
+// the estimate is a sum of path terms, each a product of BSDF values, one
+// emission and parameter-free factors. Per term:
+//   d/d rho_c   = term_c / rho_c    for every vertex of that material on it,
+//   d/d sigma   = term . adj * dlog f / d sigma at that vertex,
+//   d/d M_c     = term_c / M_c      when it ends on the emitter.
+// A replay records, at each bounce from a differentiable vertex, the radiance
+// accumulated so far; at the end every later term carries that vertex's
+// sampled BSDF value, so their sum is L - prefix (path replay
+// backpropagation, Vicini et al. 2021). Light-sample terms are differentiated
+// where they occur, with the BSDF value of the light direction.
+constexpr int kMaxAdjVertices = 16;
+constexpr uint32_t kMaxDiffMaterials = 256;
+
+struct AdjAcc {
+    float alb[kMaxDiffMaterials * 3];
+    float sig[kMaxDiffMaterials];
+    float em[3];
+};
+
+// Oren-Nayar angular factor X of f = rho / pi * (A + B X) (same terms as fx).
+__device__ __forceinline__ float orenNayarX(V3 wo, V3 wi) {
+    const float sti = sinTh(wi), sto = sinTh(wo);
+    float maxCos = 0.0f;
+    if (sti > 1e-4f && sto > 1e-4f) maxCos = fmaxf(0.0f, cosPhi(wi) * cosPhi(wo) + sinPhi(wi) * sinPhi(wo));
+    float sinA, tanB;
+    if (fabsf(wi.z) > fabsf(wo.z)) {
+        sinA = sto;
+        tanB = sti / fabsf(wi.z);
+    } else {
+        sinA = sti;
+        tanB = sto / fabsf(wo.z);
+    }
+    return maxCos * sinA * tanB;
+}
+
+struct AdjHook {
+    const RScene& S;
+    const uint16_t* table;  // matPtr -> parameter index + 1 (0: not differentiable)
+    uint32_t lightProp;     // lightPtr of the differentiable emitter
+    V3 M, adj;              // its emission; dLoss/dpixel / spp
+    AdjAcc* acc;            // block accumulators (LDS)
+    int n;
+    uint32_t recM[kMaxAdjVertices];
+    V3 recPrefix[kMaxAdjVertices], recRho[kMaxAdjVertices];
+    float recDlog[kMaxAdjVertices];
+
+    __device__ __forceinline__ uint32_t param(uint32_t matPtr, const BSDF& B) const {
+        if (B.n != 1 || B.c[0].id != 0u) return 0u;
+        return table[matPtr & 0xFFFFu];
+    }
+    __device__ __forceinline__ float dlogSigma(uint32_t matPtr, const BSDF& B, const SurfPt& sp, V3 vout,
+                                               V3 vin) const {
+        const BxDF& f = B.c[0];
+        if (f.a == 1.0f) return 0.0f;  // sigma = 0: f = rho / pi
+        const uint8_t* e = S.mat + ((matPtr + 8u + 3u) & ~3u);
+        const float sigma = evalFloat(S.tex + ld<uint32_t>(e + 8), sp.u, sp.v);
+        const V3 wo = toLocal(B.S, B.T, B.N, vout), wi = toLocal(B.S, B.T, B.N, vin);
+        const float X = orenNayarX(wo, wi);
+        const float s2 = sigma * sigma, qa = s2 + 0.33f, qb = s2 + 0.09f;
+        const float dA = -0.33f * sigma / (qa * qa), dB = 0.081f * sigma / (qb * qb);
+        return (dA + dB * X) / (f.a + f.b * X);
+    }
+    __device__ __forceinline__ void addAlbedo(uint32_t m, V3 term, V3 rho) const {
+        if (rho.x != 0.0f) atomicAdd(acc->alb + 3 * m + 0, adj.x * term.x / rho.x);
+        if (rho.y != 0.0f) atomicAdd(acc->alb + 3 * m + 1, adj.y * term.y / rho.y);
+        if (rho.z != 0.0f) atomicAdd(acc->alb + 3 * m + 2, adj.z * term.z / rho.z);
+    }
+    __device__ __forceinline__ void addEmission(V3 term) const {
+        if (M.x != 0.0f) atomicAdd(acc->em + 0, adj.x * term.x / M.x);
+        if (M.y != 0.0f) atomicAdd(acc->em + 1, adj.y * term.y / M.y);
+        if (M.z != 0.0f) atomicAdd(acc->em + 2, adj.z * term.z / M.z);
+    }
+    __device__ __forceinline__ void emission(V3 term, uint32_t lightPtr) {
+        if (lightPtr == lightProp) addEmission(term);
+    }
+    __device__ __forceinline__ void lightSample(V3 term, uint32_t lightPtr, uint32_t matPtr, const BSDF& B, V3 vout,
+                                                V3 vinL, const SurfPt& sp) {
+        if (lightPtr == lightProp) addEmission(term);
+        const uint32_t m = param(matPtr, B);
+        if (m) {
+            addAlbedo(m - 1, term, B.c[0].R);
+            atomicAdd(acc->sig + (m - 1), dot(adj, term) * dlogSigma(matPtr, B, sp, vout, vinL));
+        }
+    }
+    __device__ __forceinline__ void bounce(uint32_t matPtr, const BSDF& B, V3 vout, V3 vin, const SurfPt& sp,
+                                           V3 pixSoFar) {
+        const uint32_t m = param(matPtr, B);
+        if (!m || n >= kMaxAdjVertices) return;
+        recM[n] = m - 1;
+        recPrefix[n] = pixSoFar;
+        recRho[n] = B.c[0].R;
+        recDlog[n] = dlogSigma(matPtr, B, sp, vout, vin);
+        ++n;
+    }
+    __device__ __forceinline__ void finish(V3 L) {
+        for (int k = 0; k < n; ++k) {
+            const V3 after = L - recPrefix[k];
+            addAlbedo(recM[k], after, recRho[k]);
+            atomicAdd(acc->sig + recM[k], dot(adj, after) * recDlog[k]);
+        }
+        n = 0;
+    }
+};
+
 }  // namespace dev
 
 namespace {
@@ -139,7 +264,7 @@ constexpr int kTile = 8;  // 8x8 pixels per wave
 
 template <bool ORDERED>
 __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t width, uint32_t height,
-                                                    uint32_t spp, uint32_t maxDepth,
+                                                    uint32_t spp, uint32_t maxDepth, bool noRR,
                                                     uint32_t* __restrict__ states,
                                                     clr_float3* __restrict__ pixels) {
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
@@ -153,14 +278,77 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
     uint4 st = reinterpret_cast<uint4*>(states)[pi];
     dev::Rng rng{st.x, st.y, st.z, st.w};
     dev::V3 acc = dev::v3z();
+    dev::NoHook hook;
     for (uint32_t k = 0; k < spp; ++k)
-        acc += dev::tracePath<ORDERED>(S, gx, gy, maxDepth, rng, s_stack + threadIdx.x, 64);
+        acc += dev::tracePath<ORDERED>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64, hook);
     reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     float4 p = reinterpret_cast<float4*>(pixels)[pi];
     p.x += acc.x;
     p.y += acc.y;
     p.z += acc.z;
     reinterpret_cast<float4*>(pixels)[pi] = p;
+}
+
+// Adjoint of k_render_mega: the same pixels, samples and random states, each
+// sample replayed with the adjoint hook; block sums of the gradients go to
+// global memory once per block.
+template <bool ORDERED>
+__global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32_t width, uint32_t height,
+                                                       uint32_t spp, uint32_t maxDepth, bool noRR,
+                                                       const uint32_t* __restrict__ states,
+                                                       const float4* __restrict__ dLdI,
+                                                       const uint16_t* __restrict__ table, uint32_t numMat,
+                                                       uint32_t lightProp, float* __restrict__ gAlb,
+                                                       float* __restrict__ gSig, float* __restrict__ gEm) {
+    __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+    __shared__ dev::AdjAcc s_acc;
+    for (uint32_t i = threadIdx.x; i < dev::kMaxDiffMaterials * 3; i += 64) s_acc.alb[i] = 0.0f;
+    for (uint32_t i = threadIdx.x; i < dev::kMaxDiffMaterials; i += 64) s_acc.sig[i] = 0.0f;
+    if (threadIdx.x < 3) s_acc.em[threadIdx.x] = 0.0f;
+    __syncthreads();
+    dev::RScene S = scene;
+    dev::finishScene(S);
+    const uint32_t tilesX = (width + kTile - 1) / kTile;
+    const uint32_t gx = (blockIdx.x % tilesX) * kTile + (threadIdx.x % kTile);
+    const uint32_t gy = (blockIdx.x / tilesX) * kTile + (threadIdx.x / kTile);
+    if (gx < width && gy < height) {
+        const size_t pi = (size_t)gy * width + gx;
+        const uint4 st = reinterpret_cast<const uint4*>(states)[pi];
+        dev::Rng rng{st.x, st.y, st.z, st.w};
+        dev::V3 M = dev::v3z();
+        if (lightProp != 0xFFFFFFFFu) {
+            const uint8_t* e = S.mat + ((lightProp + 1u + 3u) & ~3u);
+            M = dev::evalColor(S.tex + dev::ld<uint32_t>(e + 4), 0.0f, 0.0f);
+        }
+        const float4 a = dLdI[pi];
+        const float inv = 1.0f / (float)spp;
+        dev::AdjHook hook{S, table, lightProp, M, {a.x * inv, a.y * inv, a.z * inv}, &s_acc, 0};
+        for (uint32_t k = 0; k < spp; ++k) {
+            const dev::V3 L = dev::tracePath<ORDERED>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64,
+                                                      hook);
+            hook.finish(L);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < numMat * 3; i += 64)
+        if (s_acc.alb[i] != 0.0f) atomicAdd(gAlb + i, s_acc.alb[i]);
+    for (uint32_t i = threadIdx.x; i < numMat; i += 64)
+        if (s_acc.sig[i] != 0.0f) atomicAdd(gSig + i, s_acc.sig[i]);
+    if (threadIdx.x < 3 && gEm && s_acc.em[threadIdx.x] != 0.0f) atomicAdd(gEm + threadIdx.x, s_acc.em[threadIdx.x]);
+}
+
+// Clears the matPtr -> parameter table and the gradient outputs, then (second
+// kernel) enters the differentiable materials.
+__global__ void k_adj_clear(uint16_t* table, float* gAlb, float* gSig, float* gEm, uint32_t numMat) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 65536u) table[i] = 0;
+    if (i < numMat * 3) gAlb[i] = 0.0f;
+    if (i < numMat) gSig[i] = 0.0f;
+    if (i < 3 && gEm) gEm[i] = 0.0f;
+}
+__global__ void k_adj_fill(uint16_t* table, const uint32_t* offsets, uint32_t numMat) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numMat) table[offsets[i]] = (uint16_t)(i + 1);
 }
 
 // faceLight[f] = first light index whose reference is f; envLight = first
@@ -237,11 +425,48 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
     if (rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)
         k_render_mega<false><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                       states, pixels);
+                                                       (rp.flags & CLR_RENDER_NO_RR) != 0, states, pixels);
     else
         k_render_mega<true><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                      states, pixels);
+                                                      (rp.flags & CLR_RENDER_NO_RR) != 0, states, pixels);
     return hipCheck(ctx, hipGetLastError(), "render launch");
+}
+
+int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, const clr_diff_params& dp,
+                  const uint32_t* states, const clr_float3* dLdI, float* gAlb, float* gSig, float* gEm,
+                  hipStream_t stream) {
+    if (!ctx->adjTable) {
+        hipError_t e = hipMalloc(&ctx->adjTable, 65536 * sizeof(uint16_t) + 256 * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->adjTable = nullptr;
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "adjoint parameter table");
+        }
+    }
+    uint16_t* table = static_cast<uint16_t*>(ctx->adjTable);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(table + 65536);
+    int rc = CLR_OK;
+    if (dp.numMaterials)
+        rc = hipCheck(ctx, hipMemcpyAsync(offsets, dp.matOffsets, dp.numMaterials * sizeof(uint32_t),
+                                          hipMemcpyHostToDevice, stream), "parameter offsets");
+    if (rc) return rc;
+    k_adj_clear<<<256, 256, 0, stream>>>(table, gAlb, gSig, gEm, dp.numMaterials);
+    if (dp.numMaterials) k_adj_fill<<<1, 256, 0, stream>>>(table, offsets, dp.numMaterials);
+    dev::RScene S;
+    rc = prepareRenderScene(ctx, sd, S, stream);
+    if (rc) return rc;
+    const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
+    const bool noRR = (rp.flags & CLR_RENDER_NO_RR) != 0;
+    const float4* adj = reinterpret_cast<const float4*>(dLdI);
+    if (rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)
+        k_render_adjoint<false><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
+                                                          noRR, states, adj, table, dp.numMaterials, dp.lightProp,
+                                                          gAlb, gSig, gEm);
+    else
+        k_render_adjoint<true><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
+                                                         noRR, states, adj, table, dp.numMaterials, dp.lightProp,
+                                                         gAlb, gSig, gEm);
+    return hipCheck(ctx, hipGetLastError(), "adjoint launch");
 }
 
 int seedPixelStates(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* states, hipStream_t stream) {

@@ -18,8 +18,12 @@ using namespace clr::host;
 struct clr_host_scene {
     Scene scene;
     uint32_t width = 0, height = 0;
-    std::vector<uint32_t> materialOffsets;  // per differentiable material (adjoint scenes)
-    std::vector<uint32_t> faceMaterial;     // per face: index into materialOffsets or UINT32_MAX
+    // Differentiable parameters (CLR_SCENE_ADJOINT): per material its byte
+    // offset in materialsData and the texturesData offsets of its albedo
+    // (3 floats) and roughness (1 float); the emitter's light-property offset
+    // and the texturesData offset of its emission (3 floats).
+    std::vector<uint32_t> materialOffsets, albedoOffsets, sigmaOffsets;
+    uint32_t lightProp = UINT32_MAX, emissionOffset = UINT32_MAX;
 };
 
 namespace {
@@ -508,6 +512,56 @@ void buildTerrain(clr_host_scene* hs, const clr_scene_params& p) {
                    p.fovY > 0 ? p.fovY : 45.0f, lens, focus);
 }
 
+// Differentiation scene: the closed Cornell room (walls fixed) with a
+// 200,000-triangle UV sphere (500 segments x 201 rings) of radius 0.45 whose
+// face f uses material f mod 64; each of the 64 materials is Oren-Nayar with
+// albedo U[0.2,0.8]^3 and roughness sigma U[0.1,0.6] drawn from the host RNG
+// (seed p.seed, 4 when 0). The ceiling light (emission 15) is differentiable
+// too. 512x512 by default.
+void buildAdjoint(clr_host_scene* hs, const clr_scene_params& p) {
+    Scene& s = hs->scene;
+    MaterialCreator& mc = MaterialCreator::sharedInstance();
+    mc.setScene(&s);
+    Rng rng(p.seed ? p.seed : 4);
+    mc.createFloat3ConstantTexture("R_white", 0.75f, 0.75f, 0.75f);
+    mc.createFloat3ConstantTexture("R_red", 0.6f, 0.0f, 0.0f);
+    mc.createFloat3ConstantTexture("R_green", 0.0f, 0.6f, 0.0f);
+    mc.createFloatConstantTexture("sigma_lambert", 0.0f);
+    mc.createMatteMaterial("mat_white", nullptr, "R_white", "sigma_lambert");
+    mc.createMatteMaterial("mat_red", nullptr, "R_red", "sigma_lambert");
+    mc.createMatteMaterial("mat_green", nullptr, "R_green", "sigma_lambert");
+    const uint32_t numMats = 64;
+    std::vector<uint16_t> sphereMat(numMats);
+    for (uint32_t k = 0; k < numMats; ++k) {
+        const std::string r = "R_" + std::to_string(k), sg = "S_" + std::to_string(k), m = "mat_" + std::to_string(k);
+        const float a0 = 0.2f + 0.6f * rng.uniform(), a1 = 0.2f + 0.6f * rng.uniform(), a2 = 0.2f + 0.6f * rng.uniform();
+        mc.createFloat3ConstantTexture(r.c_str(), a0, a1, a2);
+        mc.createFloatConstantTexture(sg.c_str(), 0.1f + 0.5f * rng.uniform());
+        mc.createMatteMaterial(m.c_str(), nullptr, r.c_str(), sg.c_str());
+        sphereMat[k] = (uint16_t)s.idxOfMat(m);
+        hs->materialOffsets.push_back((uint32_t)s.idxOfMat(m));
+        hs->albedoOffsets.push_back((uint32_t)s.idxOfTex(r) + 16u);
+        hs->sigmaOffsets.push_back((uint32_t)s.idxOfTex(sg) + 4u);
+    }
+    mc.createFloat3ConstantTexture("M_light", 15.0f, 15.0f, 15.0f);
+    mc.createDiffuseLightProperty("light_top", "M_light");
+    hs->lightProp = (uint32_t)s.idxOfLight("light_top");
+    hs->emissionOffset = (uint32_t)s.idxOfTex("M_light") + 16u;
+    const uint16_t W = (uint16_t)s.idxOfMat("mat_white"), R = (uint16_t)s.idxOfMat("mat_red"),
+                   G = (uint16_t)s.idxOfMat("mat_green");
+    const uint16_t mats[6] = {W, R, G, W, W, W};
+    addRoom(s, mats, true);
+    s.localToWorld.push();
+    s.localToWorld *= Mat4::scale(0.45f, 0.45f, 0.45f);
+    Mesh sph = uvsphere(500, 201);
+    addMesh(s, sph, true, true, false, [&](size_t f) { return sphereMat[f % numMats]; });
+    s.localToWorld.pop();
+    addCeilingLight(s, 0.25f, 0.9999f, W, (uint16_t)s.idxOfLight("light_top"));
+    const float eye[3] = {0.0f, 0.0f, 0.95f}, at[3] = {0.0f, 0.0f, 0.0f};
+    setPerspective(s, hs, p.width ? p.width : 512, p.height ? p.height : 512, eye, at,
+                   p.fovY > 0 ? p.fovY : 60.0f, 0.0f, 1.0f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -526,6 +580,7 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
             case CLR_SCENE_SHOWCASE: buildShowcase(hs, p); break;
             case CLR_SCENE_FURNACE: buildFurnace(hs, p); break;
             case CLR_SCENE_TERRAIN: buildTerrain(hs, p); break;
+            case CLR_SCENE_ADJOINT: buildAdjoint(hs, p); break;
             default: delete hs; return CLR_ERR_UNSUPPORTED;
         }
         hs->scene.build();
@@ -549,6 +604,26 @@ int clr_scene_info(const clr_host_scene* s, uint32_t* numFaces, uint32_t* numLig
     if (numLights) *numLights = (uint32_t)s->scene.lightInfos.size();
     if (width) *width = s->width;
     if (height) *height = s->height;
+    return CLR_OK;
+}
+
+int clr_scene_diff_params(const clr_host_scene* s, uint32_t* numMaterials, uint32_t* matOffsets,
+                          uint32_t* albedoOffsets, uint32_t* sigmaOffsets, uint32_t* lightProp,
+                          uint32_t* emissionOffset) {
+    if (!s || !numMaterials) return CLR_ERR_INVALID_ARG;
+    const uint32_t n = (uint32_t)s->materialOffsets.size();
+    if ((matOffsets || albedoOffsets || sigmaOffsets) && *numMaterials < n) {
+        *numMaterials = n;
+        return CLR_ERR_INVALID_ARG;
+    }
+    *numMaterials = n;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (matOffsets) matOffsets[k] = s->materialOffsets[k];
+        if (albedoOffsets) albedoOffsets[k] = s->albedoOffsets[k];
+        if (sigmaOffsets) sigmaOffsets[k] = s->sigmaOffsets[k];
+    }
+    if (lightProp) *lightProp = s->lightProp;
+    if (emissionOffset) *emissionOffset = s->emissionOffset;
     return CLR_OK;
 }
 

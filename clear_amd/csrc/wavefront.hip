@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
             if (depth > 0) {
                 if (depth >= A.maxDepth) {
                     live = false;
-                } else {
+                } else if (!(A.flags & CLR_RENDER_NO_RR)) {
                     const float cp = fminf(lum(alpha) * (1.0f / initY), 1.0f);
                     if (rng.uniform() < cp) {
                         alpha = alpha * (1.0f / cp);
@@ -860,7 +860,7 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     A.width = rp.width;
     A.height = rp.height;
     A.maxDepth = rp.maxDepth;
-    A.flags = rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL;
+    A.flags = rp.flags & (CLR_RENDER_REFERENCE_TRAVERSAL | CLR_RENDER_NO_RR);
     A.refillMin = w.cfg.refillMin;
     A.states = states;
     A.pixels = reinterpret_cast<float4*>(pixels);

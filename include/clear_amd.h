@@ -224,6 +224,12 @@ enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
  * triangles are hit at exactly the same distance or a ray grazes a box
  * within rounding. */
 #define CLR_RENDER_REFERENCE_TRAVERSAL 1u
+/* Turns Russian roulette off (paths stop only at maxDepth or when the BSDF
+ * sample fails): the estimator is then a smooth function of the material and
+ * emission parameters for fixed random states, as finite-difference checks
+ * with common random numbers need. Not in the reference (pathtracer.cl:167-172
+ * always applies it). */
+#define CLR_RENDER_NO_RR 2u
 
 /* Adds samplesPerPixel camera paths per pixel into `pixels` (clr_float3 per
  * pixel, row-major, not divided by the sample count: the caller scales, as
@@ -234,6 +240,34 @@ enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
  * the same image up to floating-point evaluation order. */
 int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
                uint32_t* randStates, clr_float3* pixels, int mode, void* stream);
+
+/* ---- differentiation ------------------------------------------------------
+ * No counterpart in the reference. Differentiable parameters: the albedo
+ * (RGB) and roughness sigma of materials made of one Oren-Nayar / Lambert
+ * element (reflection.cl:239-255; albedo and sigma are constant textures the
+ * caller may rewrite in texturesData between calls), and the emission M (RGB)
+ * of one diffuse emitter. */
+typedef struct clr_diff_params {
+    uint32_t numMaterials;       /* at most 256 */
+    const uint32_t* matOffsets;  /* HOST array: their byte offsets in materialsData (faces' matPtr) */
+    uint32_t lightProp;          /* the emitter's light-property offset (faces' lightPtr), or
+                                    0xFFFFFFFF for none */
+} clr_diff_params;
+
+/* Path-replay adjoint of clr_render (megakernel order, same flags). With
+ * `randStates` the per-pixel states a forward clr_render call started from
+ * (not modified here) and the same params, writes the gradients of
+ *     sum_p  dL_dimage[p] . (pixels added by that call)[p] / samplesPerPixel
+ * with respect to every listed material's albedo (gAlbedo[m][3]) and
+ * roughness (gRoughness[m]) and the emitter's emission (gEmission[3]).
+ * Sampling decisions are replayed, not differentiated (Russian roulette
+ * weights included; CLR_RENDER_NO_RR makes the estimate smooth in the
+ * parameters). Each sample is traced once: the radiance accumulated before
+ * each differentiable vertex is recorded and later terms are recovered as
+ * L - prefix. maxDepth <= 15. Gradient buffers are device memory, overwritten. */
+int clr_render_adjoint(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                       const clr_diff_params* diff, const uint32_t* randStates, const clr_float3* dL_dimage,
+                       float* gAlbedo, float* gRoughness, float* gEmission, void* stream);
 
 /* Seeds per-pixel states: seed = pixelIndex ^ 0x9E3779B9 expanded as
  * s_k = seed = 1812433253 * (seed ^ (seed >> 30)) + k, k = 0..3

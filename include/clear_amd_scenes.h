@@ -58,7 +58,13 @@ enum {
      * Beckmann coat (alpha U[0.05,0.5], dielectric eta 1.5), a 1024x512 FP32
      * sky with a sun of peak 5e4, thin lens (default radius 0.02; a negative
      * lensRadius gives a pinhole), 1920x1080 by default. */
-    CLR_SCENE_TERRAIN = 3
+    CLR_SCENE_TERRAIN = 3,
+    /* Differentiation scene: the closed Cornell room with a 200,000-triangle
+     * UV sphere whose face f uses Oren-Nayar material f mod 64 (albedo
+     * U[0.2,0.8]^3, roughness U[0.1,0.6] from `seed`, 4 when 0) and the
+     * emission-15 ceiling light; 512x512. clr_scene_diff_params lists the
+     * differentiable parameters. */
+    CLR_SCENE_ADJOINT = 4
 };
 
 enum {
@@ -81,6 +87,16 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
 int clr_scene_free(clr_host_scene* scene);
 int clr_scene_info(const clr_host_scene* scene, uint32_t* numFaces, uint32_t* numLights,
                    uint32_t* width, uint32_t* height);
+/* Differentiable parameters of a CLR_SCENE_ADJOINT scene: the number of
+ * materials, and per material its byte offset in materialsData (the faces'
+ * matPtr) and the texturesData byte offsets of its albedo (3 floats) and
+ * roughness sigma (1 float); the emitter's light-property offset (the faces'
+ * lightPtr) and the texturesData offset of its emission (3 floats). Arrays
+ * may be NULL; when given, *numMaterials is their capacity on entry. */
+int clr_scene_diff_params(const clr_host_scene* scene, uint32_t* numMaterials, uint32_t* matOffsets,
+                          uint32_t* albedoOffsets, uint32_t* sigmaOffsets, uint32_t* lightProp,
+                          uint32_t* emissionOffset);
+
 /* Pointer and byte size of one host array (CLR_ARRAY_*); valid until free. */
 int clr_scene_array(const clr_host_scene* scene, int which, const void** data, uint64_t* bytes);
 

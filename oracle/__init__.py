@@ -227,3 +227,31 @@ def disk_probe(u1, u2):
     out = (ctypes.c_float * 2)()
     fn(u1, u2, out)
     return list(out)
+
+
+def render_adjoint(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, flags, mat_offsets,
+                   light_prop, states, d_image, threads=0):
+    """CPU adjoint: (g_albedo [M,3], g_sigma [M], g_emission [3])."""
+    L = lib()
+    fn = L.orc_render_adjoint
+    fn.restype = ctypes.c_int
+    vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+    fn.argtypes = [vp, u32, u32, u32, u32, u32, u32, u32, vp, u32, vp, vp, vp, vp, vp, ctypes.c_int]
+    keep = [np.ascontiguousarray(arrays[k]) for k in
+            ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
+             "others"]]
+    keep.append(np.ascontiguousarray(inodes) if num_faces > 1 else np.zeros((1, 48), np.uint8))
+    keep.append(np.ascontiguousarray(leaves))
+    ptrs = (ctypes.c_void_p * 11)(*[k.ctypes.data for k in keep])
+    m = len(mat_offsets)
+    mo = np.ascontiguousarray(mat_offsets, dtype=np.uint32)
+    ga = np.zeros((max(m, 1), 3), np.float32)
+    gs = np.zeros(max(m, 1), np.float32)
+    ge = np.zeros(3, np.float32)
+    st = np.ascontiguousarray(states, dtype=np.uint32)
+    di = np.ascontiguousarray(d_image, dtype=np.float32)
+    rc = fn(ptrs, num_faces, width, height, spp, max_depth, flags, m, _p(mo), light_prop, _p(st), _p(di),
+            _p(ga), _p(gs), _p(ge), threads)
+    if rc:
+        raise RuntimeError(f"orc_render_adjoint failed: {rc}")
+    return ga[:m], gs[:m], ge

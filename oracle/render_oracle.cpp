@@ -1032,10 +1032,19 @@ V3 mulP4xyz(const uint8_t* m, float x, float y) {  // (x, y, 0, 1), xyz after th
 
 struct RenderArgs {
     uint32_t width, height, maxDepth;
+    bool noRR;  // CLR_RENDER_NO_RR
+};
+
+// Observer of a path's radiance terms (the adjoint's differentiates them).
+struct NoHook {
+    void emission(V3, uint32_t) {}
+    void lightSample(V3, uint32_t, uint32_t, const BSDF&, V3, V3, const SurfacePoint&) {}
+    void bounce(uint32_t, const BSDF&, V3, V3, const SurfacePoint&, V3) {}
 };
 
 // One camera path of pathtracing (pathtracer.cl:25-197) for pixel (x, y).
-V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, uint32_t* rds) {
+template <class Hook = NoHook>
+V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, uint32_t* rds, Hook& hook) {
     V3 pix = mk(0, 0, 0);
     const uint8_t* cam = s.camera;
     const uint8_t* l2w = cam + 64;
@@ -1076,7 +1085,9 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
         if (face->lightPtr != 0xFFFFu) {
             lightPointFromSurface(sp, &lp);
             edfAlloc(s, face->lightPtr, lp, &E);
-            pix += alpha * Le(E, vout);
+            const V3 term = alpha * Le(E, vout);
+            pix += term;
+            hook.emission(term, face->lightPtr);
         }
         while (true) {
             bsdfAlloc(s, face->matPtr, sp, &B);
@@ -1098,7 +1109,10 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
                     float fsPDF = fsPdf(B, vout, vinL) * absCosNsEDF(E, vinL) / dist2;
                     MIS = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
                     float w = MIS * absCosNsEDF(E, vinL) * absCosNsBSDF(B, vinL) / dist2 / lightPDF;
-                    pix += w * ((alpha * Le(E, vinLrev)) * fs(B, vout, vinL));
+                    const V3 term = w * ((alpha * Le(E, vinLrev)) * fs(B, vout, vinL));
+                    pix += term;
+                    hook.lightSample(term, lp.atInfinity ? 0xFFFFFFFFu : (uint32_t)s.faces[lp.faceID].lightPtr,
+                                     face->matPtr, B, vout, vinL, sp);
                 }
             }
             traceContinue = true;
@@ -1106,6 +1120,7 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
             V3 nd{};
             V3 f = sampleFs(B, vout, bc, b0, b1, &nd, &dirPDF, &sampledType);
             if (zeroVec(f) || dirPDF == 0.0f) { traceContinue = false; break; }
+            hook.bounce(face->matPtr, B, vout, nd, sp, pix);
             alpha *= f * (absCosNsBSDF(B, nd) / dirPDF);
             rdir = nd;
             org = sp.p + rdir * kEps;
@@ -1123,9 +1138,12 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
                     lightPDF = getAreaPDF(s, lp) * d2 / absCosNsEDF(E, vout);
                     MIS = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
                 }
-                pix += (MIS * alpha) * Le(E, vout);
+                const V3 term = (MIS * alpha) * Le(E, vout);
+                pix += term;
+                hook.emission(term, face->lightPtr);
             }
             if (depth >= ra.maxDepth) { traceContinue = false; break; }  // D6
+            if (ra.noRR) continue;
             float cp = std::fmin(luminance(alpha) * (1.0f / initY), 1.0f);
             if (getFloat(rds) < cp) {
                 alpha *= 1.0f / cp;
@@ -1155,6 +1173,85 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
     return pix;
 }
 
+// Path-replay adjoint (restates the library's): per differentiable material
+// (one Oren-Nayar element) albedo and sigma, and one emitter's emission M.
+// Light-sample and emission terms are differentiated where they occur; at
+// each bounce from a differentiable vertex the radiance so far is recorded
+// and the later terms, which all carry that vertex's sampled BSDF value, are
+// L - prefix at the end of the path. Accumulates in double.
+struct AdjHookC {
+    const SceneC& s;
+    const std::vector<int>& table;  // matPtr -> parameter index or -1
+    uint32_t lightProp;
+    V3 M, adj;
+    double* gAlb;
+    double* gSig;
+    double* gEm;
+    struct Rec {
+        int m;
+        V3 prefix, rho;
+        float dlog;
+    };
+    std::vector<Rec> recs;
+
+    int param(uint32_t matPtr, const BSDF& B) const {
+        if (B.n != 1 || B.c[0].id != 0) return -1;
+        return table[matPtr & 0xFFFFu];
+    }
+    float dlogSigma(uint32_t matPtr, const BSDF& B, const SurfacePoint& sp, V3 vout, V3 vin) const {
+        const BxDF& f = B.c[0];
+        if (f.a == 1.0f) return 0.0f;
+        const uint8_t* e = s.materials + ((matPtr + 8u + 3u) & ~3u);
+        const float sigma = evalFloat(s.textures + rd<uint32_t>(e + 8), sp.uv[0], sp.uv[1]);
+        const V3 wo = worldToLocal(B.S, B.T, B.N, vout), wi = worldToLocal(B.S, B.T, B.N, vin);
+        const float sti = sinTh(wi), sto = sinTh(wo);
+        float maxCos = 0.0f;
+        if (sti > 1e-4f && sto > 1e-4f) maxCos = std::fmax(0.0f, cosPhi(wi) * cosPhi(wo) + sinPhi(wi) * sinPhi(wo));
+        float sinA, tanB;
+        if (absCosT(wi) > absCosT(wo)) { sinA = sto; tanB = sti / absCosT(wi); }
+        else { sinA = sti; tanB = sto / absCosT(wo); }
+        const float X = maxCos * sinA * tanB;
+        const float s2 = sigma * sigma, qa = s2 + 0.33f, qb = s2 + 0.09f;
+        const float dA = -0.33f * sigma / (qa * qa), dB = 0.081f * sigma / (qb * qb);
+        return (dA + dB * X) / (f.a + f.b * X);
+    }
+    void addAlbedo(int m, V3 term, V3 rho) {
+        if (rho.x != 0.0f) gAlb[3 * m + 0] += (double)adj.x * term.x / rho.x;
+        if (rho.y != 0.0f) gAlb[3 * m + 1] += (double)adj.y * term.y / rho.y;
+        if (rho.z != 0.0f) gAlb[3 * m + 2] += (double)adj.z * term.z / rho.z;
+    }
+    void addEmission(V3 term) {
+        if (M.x != 0.0f) gEm[0] += (double)adj.x * term.x / M.x;
+        if (M.y != 0.0f) gEm[1] += (double)adj.y * term.y / M.y;
+        if (M.z != 0.0f) gEm[2] += (double)adj.z * term.z / M.z;
+    }
+    void emission(V3 term, uint32_t lightPtr) {
+        if (lightPtr == lightProp) addEmission(term);
+    }
+    void lightSample(V3 term, uint32_t lightPtr, uint32_t matPtr, const BSDF& B, V3 vout, V3 vinL,
+                     const SurfacePoint& sp) {
+        if (lightPtr == lightProp) addEmission(term);
+        const int m = param(matPtr, B);
+        if (m >= 0) {
+            addAlbedo(m, term, B.c[0].R);
+            gSig[m] += (double)dot(adj, term) * dlogSigma(matPtr, B, sp, vout, vinL);
+        }
+    }
+    void bounce(uint32_t matPtr, const BSDF& B, V3 vout, V3 vin, const SurfacePoint& sp, V3 pixSoFar) {
+        const int m = param(matPtr, B);
+        if (m < 0) return;
+        recs.push_back({m, pixSoFar, B.c[0].R, dlogSigma(matPtr, B, sp, vout, vin)});
+    }
+    void finish(V3 L) {
+        for (const Rec& r : recs) {
+            const V3 after = L - r.prefix;
+            addAlbedo(r.m, after, r.rho);
+            gSig[r.m] += (double)dot(adj, after) * r.dlog;
+        }
+        recs.clear();
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1174,6 +1271,71 @@ int orc_seed_states(uint32_t width, uint32_t height, uint32_t* states) {
 // accumulating into pixels (clr_float3 per pixel) and advancing states.
 // arrays: vertices, normals, tangents, uvs, faces, lights, materials, textures,
 // others, inodes, leaves. threads <= 0 uses all hardware threads.
+// Adjoint of orc_render over pixels [0, width * height): gradients (float out)
+// of sum_p dLdI[p] . (radiance added for p) / spp, states not modified.
+int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t width, uint32_t height,
+                       uint32_t spp, uint32_t maxDepth, uint32_t flags, uint32_t numMat,
+                       const uint32_t* matOffsets, uint32_t lightProp, const uint32_t* states,
+                       const clr_float3* dLdI, float* gAlb, float* gSig, float* gEm, int threads) {
+    SceneC s{};
+    s.vertices = (const clr_float3*)arrays[0];
+    s.normals = (const clr_float3*)arrays[1];
+    s.tangents = (const clr_float3*)arrays[2];
+    s.uvs = (const float*)arrays[3];
+    s.faces = (const clr_face*)arrays[4];
+    s.lights = (const clr_light_info*)arrays[5];
+    s.materials = (const uint8_t*)arrays[6];
+    s.textures = (const uint8_t*)arrays[7];
+    s.others = (const uint8_t*)arrays[8];
+    s.inodes = (const clr_lbvh_inode*)arrays[9];
+    s.leaves = (const clr_lbvh_leaf*)arrays[10];
+    s.numFaces = numFaces;
+    s.camera = s.others + rd<uint32_t>(s.others + 0);
+    s.environment = s.others + rd<uint32_t>(s.others + 4);
+    s.lightDist = s.others + rd<uint32_t>(s.others + 8);
+    s.ordered = (flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
+    RenderArgs ra{width, height, maxDepth, (flags & CLR_RENDER_NO_RR) != 0};
+    std::vector<int> table(65536, -1);
+    for (uint32_t k = 0; k < numMat; ++k) table[matOffsets[k] & 0xFFFFu] = (int)k;
+    V3 M = mk(0, 0, 0);
+    if (lightProp != 0xFFFFFFFFu) {
+        const uint8_t* e = s.materials + ((lightProp + 1u + 3u) & ~3u);
+        M = evalColor(s.textures + rd<uint32_t>(e + 4), 0.0f, 0.0f);
+    }
+    const uint32_t numPix = width * height;
+    const int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::vector<double>> part(nt, std::vector<double>(4 * (size_t)numMat + 3, 0.0));
+    std::atomic<uint32_t> next{0};
+    auto work = [&](int ti) {
+        double* gA = part[ti].data();
+        double* gS = gA + 3 * (size_t)numMat;
+        double* gE = gS + numMat;
+        AdjHookC hook{s, table, lightProp, M, mk(0, 0, 0), gA, gS, gE, {}};
+        while (true) {
+            const uint32_t b = next.fetch_add(64);
+            if (b >= numPix) break;
+            for (uint32_t i = b; i < std::min(numPix, b + 64); ++i) {
+                uint32_t st[4] = {states[4 * (size_t)i], states[4 * (size_t)i + 1], states[4 * (size_t)i + 2],
+                                  states[4 * (size_t)i + 3]};
+                const float inv = 1.0f / (float)spp;
+                hook.adj = mk(dLdI[i].x * inv, dLdI[i].y * inv, dLdI[i].z * inv);
+                for (uint32_t k = 0; k < spp; ++k) hook.finish(tracePath(s, ra, i % width, i / width, st, hook));
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nt; ++t) ts.emplace_back(work, t);
+    for (auto& t : ts) t.join();
+    for (uint32_t k = 0; k < 4 * numMat + 3; ++k) {
+        double v = 0.0;
+        for (int t = 0; t < nt; ++t) v += part[t][k];
+        if (k < 3 * numMat) gAlb[k] = (float)v;
+        else if (k < 4 * numMat) gSig[k - 3 * numMat] = (float)v;
+        else if (gEm) gEm[k - 4 * numMat] = (float)v;
+    }
+    return 0;
+}
+
 int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uint32_t height, uint32_t spp,
                uint32_t maxDepth, uint32_t* states, clr_float3* pixels, uint32_t pixBegin, uint32_t pixEnd,
                int threads, uint32_t flags) {
@@ -1194,7 +1356,7 @@ int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uin
     s.camera = s.others + rd<uint32_t>(s.others + 0);
     s.environment = s.others + rd<uint32_t>(s.others + 4);
     s.lightDist = s.others + rd<uint32_t>(s.others + 8);
-    RenderArgs ra{width, height, maxDepth};
+    RenderArgs ra{width, height, maxDepth, (flags & CLR_RENDER_NO_RR) != 0};
     if (pixEnd > width * height) pixEnd = width * height;
     int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<uint32_t> next{pixBegin};
@@ -1210,7 +1372,8 @@ int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uin
                 // pixel as it goes, pathtracer.cl:93-195; only the rounding of
                 // the running sum differs).
                 V3 acc = mk(0, 0, 0);
-                for (uint32_t k = 0; k < spp; ++k) acc += tracePath(s, ra, gx, gy, states + 4 * (size_t)i);
+                NoHook hook;
+                for (uint32_t k = 0; k < spp; ++k) acc += tracePath(s, ra, gx, gy, states + 4 * (size_t)i, hook);
                 pixels[i].x += acc.x;
                 pixels[i].y += acc.y;
                 pixels[i].z += acc.z;
