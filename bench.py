@@ -1,16 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the LBVH hot path on one MI355X.
+"""Benchmark of the CLeaR hot path on one MI355X.
 
-Workload (one "step"): one LBVH build of 2^24 triangles from a Gaussian-cluster
-soup (64 clusters, sigma 0.02, edge lengths 1e-3..1e-2, seed 1) with 30-bit
-Morton codes (10 bits per axis). Inputs are resident in HBM before timing;
-W untimed builds, then K timed builds bracketed by device synchronisation.
-`value` is Mtri/s of the build. The same run also traces 2^22 random rays for
-closest hit against the 2^24 BVH (Mrays/s), reports per-stage HIP-event
-times, the roofline of the dominant kernel, and the CPU oracle's build on the
-same soup as the CPU baseline.
+Headline ("value", one step = one LBVH build): 2^24 triangles of a
+Gaussian-cluster soup (64 clusters, sigma 0.02, edge lengths 1e-3..1e-2,
+seed 1), 30-bit Morton codes (10 bits per axis), 48-B node output; inputs
+resident in HBM; W untimed builds, then K timed builds bracketed by device
+synchronisation; value = Mtri/s. Per-stage times come from HIP events the
+library records on the build's stream; the roofline is that of the dominant
+kernel (k_hier_local) with the algorithmic bytes per triangle stated below.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+Secondary legs of the same metric family, each on its stated workload:
+  trace    2^22 random rays (origins in the unit cube, directions uniform,
+           tmax = inf) against the 2^24 BVH: Mrays/s, reference-order walk and
+           ordered walk;
+  render   the 3,117,058-triangle outdoor scene (fBm terrain 1024^2, 200
+           spheres, coat BSDF, normal map, 1024x512 HDR sky, thin lens 0.02) at
+           1920x1080, 64 spp, max depth 8: Msamples/s of the plain megakernel
+           (reference traversal), the megakernel (ordered traversal) and the
+           wavefront renderer (autotuned);
+  adjoint  the 200,000-triangle, 64-material differentiation scene, 512x512,
+           32 spp, depth 5, Russian roulette off: forward and adjoint time;
+  cpu      the CPU oracle's serial LBVH build on the same soup (cpu_baseline)
+           and its renderer on a band of the outdoor scene (all host threads).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-trace]
+                    [--no-render] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -26,12 +40,14 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# Algorithmic bytes per triangle per build stage (scene box, Morton, radix
-# sort, hierarchy = Karras topology + refit), as itemised in DESIGN.md.
-STAGE_BYTES = {"bounds": 36.0, "morton": 40.0, "sort": 64.0, "hierarchy": 184.0}
-STAGE_LAUNCHES = {"bounds": 2, "morton": 1, "sort": 4, "hierarchy": 1}
-STAGE_KERNEL = {"bounds": "k_bounds", "morton": "k_morton", "sort": "k_onesweep",
-                "hierarchy": "k_hierarchy"}
+# Algorithmic bytes per triangle per build stage: scene box (read 36 B of
+# vertices), Morton codes (read 36, write a 4-B key), radix sort (4 passes of
+# 8-bit digits over 4-B keys and payloads: 64 B), hierarchy (Karras topology
+# 20 B + refit 164 B = 184 B, all attributed to the block-local kernel, which
+# builds every node except the few above block boundaries).
+STAGES = [("bounds", "k_bounds", 36.0, 1), ("morton", "k_morton", 40.0, 1),
+          ("sort", "k_onesweep", 64.0, 4), ("hier_local", "k_hier_local", 184.0, 1),
+          ("hier_global", "k_hier_global", 0.0, 1)]
 # Traversal work model per internal-node visit (V) and triangle test (T).
 TRACE_FLOPS_V, TRACE_FLOPS_T = 40.0, 51.0
 TRACE_BYTES_V, TRACE_BYTES_T = 56.0, 40.0
@@ -42,8 +58,9 @@ def log(msg: str) -> None:
 
 
 def load_traffic():
-    """HBM bytes per launch of the dominant kernel, measured by rocprofv3 PMC
-    runs and committed under profiles/ (None if absent)."""
+    """HBM bytes per launch of k_hier_local from rocprofv3 PMC runs
+    (FETCH_SIZE, WRITE_SIZE) scaled by a copy-kernel calibration, committed
+    under profiles/ (None if absent)."""
     p = ROOT / "profiles" / "lbvh_16m_pmc.json"
     if not p.exists():
         return None
@@ -53,55 +70,19 @@ def load_traffic():
         return None
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--tris", type=int, default=1 << 24)
-    ap.add_argument("--rays", type=int, default=1 << 22)
-    ap.add_argument("--bits", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit leg")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="also run one plain device copy of this size (PMC calibration)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
-              f"multi-GPU sharding is a design note in DESIGN.md, not a measured mode.",
-              file=sys.stderr)
-        return 2
-
-    import numpy as np
-    import torch
-
-    import clear_amd as ca
-
+def lbvh_leg(ca, np, torch, args, dev):
     n, bits, K, W = args.tris, args.bits, args.steps, args.warmup
     log(f"generating {n} triangles")
     soup = ca.gen_cluster_soup(n, seed=1)
-    dev = torch.device("cuda:0")
     tsoup = torch.from_numpy(soup).to(dev)
     inodes, leaves = ca.node_buffers(n, dev)
     ctx = ca.Context(0)
     builder = ca.LBVHBuilder(ctx, n, bits)
     stream = torch.cuda.current_stream()
-
-    if args.copy_probe_mib:
-        src = torch.empty(args.copy_probe_mib << 20, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
     log(f"warmup {W}")
     for _ in range(W):
         builder.perform_soup(tsoup, n, bits, inodes, leaves, stream)
     torch.cuda.synchronize()
-
-    # Timed region; stage boundaries carry HIP events recorded on the build's
-    # own stream by the library (negligible cost).
     ctx.set_profiling(True)
     log(f"timing {K} builds")
     torch.cuda.synchronize()
@@ -112,108 +93,235 @@ def main() -> int:
     dt = time.perf_counter() - t0
     stages_ms = ctx.last_profile()
     ctx.set_profiling(False)
-    ms_per_step = dt / K * 1e3
-    mtri_s = n * K / dt / 1e6
-
-    names = ["bounds", "morton", "sort", "hierarchy"]
+    assert ctx.device_status() == 0
     stage_info = {}
-    for name, ms in zip(names, stages_ms):
-        gbs = STAGE_BYTES[name] * n / (ms * 1e-3) / 1e9
-        stage_info[name] = {"ms": round(ms, 4), "GB_s": round(gbs, 1),
-                            "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-                            "launches": STAGE_LAUNCHES[name]}
-    dom = max(stage_info, key=lambda k: stage_info[k]["ms"])
-    dom_ms_launch = stage_info[dom]["ms"] / STAGE_LAUNCHES[dom]
-    dom_bytes_launch = STAGE_BYTES[dom] * n / STAGE_LAUNCHES[dom]
-    achieved = dom_bytes_launch / (dom_ms_launch * 1e-3) / 1e9
-    traffic = None
+    for (name, kernel, bpt, launches), ms in zip(STAGES, stages_ms):
+        e = {"kernel": kernel, "ms": round(ms, 4), "launches": launches, "bytes_per_tri": bpt}
+        if bpt:
+            gbs = bpt * n / (ms * 1e-3) / 1e9
+            e.update({"GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)})
+        stage_info[name] = e
+    loc = stage_info["hier_local"]
+    achieved = 184.0 * n / (loc["ms"] * 1e-3) / 1e9
     tr = load_traffic()
-    if tr and tr.get("kernel") == STAGE_KERNEL[dom] and tr.get("tris") == n:
-        traffic = tr.get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": round(achieved, 1),
+    traffic = tr.get("hbm_bytes_per_launch") if tr and tr.get("tris") == n else None
+    roofline = {"bound": "hbm", "kernel": "k_hier_local", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
-                "algorithmic_bytes_per_launch": dom_bytes_launch,
-                "ms_per_launch": round(dom_ms_launch, 4)}
-    whole_gbs = sum(STAGE_BYTES.values()) * n / (ms_per_step * 1e-3) / 1e9
+                "traffic": traffic, "algorithmic_bytes_per_launch": 184.0 * n,
+                "ms_per_launch": loc["ms"]}
+    whole = sum(s[2] for s in STAGES) * n / (dt / K) / 1e9
+    return ctx, soup, tsoup, inodes, leaves, {
+        "mtri_s": n * K / dt / 1e6, "ms_per_step": dt / K * 1e3, "stages": stage_info,
+        "roofline": roofline, "build_effective_GB_s": round(whole, 1)}
 
-    # Closest-hit leg: 2^22 random rays against the 2^24 BVH.
-    trace = None
-    if not args.no_trace:
-        R = args.rays
-        rays = torch.from_numpy(ca.gen_random_rays(R, seed=2)).to(dev)
-        hit = torch.empty(R, dtype=torch.int32, device=dev)
-        ht = torch.empty(R, dtype=torch.float32, device=dev)
-        st = torch.empty(2 * R, dtype=torch.int32, device=dev)
-        verts, faces = ca.soup_to_mesh(soup)
-        tv = torch.from_numpy(verts).to(dev)
-        tf = torch.from_numpy(faces.view(np.uint8)).to(dev)
-        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, st)
+
+def trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev):
+    n, R = args.tris, args.rays
+    rays = torch.from_numpy(ca.gen_random_rays(R, seed=2)).to(dev)
+    hit = torch.empty(R, dtype=torch.int32, device=dev)
+    ht = torch.empty(R, dtype=torch.float32, device=dev)
+    st = torch.empty(2 * R, dtype=torch.int32, device=dev)
+    verts, faces = ca.soup_to_mesh(soup)
+    tv = torch.from_numpy(verts).to(dev)
+    tf = torch.from_numpy(faces.view(np.uint8)).to(dev)
+    out = {"rays": R, "workload": f"{R} random rays vs the {n}-triangle BVH"}
+    for name, ordered in [("reference_walk", False), ("ordered_walk", True)]:
+        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, st, ordered=ordered)
         torch.cuda.synchronize()
         stv = st.view(R, 2).to(torch.int64)
-        V = int(stv[:, 0].sum().item())
-        T = int(stv[:, 1].sum().item())
-        hits = int((hit != -1).sum().item())
-        for _ in range(2):
-            ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None)
-        Kt = max(3, K // 4)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(Kt):
-            ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None)
-        e1.record(stream)
+        V, T = int(stv[:, 0].sum()), int(stv[:, 1].sum())
+        hits = int((hit != -1).sum())
+        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered)
+        reps = 3
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered)
+        e1.record()
         torch.cuda.synchronize()
-        tms = e0.elapsed_time(e1) / Kt
+        ms = e0.elapsed_time(e1) / reps
         flops = TRACE_FLOPS_V * V + TRACE_FLOPS_T * T
         byts = TRACE_BYTES_V * V + TRACE_BYTES_T * T
-        ai = flops / byts
-        roof = min(FP32_PEAK_TFLOPS, ai * HBM_PEAK_GBS / 1e3)
-        achieved_tf = flops / (tms * 1e-3) / 1e12
-        trace = {"rays": R, "ms": round(tms, 3), "Mrays_s": round(R / (tms * 1e-3) / 1e6, 1),
-                 "hit_fraction": round(hits / R, 4), "visits_per_ray": round(V / R, 2),
-                 "tests_per_ray": round(T / R, 2), "TFLOP_s": round(achieved_tf, 3),
-                 "roof_TFLOP_s": round(roof, 2), "frac": round(achieved_tf / roof, 4)}
-        del rays, hit, ht, st, tv, tf
-        log(f"trace: {trace}")
+        roof = min(FP32_PEAK_TFLOPS, flops / byts * HBM_PEAK_GBS / 1e3)
+        tf_s = flops / (ms * 1e-3) / 1e12
+        out[name] = {"ms": round(ms, 3), "Mrays_s": round(R / (ms * 1e-3) / 1e6, 1),
+                     "hit_fraction": round(hits / R, 4), "visits_per_ray": round(V / R, 2),
+                     "tests_per_ray": round(T / R, 2), "TFLOP_s": round(tf_s, 3),
+                     "roof_TFLOP_s": round(roof, 2), "frac": round(tf_s / roof, 4)}
+        log(f"trace {name}: {out[name]}")
+    assert ctx.device_status() == 0
+    return out
 
-    # CPU baseline: the oracle's serial build (1 thread) on the same soup.
-    cpu = None
+
+def render_leg(ca, np, torch, ctx, args):
+    W, H, spp, depth = 1920, 1080, args.render_spp, 8
+    log("building the outdoor scene")
+    host = ca.HostScene(ca.SCENE_TERRAIN, W, H)
+    ds = ca.DeviceScene(ctx, host)
+    states, pixels = ca.new_image(W, H)
+    out = {"workload": f"outdoor scene, {host.num_faces} triangles, {W}x{H}, {spp} spp, "
+                       f"max depth {depth}, thin lens 0.02", "triangles": host.num_faces,
+           "spp": spp}
+    tune_t = time.perf_counter()
+    cfg = ca.wavefront_autotune(ctx, ds, 1, depth)
+    out["wavefront_config"] = cfg.as_dict()
+    out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
+    legs = [("megakernel_plain", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL),
+            ("megakernel", ca.RENDER_MEGAKERNEL, 0), ("wavefront", ca.RENDER_WAVEFRONT, 0)]
+    for name, mode, flags in legs:
+        ca.seed_pixel_states(ctx, W, H, states)
+        ca.render(ctx, ds, 1, depth, states, pixels, mode=mode, flags=flags)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ca.render(ctx, ds, spp, depth, states, pixels, mode=mode, flags=flags)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        out[name] = {"ms": round(ms, 2), "Msamples_s": round(W * H * spp / (ms * 1e-3) / 1e6, 1)}
+        log(f"render {name}: {out[name]}")
+    assert ctx.device_status() == 0
+    out["wavefront_vs_megakernel"] = round(out["wavefront"]["Msamples_s"] / out["megakernel"]["Msamples_s"], 3)
+    out["wavefront_vs_plain_megakernel"] = round(
+        out["wavefront"]["Msamples_s"] / out["megakernel_plain"]["Msamples_s"], 3)
+    return out, host
+
+
+def adjoint_leg(ca, np, torch, ctx):
+    from clear_amd.diff import DiffRenderer
+
+    host = ca.HostScene(ca.SCENE_ADJOINT, 512, 512)
+    r = DiffRenderer(ctx, host, 32, 5, flags=ca.RENDER_NO_RR)
+    states = r.seed_states()
+    g = torch.ones(512, 512, 3, device="cuda")
+    r.render(states)
+    r.adjoint(states, g)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.render(states)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        r.adjoint(states, g)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, adj = (t1 - t0) / reps * 1e3, (t2 - t1) / reps * 1e3
+    assert ctx.device_status() == 0
+    out = {"workload": f"differentiation scene, {host.num_faces} triangles, 64 materials, 512x512, "
+                       "32 spp, depth 5, no Russian roulette",
+           "forward_ms": round(fwd, 2), "adjoint_ms": round(adj, 2), "adjoint_over_forward": round(adj / fwd, 3)}
+    log(f"adjoint: {out}")
+    return out
+
+
+def cpu_lbvh(ca, soup, bits):
+    import oracle as orc
+
+    m = soup.shape[0]
+    verts, faces = ca.soup_to_mesh(soup)
+    log(f"cpu baseline: oracle LBVH build of {m} triangles")
+    c0 = time.perf_counter()
+    orc.lbvh_build(verts, faces, bits)
+    cdt = time.perf_counter() - c0
+    return {"value": round(m / cdt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+            "sample": f"oracle LBVH build (serial C++) of the same {m}-triangle soup, "
+                      f"{cdt:.1f} s; host threads available: {os.cpu_count()}"}
+
+
+def cpu_render(ca, np, host, spp, depth):
+    import oracle as orc
+
+    W, H = host.width, host.height
+    rows = 96
+    threads = min(16, os.cpu_count() or 1)
+    o = orc.lbvh_build(host.vertices, host.faces, 10)
+    st = orc.seed_states(W, H)
+    px = np.zeros(W * H * 4, np.float32)
+    b = (H // 2) * W
+    e = b + rows * W
+    c0 = time.perf_counter()
+    orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, spp, depth, st, px,
+               pix_begin=b, pix_end=e, threads=threads)
+    cdt = time.perf_counter() - c0
+    return {"value": round(rows * W * spp / cdt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+            "kind": "port", "sample": f"oracle renderer (ordered traversal) on {rows} middle rows of the "
+                                      f"1920x1080 outdoor scene, {spp} spp, depth {depth}: {cdt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=1 << 24)
+    ap.add_argument("--rays", type=int, default=1 << 22)
+    ap.add_argument("--bits", type=int, default=10)
+    ap.add_argument("--render-spp", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the render leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one plain device-to-device copy of this size (calibrates "
+                         "HBM byte counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding "
+              f"of this path is a design, not a measured mode.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import clear_amd as ca
+
+    dev = torch.device("cuda:0")
+    ctx, soup, tsoup, inodes, leaves, lb = lbvh_leg(ca, np, torch, args, dev)
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    trace = None if args.no_trace else trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev)
+    del tsoup
+    render, host = (None, None) if args.no_render else render_leg(ca, np, torch, ctx, args)
+    adjoint = None if args.no_adjoint else adjoint_leg(ca, np, torch, ctx)
+    cpu = cpu_r = None
     if not args.no_cpu:
-        import oracle as orc
-
-        m = min(n, 1 << 24)
-        verts, faces = ca.soup_to_mesh(soup[:m])
-        log(f"cpu baseline on {m} triangles")
-        c0 = time.perf_counter()
-        orc.lbvh_build(verts, faces, bits)
-        cdt = time.perf_counter() - c0
-        cpu = {"value": round(m / cdt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
-               "sample": f"oracle LBVH build (serial C++) of the same 2^{m.bit_length() - 1} "
-                         f"triangle soup, {cdt:.1f} s, host cores available: {os.cpu_count()}"}
+        cpu = cpu_lbvh(ca, soup, args.bits)
+        if host is not None:
+            cpu_r = cpu_render(ca, np, host, args.render_spp, 8)
+            log(f"cpu render: {cpu_r}")
 
     line = {
         "metric": "LBVH Mtri/s; path-trace Msamples/s wavefront vs megakernel; adjoint/fwd time",
-        "value": round(mtri_s, 2),
+        "value": round(lb["mtri_s"], 2),
         "unit": "Mtri/s",
         "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": round(ms_per_step, 4),
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(lb["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "LBVH build of a 2^24-triangle Gaussian-cluster soup "
+        "config": {"workload": f"LBVH build of a {args.tris}-triangle Gaussian-cluster soup "
                                "(64 clusters, sigma 0.02, edges 1e-3..1e-2, seed 1), "
                                "30-bit Morton (10 bits/axis), 48-B node output",
-                   "tris": n, "bits_per_dim": bits, "parallelism": "single GPU"},
-        "roofline": roofline,
+                   "tris": args.tris, "bits_per_dim": args.bits, "parallelism": "single GPU"},
+        "roofline": lb["roofline"],
         "cpu_baseline": cpu,
-        "stages": stage_info,
-        "build_effective_GB_s": round(whole_gbs, 1),
+        "stages": lb["stages"],
+        "build_effective_GB_s": lb["build_effective_GB_s"],
         "trace": trace,
+        "render": render,
+        "adjoint": adjoint,
+        "cpu_render_baseline": cpu_r,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

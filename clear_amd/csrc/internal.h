@@ -22,7 +22,7 @@ struct clr_ctx {
     // Profiling of the LBVH build stages: one event per stage boundary for up
     // to kMaxProfiledBuilds builds since profiling was (re)enabled.
     static constexpr int kMaxProfiledBuilds = 64;
-    static constexpr int kStageEvents = 5;
+    static constexpr int kStageEvents = 6;
     bool profiling = false;
     hipEvent_t events[kMaxProfiledBuilds][kStageEvents] = {};
     int profiledBuilds = 0;

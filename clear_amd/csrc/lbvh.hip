@@ -851,10 +851,11 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
     k_hier_local<SOUP, KeyT><<<localBlocks, kLocalLeaves, 0, stream>>>(
         src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend);
+    ev(4);
     k_hier_global<KeyT><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
         kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend,
         2 * localBlocks);
-    ev(4);
+    ev(5);
     if (prof) ++ctx->profiledBuilds;
     return hipCheck(ctx, hipGetLastError(), "LBVH build launch");
 }

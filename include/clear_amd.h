@@ -143,7 +143,8 @@ int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* se
  * HIP events at the stage boundaries of every following build (up to 64
  * builds, on the build's own stream); clr_lbvh_last_profile returns the mean
  * per-stage GPU time in ms over the recorded builds, stages =
- * [bounds, morton+histograms, radix sort, hierarchy]. It waits for the
+ * [bounds, morton+histograms, radix sort, block-local hierarchy (k_hier_local),
+ * cross-block hierarchy (k_hier_global)]. It waits for the
  * recorded builds to finish. (Replaces the profiling prints of perform,
  * LBVHBuilder.cpp:173-349.) */
 int clr_set_profiling(clr_ctx* ctx, int enable);
