@@ -32,6 +32,8 @@
 // end, which fixes every node's index on the way up.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <algorithm>
 
 #include "internal.h"
@@ -450,22 +452,50 @@ __device__ __forceinline__ int clzKey<uint64_t>(uint64_t v) {
 
 // delta(x, x+1) over extended keys (key, sorted position); the reference's
 // _numCommonBits (lbvh_construction.cl:322-332) for adjacent positions.
+// `base` is the first sorted position of x's scene (0 for a single build):
+// equal keys fall back to the scene-local positions, so a scene built inside a
+// segmented build gets the tree it gets alone.
 template <typename KeyT>
-__device__ __forceinline__ int deltaAdj(const KeyT* __restrict__ k, uint32_t x, int keyBits) {
+__device__ __forceinline__ int deltaAdj(const KeyT* __restrict__ k, uint32_t x, int keyBits, uint32_t base = 0) {
     const KeyT a = k[x], b = k[x + 1];
     if (a != b) return clzKey<KeyT>(a ^ b) - (int)(8 * sizeof(KeyT)) + keyBits;
-    return keyBits + __clz((int)(x ^ (x + 1u)));
+    return keyBits + __clz((int)((x - base) ^ (x + 1u - base)));
 }
 
+// Segment view of a segmented build: per sorted position its scene, per scene
+// its first position (segOff[s]) and end (segOff[s + 1]). Scenes keep their
+// input ranges through the sort (scene id is the key's top digit), so a
+// position's scene is that of the input triangle at the same index.
+struct SegView {
+    const uint32_t* segOf;
+    const uint32_t* segOff;
+};
+
+// delta(x, x+1), -1 outside [0, n) and, in a segmented build, across scenes.
+template <typename KeyT, bool SEG>
+__device__ __forceinline__ int deltaAt(const KeyT* __restrict__ k, int64_t x, uint32_t n, int keyBits,
+                                       const SegView& sv) {
+    if (x < 0 || x + 1 >= (int64_t)n) return -1;
+    if constexpr (SEG) {
+        const uint32_t s = sv.segOf[x];
+        if (sv.segOf[x + 1] != s) return -1;
+        return deltaAdj<KeyT>(k, (uint32_t)x, keyBits, sv.segOff[s]);
+    } else {
+        return deltaAdj<KeyT>(k, (uint32_t)x, keyBits);
+    }
+}
+
+// Node record; children are split - base and split + 1 - base (scene-local in
+// a segmented build).
 __device__ __forceinline__ void writeNode(clr_lbvh_inode* nodes, uint32_t idx, const Box& b,
-                                          uint32_t split, uint32_t l, uint32_t r) {
+                                          uint32_t split, uint32_t l, uint32_t r, uint32_t base = 0) {
     uint4* o = reinterpret_cast<uint4*>(nodes + idx);
     o[0] = make_uint4(__float_as_uint(b.lo[0]), __float_as_uint(b.lo[1]),
                       __float_as_uint(b.lo[2]), 0u);
     o[1] = make_uint4(__float_as_uint(b.hi[0]), __float_as_uint(b.hi[1]),
                       __float_as_uint(b.hi[2]), 0u);
     const uint32_t flags = (split == l ? 1u : 0u) | (split + 1u == r ? 0x100u : 0u);
-    o[2] = make_uint4(flags, split, split + 1u, 0u);
+    o[2] = make_uint4(flags, split - base, split + 1u - base, 0u);
 }
 
 __device__ __forceinline__ Box leafBox(const clr_lbvh_leaf* leaves, uint32_t i) {
@@ -551,12 +581,16 @@ struct Pending {
 };
 static_assert(sizeof(Pending) == 64, "pending record layout");
 
-template <bool SOUP, typename KeyT>
+// SEG: segmented build (sv describes the scenes): deltas are -1 across
+// scenes, a scene's root is the node covering its whole range (stored at its
+// first inode slot), inodes of scene s are numbered locally and stored at
+// segOff[s] - s + local, children and leaf object indices are scene-local.
+template <bool SOUP, typename KeyT, bool SEG = false>
 __global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
     PrimSrc src, const KeyT* __restrict__ skeys, const uint32_t* __restrict__ sidx, uint32_t n,
     int keyBits, uint32_t* __restrict__ otherEnd, uint64_t* __restrict__ xchg, uint32_t epoch,
     clr_lbvh_inode* __restrict__ inodes, clr_lbvh_leaf* __restrict__ leaves,
-    Pending* __restrict__ pend) {
+    Pending* __restrict__ pend, SegView sv = SegView{nullptr, nullptr}) {
     constexpr int B = kLocalLeaves;
     __shared__ uint32_t s_other[B];
     __shared__ float s_lbox[6][B];
@@ -568,11 +602,17 @@ __global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
     const uint32_t be = min(bs + (uint32_t)B, n);
     const uint32_t i = bs + t;
     s_other[t] = kEmpty;
+    uint32_t seg = 0, segS = 0, segE = n;
+    if (SEG && i < be) {
+        seg = sv.segOf[i];
+        segS = sv.segOff[seg];
+        segE = sv.segOff[seg + 1];
+    }
     if (i < be) {
         // Leaf i: gather the sorted primitive, write its LeafNode record.
         const uint32_t obj = sidx[i];
         const Box b = loadBoxRec(src.boxes, obj);
-        writeLeaf(leaves, i, b, obj);
+        writeLeaf(leaves, i, b, obj - segS);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             s_lbox[k][t] = b.lo[k];
@@ -581,16 +621,14 @@ __global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
     }
     for (uint32_t j = t; j <= (uint32_t)B; j += B) {
         const int64_t x = (int64_t)bs - 1 + j;
-        s_delta[j] = (x >= 0 && x + 1 < (int64_t)n && x < (int64_t)be)
-                         ? deltaAdj<KeyT>(skeys, (uint32_t)x, keyBits)
-                         : -1;
+        s_delta[j] = x < (int64_t)be ? deltaAt<KeyT, SEG>(skeys, x, n, keyBits, sv) : -1;
     }
     if (t < 2) pend[2 * blockIdx.x + t].valid = 0u;
     __syncthreads();
 
     // delta(x, x+1) for x in [bs-1, be-1]; x = bs-1 may be -1 (then -1).
     auto D = [&](int64_t x) -> int { return s_delta[x - (int64_t)bs + 1]; };
-    if (i < be && n > 1) {
+    if (i < be && (SEG || n > 1)) {
         Box box;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -601,15 +639,15 @@ __global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
         int dl = D((int64_t)l - 1), dr = D(r);  // delta(l-1, l), delta(r, r+1)
         bool internal = false;
         while (true) {
-            if (l == 0 && r == n - 1) {
-                writeNode(inodes, 0, box, split, l, r);
+            if (l == segS && r == segE - 1) {
+                if (internal) writeNode(inodes, segS - seg, box, split, l, r, segS);
                 break;
             }
             const bool isLeft = dr > dl;
             const uint32_t parentSplit = isLeft ? r : l - 1;
             if (internal) {
                 const uint32_t self = isLeft ? r : l;
-                writeNode(inodes, self, box, split, l, r);
+                writeNode(inodes, self - seg, box, split, l, r, segS);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     s_ibox[k][self - bs] = box.lo[k];
@@ -684,12 +722,12 @@ __global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
     }
 }
 
-template <typename KeyT>
+template <typename KeyT, bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_hier_global(
     const KeyT* __restrict__ skeys, uint32_t n, int keyBits, uint32_t* __restrict__ otherEnd,
     uint64_t* __restrict__ xchg, uint32_t epoch, uint32_t* __restrict__ devStatus,
     clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
-    const Pending* __restrict__ pend, uint32_t numPending) {
+    const Pending* __restrict__ pend, uint32_t numPending, SegView sv = SegView{nullptr, nullptr}) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= numPending) return;
     const Pending P = pend[p];
@@ -703,17 +741,23 @@ __global__ __launch_bounds__(kBlock) void k_hier_global(
         box.lo[k] = P.box[k];
         box.hi[k] = P.box[3 + k];
     }
+    uint32_t seg = 0, segS = 0, segE = n;
+    if constexpr (SEG) {
+        seg = sv.segOf[l];
+        segS = sv.segOff[seg];
+        segE = sv.segOff[seg + 1];
+    }
     bool written = true;  // phase 1 already wrote this node's record
     while (true) {
-        if (l == 0 && r == n - 1) {
-            writeNode(inodes, 0, box, split, l, r);
+        if (l == segS && r == segE - 1) {
+            if (internal) writeNode(inodes, segS - seg, box, split, l, r, segS);
             return;
         }
         const bool isLeft = dr > dl;
         const uint32_t parentSplit = isLeft ? r : l - 1;
         if (internal) {
             const uint32_t self = isLeft ? r : l;
-            if (!written) writeNode(inodes, self, box, split, l, r);
+            if (!written) writeNode(inodes, self - seg, box, split, l, r, segS);
             publishBox(xchg + 6 * (size_t)self, box, epoch);
         }
         written = false;
@@ -726,9 +770,9 @@ __global__ __launch_bounds__(kBlock) void k_hier_global(
         const uint32_t s = isLeft ? parentSplit + 1u : parentSplit;
         int dNew;
         if (isLeft)
-            dNew = other < n - 1 ? deltaAdj<KeyT>(skeys, other, keyBits) : -1;
+            dNew = deltaAt<KeyT, SEG>(skeys, (int64_t)other, n, keyBits, sv);
         else
-            dNew = other > 0 ? deltaAdj<KeyT>(skeys, other - 1, keyBits) : -1;
+            dNew = deltaAt<KeyT, SEG>(skeys, (int64_t)other - 1, n, keyBits, sv);
         const Box sb = (s == other) ? leafBox(leaves, s)
                                     : awaitBox(xchg + 6 * (size_t)s, epoch, devStatus);
         if (isLeft) {
@@ -775,12 +819,9 @@ inline uint32_t tileFor(uint32_t bits) {
                       : (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems);
 }
 
-Layout layoutFor(void* base, uint32_t n, uint32_t bits) {
-    Carve c(base);
+Layout layoutFor(Carve& c, uint32_t n, size_t keyBytes, int P, uint32_t tile) {
     Layout L{};
-    const int P = numPassesFor(bits);
-    const uint32_t parts = (n + tileFor(bits) - 1) / tileFor(bits);
-    const size_t keyBytes = bits <= 10 ? 4 : 8;
+    const uint32_t parts = (n + tile - 1) / tile;
     L.sceneOrd = c.take<uint32_t>(8);
     // hist, counters and status carved back to back so one fill clears them
     L.hist = c.take<uint32_t>((size_t)kMaxPasses * 256 + 64 + (size_t)P * parts * 256);
@@ -796,6 +837,10 @@ Layout layoutFor(void* base, uint32_t n, uint32_t bits) {
     L.pend = c.take<Pending>(2 * (size_t)((n + kLocalLeaves - 1) / kLocalLeaves));
     L.bytes = c.off + 256;
     return L;
+}
+Layout layoutFor(void* base, uint32_t n, uint32_t bits) {
+    Carve c(base);
+    return layoutFor(c, n, bits <= 10 ? 4 : 8, numPassesFor(bits), tileFor(bits));
 }
 
 inline unsigned gridFor(uint32_t n, unsigned cap) {
@@ -860,6 +905,214 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     return hipCheck(ctx, hipGetLastError(), "LBVH build launch");
 }
 
+// ------------------------------------------------------ segmented build ----
+// Many scenes (contiguous ranges of one triangle soup) in one set of
+// launches. The host splits every scene into chunks of at most kSegChunk
+// triangles; a block handles one chunk, so it knows its scene without a
+// search. Keys are (scene << 3 * bits) | Morton code within the scene's own
+// box, so one LSD sort orders scenes by index and each scene by its codes with
+// the index tie-break, exactly as a build of that scene alone would.
+constexpr uint32_t kSegChunk = 4096;
+
+__global__ __launch_bounds__(kBlock) void k_seg_init(uint32_t* segBox, uint32_t numSegments) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < 6 * numSegments) segBox[i] = (i % 6) < 3 ? ordOf(INFINITY) : ordOf(-INFINITY);
+}
+
+// Per-scene box of the triangle boxes (block reduction per chunk, then
+// ordered-int atomics on the scene's six words).
+__global__ __launch_bounds__(kBlock) void k_seg_bounds(const float* __restrict__ tris,
+                                                       const uint4* __restrict__ chunks,
+                                                       uint32_t* __restrict__ segBox) {
+    const uint4 ch = chunks[blockIdx.x];
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = ch.y + threadIdx.x; i < ch.z; i += kBlock) {
+        const Box b = soupBox(tris, i);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = omin(lo[k], b.lo[k]);
+            hi[k] = omax(hi[k], b.hi[k]);
+        }
+    }
+    __shared__ float s_red[kBlock / 64][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = waveMin(lo[k]);
+        hi[k] = waveMax(hi[k]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_red[wave][k] = lo[k];
+            s_red[wave][3 + k] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float v = s_red[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) v = k < 3 ? omin(v, s_red[w][k]) : omax(v, s_red[w][k]);
+        uint32_t* box = segBox + 6 * (size_t)ch.x;
+        if (k < 3) atomicMin(box + k, ordOf(v));
+        else atomicMax(box + k, ordOf(v));
+    }
+}
+
+// Box records, scene-tagged 64-bit keys, per-position scene ids, arrival words
+// and the digit histograms of all passes.
+__global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__ tris, float4* __restrict__ boxes,
+                                                       uint32_t n, uint32_t bits, const uint4* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ segBox,
+                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ segOf,
+                                                       uint32_t* __restrict__ otherEnd, uint32_t* __restrict__ hist,
+                                                       int numPasses) {
+    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    for (int j = threadIdx.x; j < numPasses * 256; j += kBlock) s_hist[j] = 0u;
+    __syncthreads();
+    const uint4 ch = chunks[blockIdx.x];
+    const uint32_t* box = segBox + 6 * (size_t)ch.x;
+    float smin[3], ssize[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        smin[k] = floatOf(box[k]);
+        ssize[k] = floatOf(box[3 + k]) - smin[k];
+    }
+    const uint32_t numDivs = 1u << bits;
+    const float fDivs = (float)numDivs;
+    const uint64_t tag = (uint64_t)ch.x << (3 * bits);
+    for (uint32_t i = ch.y + threadIdx.x; i < ch.z; i += kBlock) {
+        const Box b = soupBox(tris, i);
+        storeBoxRec(boxes, i, b);
+        uint32_t q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float c = (b.lo[k] + b.hi[k]) * 0.5f;
+            const float np = (c - smin[k]) / ssize[k];
+            q[k] = quantize(np * fDivs, numDivs);
+        }
+        const uint64_t key =
+            tag | spread3<uint64_t>(q[0]) | (spread3<uint64_t>(q[1]) << 1) | (spread3<uint64_t>(q[2]) << 2);
+        keys[i] = key;
+        segOf[i] = ch.x;
+        if (i + 1 < n) otherEnd[i] = kEmpty;
+        for (int p = 0; p < numPasses; ++p)
+            atomicAdd(&s_hist[p * 256 + (uint32_t)((key >> (8 * p)) & 0xFFu)], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < numPasses * 256; j += kBlock)
+        if (s_hist[j]) atomicAdd(hist + j, s_hist[j]);
+}
+
+struct SegLayout {
+    Layout L;
+    uint32_t* segOf;
+    uint32_t* segOff;
+    uint32_t* segBox;
+    uint4* chunks;
+    size_t bytes;
+};
+
+int segKeyBits(uint32_t bits, uint32_t numSegments) {
+    int sb = 1;
+    while (sb < 32 && (1u << sb) < numSegments) ++sb;
+    return (int)(3 * bits) + sb;
+}
+
+SegLayout segLayoutFor(void* base, uint32_t n, uint32_t numSegments, uint32_t numChunks, int P) {
+    Carve c(base);
+    SegLayout S;
+    S.L = layoutFor(c, n, 8, P, (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems));
+    S.segOf = c.take<uint32_t>(n);
+    S.segOff = c.take<uint32_t>((size_t)numSegments + 1);
+    S.segBox = c.take<uint32_t>(6 * (size_t)numSegments);
+    S.chunks = c.take<uint4>(numChunks);
+    S.bytes = c.off + 256;
+    return S;
+}
+
+uint32_t countChunks(const uint32_t* off, uint32_t numSegments) {
+    uint64_t c = 0;
+    for (uint32_t s = 0; s < numSegments; ++s) c += (off[s + 1] - off[s] + kSegChunk - 1) / kSegChunk;
+    return (uint32_t)c;
+}
+
+}  // namespace
+
+size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bitsPerDim) {
+    const int P = (segKeyBits(bitsPerDim, numSegments) + 7) / 8;
+    const uint32_t chunks = totalTris / kSegChunk + numSegments;  // upper bound
+    return segLayoutFor(nullptr, totalTris, numSegments, chunks, P).bytes;
+}
+
+int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegOffsets, uint32_t numSegments,
+                       uint32_t bits, clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
+    const uint32_t n = hostSegOffsets[numSegments];
+    const int keyBits = segKeyBits(bits, numSegments);
+    const int P = (keyBits + 7) / 8;
+    if (P > kMaxPasses) return fail(ctx, CLR_ERR_UNSUPPORTED, "too many segments for 64-bit keys");
+    int rc = ensurePool(ctx, lbvhSegScratchBytes(n, numSegments, bits));
+    if (rc) return rc;
+    rc = ensureStatus(ctx);
+    if (rc) return rc;
+    uint32_t epoch = 0;
+    rc = ensureExchange(ctx, n, &epoch);
+    if (rc) return rc;
+    const uint32_t numChunks = countChunks(hostSegOffsets, numSegments);
+    SegLayout S = segLayoutFor(ctx->pool, n, numSegments, numChunks, P);
+    Layout& L = S.L;
+    // Chunk table and scene offsets: built on the host (the caller passes
+    // host offsets), copied once.
+    std::vector<uint4> chunks;
+    chunks.reserve(numChunks);
+    for (uint32_t s = 0; s < numSegments; ++s)
+        for (uint32_t b = hostSegOffsets[s]; b < hostSegOffsets[s + 1]; b += kSegChunk)
+            chunks.push_back(make_uint4(s, b, std::min(b + kSegChunk, hostSegOffsets[s + 1]), 0u));
+    rc = hipCheck(ctx, hipMemcpyAsync(S.chunks, chunks.data(), sizeof(uint4) * numChunks, hipMemcpyHostToDevice,
+                                      stream), "chunk table copy");
+    if (!rc)
+        rc = hipCheck(ctx, hipMemcpyAsync(S.segOff, hostSegOffsets, sizeof(uint32_t) * (numSegments + 1),
+                                          hipMemcpyHostToDevice, stream), "segment offsets copy");
+    // The copies read pageable host memory: finish them before the vector goes.
+    if (!rc) rc = hipCheck(ctx, hipStreamSynchronize(stream), "segment table sync");
+    if (rc) return rc;
+    const uint32_t tile = (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems);
+    const uint32_t parts = (n + tile - 1) / tile;
+    k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
+        L.hist, L.clearWords, nullptr);
+    k_seg_init<<<(6 * numSegments + kBlock - 1) / kBlock, kBlock, 0, stream>>>(S.segBox, numSegments);
+    k_seg_bounds<<<numChunks, kBlock, 0, stream>>>(tris, S.chunks, S.segBox);
+    k_seg_morton<<<numChunks, kBlock, 0, stream>>>(tris, L.boxes, n, bits, S.chunks, S.segBox,
+                                                   static_cast<uint64_t*>(L.keysA), S.segOf, L.otherEnd, L.hist, P);
+    uint64_t* kin = static_cast<uint64_t*>(L.keysA);
+    uint64_t* kout = static_cast<uint64_t*>(L.keysB);
+    uint32_t* vin = L.valsA;
+    uint32_t* vout = L.valsB;
+    for (int p = 0; p < P; ++p) {
+        uint32_t* st = L.status + (size_t)p * parts * 256;
+        if (p == 0)
+            k_onesweep<uint64_t, true><<<parts, kSortBlock, 0, stream>>>(kin, vin, kout, vout, n, 8 * p,
+                                                                         L.hist + 256 * p, st, L.partCounters + p,
+                                                                         ctx->devStatus);
+        else
+            k_onesweep<uint64_t, false><<<parts, kSortBlock, 0, stream>>>(kin, vin, kout, vout, n, 8 * p,
+                                                                          L.hist + 256 * p, st, L.partCounters + p,
+                                                                          ctx->devStatus);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    const SegView sv{S.segOf, S.segOff};
+    PrimSrc src{tris, L.boxes};
+    const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
+    k_hier_local<true, uint64_t, true><<<localBlocks, kLocalLeaves, 0, stream>>>(
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, sv);
+    k_hier_global<uint64_t, true><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend, 2 * localBlocks, sv);
+    return hipCheck(ctx, hipGetLastError(), "segmented LBVH build launch");
+}
+
+namespace {
 }  // namespace
 
 void fillZero(uint32_t* words, size_t count, hipStream_t stream) {

@@ -139,3 +139,75 @@ def test_full_size_16m_properties(gpu_ctx):
     leaf_children = c[leaf]
     assert np.array_equal(np.sort(internal_children), np.arange(1, n - 1))
     assert np.array_equal(np.sort(leaf_children), np.arange(n))
+
+
+# ---- segmented build ---------------------------------------------------------
+
+def segmented_soup(sizes, seed0):
+    soups = [ca.gen_cluster_soup(int(m), seed=seed0 + k) for k, m in enumerate(sizes)]
+    offs = np.zeros(len(sizes) + 1, np.uint32)
+    offs[1:] = np.cumsum(sizes)
+    return np.concatenate(soups, axis=0), offs, soups
+
+
+def run_segmented(ctx, soup, offs, bits=10):
+    import torch
+
+    n = int(offs[-1])
+    k = len(offs) - 1
+    inodes = torch.zeros(max(n - k, 1) * 48, dtype=torch.uint8, device="cuda")
+    leaves = torch.zeros(n * 48, dtype=torch.uint8, device="cuda")
+    ca.LBVHBuilder(ctx, n, bits).perform_segmented(torch.from_numpy(soup).cuda(), offs, bits,
+                                                   inodes, leaves)
+    torch.cuda.synchronize()
+    assert ctx.device_status() == 0
+    return inodes.cpu().numpy().reshape(-1, 48), leaves.cpu().numpy().reshape(-1, 48)
+
+
+def test_segmented_matches_oracle_per_scene(gpu_ctx):
+    """Scenes of 1, 2, 3 ... 20,000 triangles built in one segmented build equal
+    the oracle's build of each scene alone, byte for byte."""
+    rng = np.random.default_rng(11)
+    sizes = [1, 2, 3, 5, 1000, 1023, 1024, 1025] + list(np.exp(rng.uniform(0, np.log(20000), 29)).astype(int) + 1)
+    soup, offs, soups = segmented_soup(sizes, 300)
+    print(f"[lbvh segmented] {len(sizes)} scenes, {int(offs[-1])} triangles", flush=True)
+    gi, gl = run_segmented(gpu_ctx, soup, offs)
+    for s, sp in enumerate(soups):
+        n = sp.shape[0]
+        v, f = ca.soup_to_mesh(sp)
+        o = orc.lbvh_build(v, f, 10)
+        b = int(offs[s])
+        np.testing.assert_array_equal(gl[b:b + n], o["leaves"], err_msg=f"scene {s} leaves")
+        if n > 1:
+            np.testing.assert_array_equal(gi[b - s:b - s + n - 1], o["inodes"], err_msg=f"scene {s} inodes")
+
+
+def test_segmented_256_scenes_match_single_builds(gpu_ctx):
+    """256 log-uniform scenes (1k..100k triangles) against one GPU build per scene."""
+    import torch
+
+    rng = np.random.default_rng(100)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), 256)).astype(int)
+    soup, offs, soups = segmented_soup(sizes, 100)
+    print(f"[lbvh segmented] 256 scenes, {int(offs[-1])} triangles", flush=True)
+    gi, gl = run_segmented(gpu_ctx, soup, offs)
+    for s in range(0, 256, 17):
+        n = soups[s].shape[0]
+        inodes, leaves = ca.node_buffers(n)
+        ca.LBVHBuilder(gpu_ctx, n, 10).perform_soup(torch.from_numpy(soups[s]).cuda(), n, 10, inodes, leaves)
+        torch.cuda.synchronize()
+        b = int(offs[s])
+        np.testing.assert_array_equal(gl[b:b + n], leaves.cpu().numpy().reshape(-1, 48))
+        np.testing.assert_array_equal(gi[b - s:b - s + n - 1], inodes.cpu().numpy().reshape(-1, 48)[: n - 1])
+
+
+def test_segmented_rejects_bad_offsets(gpu_ctx):
+    import torch
+
+    soup = ca.gen_cluster_soup(100, seed=1)
+    inodes = torch.zeros(100 * 48, dtype=torch.uint8, device="cuda")
+    leaves = torch.zeros(100 * 48, dtype=torch.uint8, device="cuda")
+    b = ca.LBVHBuilder(gpu_ctx, 100, 10)
+    for offs in ([0, 50, 50, 100], [1, 100], [0, 60, 40]):
+        with pytest.raises(ca.ClrError):
+            b.perform_segmented(torch.from_numpy(soup).cuda(), np.array(offs, np.uint32), 10, inodes, leaves)
