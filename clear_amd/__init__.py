@@ -67,6 +67,7 @@ def lib():
         "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
         "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "clr_render_batch": (i32, [vp, vp, u32, vp, vp, vp, vp]),
         "clr_scene_diff_params": (i32, [vp, ctypes.POINTER(u32), vp, vp, vp, ctypes.POINTER(u32),
                                         ctypes.POINTER(u32)]),
         "clr_wavefront_get_config": (i32, [vp, vp]),
@@ -277,7 +278,7 @@ def decode_leaves(buf: np.ndarray, n: int):
 
 # ---- scenes and rendering -------------------------------------------------
 
-SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN, SCENE_ADJOINT = 0, 1, 2, 3, 4
+SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN, SCENE_ADJOINT, SCENE_CLUSTERS = 0, 1, 2, 3, 4, 5
 ARRAY_NAMES = ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials",
                "textures", "others", "bvh_nodes"]
 RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1
@@ -288,7 +289,7 @@ class SceneParams(ctypes.Structure):
                 ("detail", ctypes.c_uint32), ("seed", ctypes.c_int),
                 ("lensRadius", ctypes.c_float), ("focusDistance", ctypes.c_float),
                 ("buildCpuBvh", ctypes.c_uint32), ("albedo", ctypes.c_float),
-                ("fovY", ctypes.c_float)]
+                ("fovY", ctypes.c_float), ("numTriangles", ctypes.c_uint32)]
 
 
 class SceneDev(ctypes.Structure):
@@ -311,9 +312,9 @@ class HostScene:
 
     def __init__(self, kind: int, width: int = 0, height: int = 0, detail: int = 0, seed: int = 0,
                  lens_radius: float = 0.0, focus_distance: float = 0.0, cpu_bvh: bool = False,
-                 albedo: float = 0.5, fov_y: float = 0.0):
+                 albedo: float = 0.5, fov_y: float = 0.0, num_triangles: int = 0):
         p = SceneParams(width, height, detail, seed, lens_radius, focus_distance, int(cpu_bvh),
-                        albedo, fov_y)
+                        albedo, fov_y, num_triangles)
         h = ctypes.c_void_p()
         rc = lib().clr_scene_build(kind, ctypes.byref(p), ctypes.byref(h))
         if rc:
@@ -371,6 +372,72 @@ class DeviceScene:
             _ptr(self.t["textures"]).value, _ptr(self.t["others"]).value,
             _ptr(self.inodes).value, _ptr(self.leaves).value, n, host.num_lights)
         self.width, self.height = host.width, host.height
+
+
+def soup_of(hosts: list, offsets: np.ndarray) -> np.ndarray:
+    """float32 [sum faces, 9]: every scene's triangles in face order (corner k
+    of face f = vertex p[k]), scene s at rows offsets[s]..offsets[s+1]."""
+    soup = np.empty((int(offsets[-1]), 9), np.float32)
+    for s, x in enumerate(hosts):
+        p = x.faces["p"].astype(np.int64)
+        o = int(offsets[s])
+        soup[o:o + x.num_faces] = x.vertices[p][:, :, :3].reshape(-1, 9)
+    return soup
+
+
+class SceneBatch:
+    """Many scenes resident on the GPU whose LBVHs come from ONE segmented
+    build (clr_lbvh_build_segmented) over their concatenated triangles; scene
+    s's nodes sit at leaves[offsets[s]] and inodes[offsets[s] - s], exactly
+    the nodes clr_lbvh_build gives for that scene alone. `devs` is the
+    clr_scene_dev array clr_render_batch takes."""
+
+    def __init__(self, ctx: Context, hosts: list, bits: int = 10, device="cuda", stream=None):
+        import torch
+
+        if not hosts:
+            raise ValueError("empty scene batch")
+        w, h = hosts[0].width, hosts[0].height
+        if any(x.width != w or x.height != h for x in hosts):
+            raise ValueError("all scenes of a batch render at one image size")
+        self.ctx, self.hosts, self.width, self.height = ctx, hosts, w, h
+        self.t = [x.to_device(device) for x in hosts]
+        counts = np.array([x.num_faces for x in hosts], np.int64)
+        self.offsets = np.zeros(len(hosts) + 1, np.uint32)
+        self.offsets[1:] = np.cumsum(counts)
+        total = int(self.offsets[-1])
+        self.tris_host = soup_of(hosts, self.offsets)
+        self.tris = torch.from_numpy(self.tris_host.reshape(-1)).to(device)
+        self.inodes = torch.empty(max(total - len(hosts), 1) * INODE_BYTES, dtype=torch.uint8, device=device)
+        self.leaves = torch.empty(total * LEAF_BYTES, dtype=torch.uint8, device=device)
+        self.build(bits, stream)
+        ib, lb = self.inodes.data_ptr(), self.leaves.data_ptr()
+        self.devs = (SceneDev * len(hosts))()
+        for s, (x, t) in enumerate(zip(hosts, self.t)):
+            o = int(self.offsets[s])
+            self.devs[s] = SceneDev(
+                _ptr(t["vertices"]).value, _ptr(t["normals"]).value, _ptr(t["tangents"]).value,
+                _ptr(t["uvs"]).value, _ptr(t["faces"]).value, _ptr(t["lights"]).value,
+                _ptr(t["materials"]).value, _ptr(t["textures"]).value, _ptr(t["others"]).value,
+                ib + (o - s) * INODE_BYTES if x.num_faces > 1 else None, lb + o * LEAF_BYTES,
+                x.num_faces, x.num_lights)
+
+    def build(self, bits: int = 10, stream=None) -> None:
+        """(Re)builds every scene's LBVH in one segmented build."""
+        b = LBVHBuilder(self.ctx, 0, bits)
+        b.perform_segmented(self.tris, self.offsets, bits, self.inodes, self.leaves, stream)
+
+    def __len__(self) -> int:
+        return len(self.hosts)
+
+
+def render_batch(ctx: Context, batch: SceneBatch, spp: int, max_depth: int, states, pixels,
+                 stream=None, flags: int = 0) -> None:
+    """clr_render_batch: every scene of `batch` in one wavefront pass; states
+    and pixels hold the scenes' images back to back."""
+    rp = RenderParams(batch.width, batch.height, spp, max_depth, flags)
+    ctx.check(lib().clr_render_batch(ctx.handle, batch.devs, len(batch), ctypes.byref(rp), _ptr(states),
+                                     _ptr(pixels), _stream(stream)))
 
 
 def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None) -> None:

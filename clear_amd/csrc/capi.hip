@@ -204,6 +204,7 @@ int clr_destroy(clr_ctx* ctx) {
         if (ctx->traceSpill) (void)hipFree(ctx->traceSpill);
         if (ctx->accel) (void)hipFree(ctx->accel);
         if (ctx->adjTable) (void)hipFree(ctx->adjTable);
+        if (ctx->batch) (void)hipFree(ctx->batch);
         destroyWfState(ctx);
         for (auto& row : ctx->events)
             for (auto& e : row)
@@ -406,6 +407,30 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
     if (mode == CLR_RENDER_MEGAKERNEL) return renderMegakernel(ctx, *scene, *params, randStates, pixels, st);
     if (mode == CLR_RENDER_WAVEFRONT) return renderWavefront(ctx, *scene, *params, randStates, pixels, st);
     return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
+}
+
+int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                     const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
+                     void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!scenes || !randStates || !pixels || !params) return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
+    if (numScenes == 0) return CLR_OK;
+    for (uint32_t s = 0; s < numScenes; ++s) {
+        int rc = checkRenderArgs(ctx, scenes + s, params);
+        if (rc) return rc;
+    }
+    if (!aligned16(randStates) || !aligned16(pixels))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
+    if (params->maxDepth > 255) return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
+    if (params->flags & CLR_RENDER_REFERENCE_TRAVERSAL)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "batched render uses the ordered traversal only");
+    const uint64_t tiles = (uint64_t)((params->width + 7) / 8) * ((params->height + 7) / 8) * 64;
+    if ((uint64_t)params->width * params->height * numScenes >= (1ull << 31) || tiles * numScenes >= (1ull << 31))
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "batch too large (scenes x pixels >= 2^31)");
+    if (params->samplesPerPixel == 0) return CLR_OK;
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return renderWavefrontBatch(ctx, scenes, numScenes, *params, randStates, pixels, static_cast<hipStream_t>(stream));
 }
 
 int clr_render_adjoint(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,

@@ -45,6 +45,9 @@ struct clr_ctx {
     void* wfPool = nullptr;
     size_t wfPoolBytes = 0;
     clr::WfState* wf = nullptr;
+    // Batched rendering: scene table, traversal layouts, face-light maps.
+    void* batch = nullptr;
+    size_t batchBytes = 0;
     // Adjoint: matPtr -> parameter table (65536 x uint16) + 256 offsets.
     void* adjTable = nullptr;
     // Traversal layout of the ordered walk (rebuilt by every call that uses it).
@@ -131,6 +134,15 @@ int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_p
 
 // ---- traversal (traverse.hip) ----
 int ensureTraceSpill(clr_ctx* ctx, size_t bytes);
+// Launches the traversal-layout build into caller-given arrays.
+void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+                      const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
+                      hipStream_t stream);
+// Face -> first light map and first environment light of one scene.
+void launchFaceLights(const clr_light_info* lights, uint32_t numLights, uint32_t numFaces, uint32_t* faceLight,
+                      int* envLight, hipStream_t stream);
+int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
 // Builds the ordered walk's traversal layout from the drop-in nodes on `stream`
 // into context memory and returns its two arrays.
 int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,

@@ -379,6 +379,26 @@ __global__ void k_seed_states(uint32_t width, uint32_t height, uint32_t* states)
 
 }  // namespace
 
+void launchFaceLights(const clr_light_info* lights, uint32_t numLights, uint32_t numFaces, uint32_t* faceLight,
+                      int* envLight, hipStream_t stream) {
+    k_face_lights_clear<<<(numFaces + 255) / 256 + 1, 256, 0, stream>>>(faceLight, numFaces, envLight);
+    if (numLights)
+        k_face_lights_fill<<<(numLights + 255) / 256, 256, 0, stream>>>(lights, numLights, numFaces, faceLight,
+                                                                        envLight);
+}
+
+// Resolves the blob-header fields of a device table of scenes in place.
+__global__ void k_resolve_scenes(dev::RScene* table, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    dev::RScene S = table[i];
+    dev::finishScene(S);
+    table[i] = S;
+}
+void launchResolveScenes(dev::RScene* table, uint32_t count, hipStream_t stream) {
+    k_resolve_scenes<<<(count + 63) / 64, 64, 0, stream>>>(table, count);
+}
+
 // Device-side scene view. The blob header (otherResources[0..2] = offsets of
 // the camera, the environment head and the light power distribution,
 // Scene.cpp:39-41) is read by the kernels themselves; the face->light map and

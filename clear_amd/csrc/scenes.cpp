@@ -562,6 +562,61 @@ void buildAdjoint(clr_host_scene* hs, const clr_scene_params& p) {
                    p.fovY > 0 ? p.fovY : 60.0f, 0.0f, 1.0f);
 }
 
+// Batch-render scene: the cluster soup (faces 0..n-1, material i mod 8),
+// the area light quad (faces n, n+1), a white floor (faces n+2, n+3).
+void buildClusters(clr_host_scene* hs, const clr_scene_params& p) {
+    Scene& s = hs->scene;
+    MaterialCreator& mc = MaterialCreator::sharedInstance();
+    mc.setScene(&s);
+    const int seed = p.seed ? p.seed : 100;
+    Rng rng(seed);
+    mc.createFloatConstantTexture("sigma0", 0.0f);
+    mc.createFloat3ConstantTexture("R_coat", 1.0f, 1.0f, 1.0f);
+    mc.createFloat3ConstantTexture("R_white", 0.75f, 0.75f, 0.75f);
+    mc.createMatteMaterial("mat_white", nullptr, "R_white", "sigma0");
+    uint16_t mats[8];
+    for (int k = 0; k < 8; ++k) {
+        const std::string rd = "Rd_" + std::to_string(k), m = "mat_" + std::to_string(k);
+        const float r = 0.3f + 0.6f * rng.uniform(), g = 0.3f + 0.6f * rng.uniform(), b = 0.3f + 0.6f * rng.uniform();
+        mc.createFloat3ConstantTexture(rd.c_str(), r, g, b);
+        if (k < 4) {
+            mc.createMatteMaterial(m.c_str(), nullptr, rd.c_str(), "sigma0");
+        } else {
+            const std::string al = "alpha_" + std::to_string(k);
+            mc.createFloatConstantTexture(al.c_str(), 0.05f + 0.45f * rng.uniform());
+            mc.createCoatedMaterial(m.c_str(), nullptr, rd.c_str(), "sigma0", "R_coat", al.c_str(), 1.5f);
+        }
+        mats[k] = (uint16_t)s.idxOfMat(m);
+    }
+    mc.createFloat3ConstantTexture("M_light", 10.0f, 10.0f, 10.0f);
+    mc.createDiffuseLightProperty("light_top", "M_light");
+    const uint32_t n = p.numTriangles ? p.numTriangles : 10000;
+    Mesh soup;
+    soup.p.resize(9 * (size_t)n);
+    if (clr_gen_cluster_soup(n, seed, 64, 0.02f, 1e-3f, 1e-2f, soup.p.data()) != 0)
+        throw std::runtime_error("cluster soup");
+    soup.tri.resize(3 * (size_t)n);
+    for (size_t i = 0; i < soup.tri.size(); ++i) soup.tri[i] = (uint32_t)i;
+    addMesh(s, soup, false, false, false, [&](size_t f) { return mats[f % 8]; });
+    const uint16_t white = (uint16_t)s.idxOfMat("mat_white");
+    s.localToWorld.push();
+    s.localToWorld *= Mat4::translate(0.5f, 1.3f, 0.5f);
+    addCeilingLight(s, 0.3f, 0.0f, white, (uint16_t)s.idxOfLight("light_top"));
+    s.localToWorld.pop();
+    // Floor [-1.5, 2.5]^2 at y = -0.1, facing up.
+    s.beginObject();
+    s.addVertex(-1.5f, -0.1f, -1.5f);
+    s.addVertex(2.5f, -0.1f, -1.5f);
+    s.addVertex(2.5f, -0.1f, 2.5f);
+    s.addVertex(-1.5f, -0.1f, 2.5f);
+    s.addFace(Face_P(0, 2, 1, white));
+    s.addFace(Face_P(0, 3, 2, white));
+    s.endObject();
+    const float eye[3] = {0.5f, 0.6f, 2.0f}, at[3] = {0.5f, 0.5f, 0.5f};
+    setPerspective(s, hs, p.width ? p.width : 128, p.height ? p.height : 128, eye, at,
+                   p.fovY > 0 ? p.fovY : 45.0f, 0.0f, 1.0f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -581,6 +636,7 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
             case CLR_SCENE_FURNACE: buildFurnace(hs, p); break;
             case CLR_SCENE_TERRAIN: buildTerrain(hs, p); break;
             case CLR_SCENE_ADJOINT: buildAdjoint(hs, p); break;
+            case CLR_SCENE_CLUSTERS: buildClusters(hs, p); break;
             default: delete hs; return CLR_ERR_UNSUPPORTED;
         }
         hs->scene.build();

@@ -152,6 +152,12 @@ int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* 
     return hipCheck(ctx, hipGetLastError(), "traversal layout launch");
 }
 
+void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
+                      const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
+                      hipStream_t stream) {
+    k_accel_build<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, vertices, faces, anodes, atris);
+}
+
 int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
                  const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
                  uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream) {

@@ -324,9 +324,9 @@ __device__ __forceinline__ bool triTestPacked(const float4* tris, V3 o, V3 d, ui
 // distance (occlusion queries).
 template <bool STATS, class Alpha, class Stack>
 struct Ordered {
-    const GeomScene& g;
+    GeomScene g;  // by value: rebind() switches scenes between rays
     const Stack& stack;
-    const Alpha& alpha;
+    Alpha alpha;
     V3 o, d, inv;
     float t, b1, b2;
     uint32_t best, node, steps;
@@ -336,6 +336,10 @@ struct Ordered {
 
     __device__ __forceinline__ Ordered(const GeomScene& g_, const Stack& s_, const Alpha& a_)
         : g(g_), stack(s_), alpha(a_) {}
+    __device__ __forceinline__ void rebind(const GeomScene& g_, const Alpha& a_) {
+        g = g_;
+        alpha = a_;
+    }
 
     __device__ __forceinline__ void leaf(uint32_t face) {
         if constexpr (STATS) ++st.tests;

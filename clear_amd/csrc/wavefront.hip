@@ -39,6 +39,7 @@
 namespace clr {
 
 int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hipStream_t stream);
+void launchResolveScenes(dev::RScene* table, uint32_t count, hipStream_t stream);
 
 namespace {
 
@@ -76,12 +77,31 @@ struct WfBufs {
 };
 
 struct WfArgs {
-    RScene S;
+    RScene S;                 // single-scene renders
+    const RScene* table;      // batched renders: resolved scenes, one per image
     WfBufs B;
     uint32_t width, height, maxDepth, flags, refillMin;
-    uint32_t* states;
-    float4* pixels;
+    uint32_t posPerScene;     // tile-major positions per image
+    uint32_t* states;         // [scene][pixel]
+    float4* pixels;           // [scene][pixel]
 };
+
+// The scene and image position of a slot. BATCH: positions are scene-major
+// (posPerScene per image) and the scene comes from the table; otherwise S is
+// the kernel's own resolved copy of A.S.
+template <bool BATCH>
+__device__ __forceinline__ void slotScene(const WfArgs& A, uint32_t v, RScene& S, uint32_t& vLocal,
+                                          uint32_t& pixBase) {
+    if constexpr (BATCH) {
+        const uint32_t sc = v / A.posPerScene;
+        vLocal = v - sc * A.posPerScene;
+        pixBase = sc * A.width * A.height;
+        S = A.table[sc];
+    } else {
+        vLocal = v;
+        pixBase = 0;
+    }
+}
 
 size_t wfBytes(uint32_t P, uint32_t lanes) {
     size_t b = 0;
@@ -213,8 +233,8 @@ __device__ __forceinline__ void startPath(const RScene& S, uint32_t pixel, uint3
 
 // End of one sample of `slot`: acc += pix; start the next sample (returns
 // true with the new ray) or, after the last one, write the pixel and state.
-__device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint32_t slot, Rng& rng, V3 pix, V3& org,
-                                          V3& dir) {
+__device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint32_t slot, uint32_t vLocal,
+                                          uint32_t pixBase, Rng& rng, V3 pix, V3& org, V3& dir) {
     const WfBufs& B = A.B;
     float4 acc = B.acc[slot];
     acc.x += pix.x;
@@ -224,7 +244,7 @@ __device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint
     acc.w = __uint_as_float(left);
     B.acc[slot] = acc;
     uint32_t pixel;
-    tilePixel(B.counts[kCntChunk] + slot, A.width, A.height, pixel);
+    tilePixel(vLocal, A.width, A.height, pixel);
     if (left > 0) {
         V3 alpha;
         float initY, dirPDF;
@@ -235,12 +255,12 @@ __device__ __forceinline__ bool endSample(const WfArgs& A, const RScene& S, uint
         B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
         return true;
     }
-    float4 p = A.pixels[pixel];
+    float4 p = A.pixels[pixBase + pixel];
     p.x += acc.x;
     p.y += acc.y;
     p.z += acc.z;
-    A.pixels[pixel] = p;
-    reinterpret_cast<uint4*>(A.states)[pixel] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+    A.pixels[pixBase + pixel] = p;
+    reinterpret_cast<uint4*>(A.states)[pixBase + pixel] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     return false;
 }
 
@@ -263,16 +283,18 @@ __global__ void k_wf_reset(WfArgs A, uint32_t chunk) {
 
 // First camera path of every in-image position of the chunk
 // [chunk, chunk + count) of tile-major positions.
+template <bool BATCH>
 __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp) {
     const WfBufs& B = A.B;
     RScene S = A.S;
-    finishScene(S);
+    if (!BATCH) finishScene(S);
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t pixel = 0;
-    const bool valid = slot < count && tilePixel(chunk + slot, A.width, A.height, pixel);
+    uint32_t pixel = 0, vLocal = 0, pixBase = 0;
+    if (slot < count) slotScene<BATCH>(A, chunk + slot, S, vLocal, pixBase);
+    const bool valid = slot < count && tilePixel(vLocal, A.width, A.height, pixel);
     V3 org = v3z(), dir = v3z();
     if (valid) {
-        const uint4 s = reinterpret_cast<const uint4*>(A.states)[pixel];
+        const uint4 s = reinterpret_cast<const uint4*>(A.states)[pixBase + pixel];
         Rng rng{s.x, s.y, s.z, s.w};
         V3 alpha;
         float initY, dirPDF;
@@ -288,11 +310,12 @@ __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp
 
 // Closest hit of extension queue `in`; resets the counters the shade / miss /
 // shadow kernels of this iteration append to.
-template <bool ORDERED, int LD>
+template <bool ORDERED, int LD, bool BATCH>
 __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     extern __shared__ uint32_t s_stack[];
     const WfBufs& B = A.B;
     const RScene S = A.S;
+    const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lanes = gridDim.x * blockDim.x;
@@ -313,6 +336,10 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
         uint32_t j = 0;
         while (true) {
             if (pool.refill(A.refillMin, busy, j)) {
+                if constexpr (BATCH) {
+                    const RScene* Sp = A.table + (chunk + B.rqSlot[in][j]) / A.posPerScene;
+                    w.rebind(Sp->g, TexAlpha{Sp});
+                }
                 w.begin(xyz(B.rqOrg[in][j]), xyz(B.rqDir[in][j]), INFINITY);
                 busy = true;
             }
@@ -356,10 +383,12 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
 }
 
 // Surface vertices (hit queue of extension queue `in`).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
     RScene S = A.S;
-    finishScene(S);
+    if (!BATCH) finishScene(S);
     const WfBufs& B = A.B;
+    const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
     const uint32_t lanes = gridDim.x * blockDim.x;
     const uint32_t n = min(B.counts[kCntHit], B.slots);
@@ -372,6 +401,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
         if (active) {
             const uint32_t j = B.hitQ[i];
             slot = B.rqSlot[in][j];
+            uint32_t vLocal, pixBase;
+            slotScene<BATCH>(A, chunk + slot, S, vLocal, pixBase);
             org = xyz(B.rqOrg[in][j]);
             dir = xyz(B.rqDir[in][j]);
             const uint4 h = B.hit[j];
@@ -464,7 +495,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
                 B.pix[slot] = f4(pix, dirPDF);
                 B.rng[slot] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
             } else if (!live) {
-                wantRay = endSample(A, S, slot, rng, pix, org, dir);
+                wantRay = endSample(A, S, slot, vLocal, pixBase, rng, pix, org, dir);
                 meta = 0u;
             } else {
                 B.alpha[slot] = f4(alpha, initY);
@@ -485,10 +516,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
 }
 
 // Environment vertices (miss queue of extension queue `in`).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
     RScene S = A.S;
-    finishScene(S);
+    if (!BATCH) finishScene(S);
     const WfBufs& B = A.B;
+    const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
     const uint32_t lanes = gridDim.x * blockDim.x;
     const uint32_t n = min(B.counts[kCntMiss], B.slots);
@@ -501,6 +534,8 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
         if (active) {
             const uint32_t j = B.missQ[i];
             slot = B.rqSlot[in][j];
+            uint32_t vLocal, pixBase;
+            slotScene<BATCH>(A, chunk + slot, S, vLocal, pixBase);
             dir = xyz(B.rqDir[in][j]);
             const uint4 rs = B.rng[slot];
             Rng rng{rs.x, rs.y, rs.z, rs.w};
@@ -522,7 +557,7 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
                 mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
             }
             pix += (mis * alpha) * evalLe(E, vout);
-            wantRay = endSample(A, S, slot, rng, pix, org, dir);
+            wantRay = endSample(A, S, slot, vLocal, pixBase, rng, pix, org, dir);
         }
         pushExt(B, out, active && wantRay, slot, org, dir);
     }
@@ -530,10 +565,11 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
 
 // Shadow rays of this iteration; resets the hit / miss counters for the next
 // iteration's extend kernel.
-template <bool ORDERED, int LD>
+template <bool ORDERED, int LD, bool BATCH>
 __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
     RScene S = A.S;
-    finishScene(S);
+    if (!BATCH) finishScene(S);
+    const uint32_t chunk = A.B.counts[kCntChunk];
     extern __shared__ uint32_t s_stack[];
     const WfBufs& B = A.B;
     const uint32_t out = in ^ 1u;
@@ -557,6 +593,10 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         uint32_t i = 0, target = kNone;
         while (true) {
             if (pool.refill(A.refillMin, busy, i)) {
+                if constexpr (BATCH) {
+                    const RScene* Sp = A.table + (chunk + B.sqSlot[i]) / A.posPerScene;
+                    w.rebind(Sp->g, TexAlpha{Sp});
+                }
                 const float4 o4 = B.sqOrg[i];
                 target = __float_as_uint(o4.w);
                 w.begin(xyz(o4), xyz(B.sqDir[i]), INFINITY, target == kNone);
@@ -616,7 +656,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
             if (B.meta[slot] & kFlagEndAfterShadow) {
                 const uint4 rs = B.rng[slot];
                 Rng rng{rs.x, rs.y, rs.z, rs.w};
-                wantRay = endSample(A, S, slot, rng, xyz(p4), org, dir);
+                wantRay = endSample(A, S, slot, chunk + slot, 0u, rng, xyz(p4), org, dir);
             } else {
                 B.pix[slot] = p4;
             }
@@ -647,10 +687,12 @@ __global__ __launch_bounds__(256) void k_wf_compact(WfArgs A, uint32_t in) {
 
 // Ends the samples whose last vertex had a light sample but no continuation
 // (their radiance is complete once the shadow kernel ran).
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_wf_shadow_end(WfArgs A, uint32_t in) {
     RScene S = A.S;
-    finishScene(S);
+    if (!BATCH) finishScene(S);
     const WfBufs& B = A.B;
+    const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
     const uint32_t n = min(B.counts[kCntShadow], B.slots);
     const uint32_t lanes = gridDim.x * blockDim.x;
@@ -662,9 +704,11 @@ __global__ __launch_bounds__(256) void k_wf_shadow_end(WfArgs A, uint32_t in) {
         if (i < n) {
             slot = B.sqSlot[i];
             if (B.meta[slot] & kFlagEndAfterShadow) {
+                uint32_t vLocal, pixBase;
+                slotScene<BATCH>(A, chunk + slot, S, vLocal, pixBase);
                 const uint4 rs = B.rng[slot];
                 Rng rng{rs.x, rs.y, rs.z, rs.w};
-                wantRay = endSample(A, S, slot, rng, xyz(B.pix[slot]), org, dir);
+                wantRay = endSample(A, S, slot, vLocal, pixBase, rng, xyz(B.pix[slot]), org, dir);
             }
         }
         pushExt(B, out, wantRay, slot, org, dir);
@@ -739,31 +783,37 @@ Launch shadeLaunch(const WfState& w) {
     return {std::max(1u, (uint32_t)w.numCUs * w.cfg.shadeWavesPerCU * 64u / b), b};
 }
 
+template <bool BATCH>
+void launchIteration(const WfArgs& A, const Launch& tl, const Launch& sl, size_t lds, bool ld16, bool ordered,
+                     uint32_t in, hipStream_t st) {
+    if (ordered) {
+        if (ld16) k_wf_extend<true, 16, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_extend<true, 32, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+        k_wf_compact<<<sl.grid, sl.block, 0, st>>>(A, in);
+    } else if constexpr (!BATCH) {
+        if (ld16) k_wf_extend<false, 16, false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_extend<false, 32, false><<<tl.grid, tl.block, lds, st>>>(A, in);
+    }
+    k_wf_shade<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
+    k_wf_miss<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
+    if (ordered) {
+        if (ld16) k_wf_shadow<true, 16, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_shadow<true, 32, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+        k_wf_shadow_end<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
+    } else if constexpr (!BATCH) {
+        if (ld16) k_wf_shadow<false, 16, false><<<tl.grid, tl.block, lds, st>>>(A, in);
+        else k_wf_shadow<false, 32, false><<<tl.grid, tl.block, lds, st>>>(A, in);
+    }
+}
+
 void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStream_t st) {
     const Launch tl = traceLaunch(w), sl = shadeLaunch(w);
     const bool ordered = (A.flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
     const bool ld16 = w.cfg.ldsDepth == 16;
     const size_t lds = sizeof(uint32_t) * (ld16 ? 16 : 32) * tl.block;
     for (uint32_t it = 0; it < iters; ++it) {
-        const uint32_t in = it & 1u;
-        if (ordered) {
-            if (ld16) k_wf_extend<true, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
-            else k_wf_extend<true, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
-            k_wf_compact<<<sl.grid, sl.block, 0, st>>>(A, in);
-        } else {
-            if (ld16) k_wf_extend<false, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
-            else k_wf_extend<false, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
-        }
-        k_wf_shade<<<sl.grid, sl.block, 0, st>>>(A, in);
-        k_wf_miss<<<sl.grid, sl.block, 0, st>>>(A, in);
-        if (ordered) {
-            if (ld16) k_wf_shadow<true, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
-            else k_wf_shadow<true, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
-            k_wf_shadow_end<<<sl.grid, sl.block, 0, st>>>(A, in);
-        } else {
-            if (ld16) k_wf_shadow<false, 16><<<tl.grid, tl.block, lds, st>>>(A, in);
-            else k_wf_shadow<false, 32><<<tl.grid, tl.block, lds, st>>>(A, in);
-        }
+        if (A.table) launchIteration<true>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+        else launchIteration<false>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
     }
 }
 
@@ -826,19 +876,12 @@ int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg) {
     return CLR_OK;
 }
 
-int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, uint32_t* states,
-                    clr_float3* pixels, hipStream_t stream) {
-    int rc = ensureWfState(ctx);
-    if (rc) return rc;
-    WfState& w = *ctx->wf;
-    WfArgs A;
-    std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
-    rc = prepareRenderScene(ctx, sd, A.S, stream);
-    if (rc) return rc;
-    // Tile-major positions: whole 8x8 tiles, edge-tile positions outside the
-    // image are skipped by k_wf_init.
-    const uint64_t numPix = (uint64_t)((rp.width + 7) / 8) * ((rp.height + 7) / 8) * 64;
-    const uint32_t P = (uint32_t)std::min<uint64_t>(numPix, w.cfg.maxSlots);
+// Runs the iterations over `numPos` tile-major positions (all scenes) in
+// chunks of at most maxSlots path slots. A carries the scene(s) and outputs.
+int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_render_params& rp,
+              hipStream_t stream) {
+    int rc = CLR_OK;
+    const uint32_t P = (uint32_t)std::min<uint64_t>(numPos, w.cfg.maxSlots);
     const Launch tl = traceLaunch(w);
     const uint32_t lanes = tl.grid * tl.block;
     const size_t need = wfBytes(P, lanes);
@@ -862,8 +905,6 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
     A.maxDepth = rp.maxDepth;
     A.flags = rp.flags & (CLR_RENDER_REFERENCE_TRAVERSAL | CLR_RENDER_NO_RR);
     A.refillMin = w.cfg.refillMin;
-    A.states = states;
-    A.pixels = reinterpret_cast<float4*>(pixels);
     // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).
     const uint64_t bound = (uint64_t)rp.samplesPerPixel * (rp.maxDepth + 1ull) + 1ull;
     const uint32_t G = w.cfg.itersPerGraph;
@@ -871,10 +912,13 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
         rc = ensureGraph(ctx, w, A);
         if (rc) return rc;
     }
-    for (uint64_t chunk = 0; chunk < numPix; chunk += P) {
-        const uint32_t count = (uint32_t)std::min<uint64_t>(P, numPix - chunk);
+    for (uint64_t chunk = 0; chunk < numPos; chunk += P) {
+        const uint32_t count = (uint32_t)std::min<uint64_t>(P, numPos - chunk);
         k_wf_reset<<<1, 64, 0, stream>>>(A, (uint32_t)chunk);
-        k_wf_init<<<(count + 255) / 256, 256, 0, stream>>>(A, (uint32_t)chunk, count, rp.samplesPerPixel);
+        if (A.table)
+            k_wf_init<true><<<(count + 255) / 256, 256, 0, stream>>>(A, (uint32_t)chunk, count, rp.samplesPerPixel);
+        else
+            k_wf_init<false><<<(count + 255) / 256, 256, 0, stream>>>(A, (uint32_t)chunk, count, rp.samplesPerPixel);
         rc = hipCheck(ctx, hipGetLastError(), "wavefront init launch");
         if (rc) return rc;
         uint64_t launched = 0, batches = 0;
@@ -900,6 +944,101 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
         }
     }
     return CLR_OK;
+}
+
+int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, uint32_t* states,
+                    clr_float3* pixels, hipStream_t stream) {
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    WfState& w = *ctx->wf;
+    WfArgs A;
+    std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
+    rc = prepareRenderScene(ctx, sd, A.S, stream);
+    if (rc) return rc;
+    // Tile-major positions: whole 8x8 tiles, edge-tile positions outside the
+    // image are skipped by k_wf_init.
+    const uint64_t numPos = (uint64_t)((rp.width + 7) / 8) * ((rp.height + 7) / 8) * 64;
+    A.posPerScene = (uint32_t)numPos;
+    A.states = states;
+    A.pixels = reinterpret_cast<float4*>(pixels);
+    return runChunks(ctx, w, A, numPos, rp, stream);
+}
+
+// Many scenes with one image size and one set of render parameters in one
+// pass: every scene's traversal layout and face-light map go into one
+// context buffer, a device table holds the resolved scenes, and path slots
+// cover all scenes' tile-major positions (scene-major), so the queues mix
+// scenes and stay full.
+int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream) {
+    int rc = ensureWfState(ctx);
+    if (rc) return rc;
+    rc = ensureStatus(ctx);
+    if (rc) return rc;
+    WfState& w = *ctx->wf;
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t bytes = align(sizeof(RScene) * numScenes) + align(sizeof(int) * numScenes);
+    for (uint32_t s = 0; s < numScenes; ++s) {
+        const uint32_t n = scenes[s].numFaces;
+        bytes += align(64ull * (std::max(n, 2u) - 1)) + align(48ull * n) + align(4ull * n);
+    }
+    if (ctx->batchBytes < bytes) {
+        rc = hipCheck(ctx, hipDeviceSynchronize(), "batch buffer regrow sync");
+        if (rc) return rc;
+        if (ctx->batch) (void)hipFree(ctx->batch);
+        ctx->batch = nullptr;
+        ctx->batchBytes = 0;
+        dropGraph(w);
+        hipError_t e = hipMalloc(&ctx->batch, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "batch buffers of " + std::to_string(bytes) + " bytes");
+        }
+        ctx->batchBytes = bytes;
+    }
+    Carve c(ctx->batch);
+    RScene* dTable = c.take<RScene>(numScenes);
+    int* envWords = c.take<int>(numScenes);
+    std::vector<RScene> table(numScenes);
+    for (uint32_t s = 0; s < numScenes; ++s) {
+        const clr_scene_dev& sd = scenes[s];
+        const uint32_t n = sd.numFaces;
+        float4* an = c.take<float4>(4ull * (std::max(n, 2u) - 1));
+        float4* at = c.take<float4>(3ull * n);
+        uint32_t* fl = c.take<uint32_t>(n);
+        launchAccelBuild(sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at, stream);
+        launchFaceLights(sd.lights, sd.numLights, n, fl, envWords + s, stream);
+        RScene& S = table[s];
+        std::memset(&S, 0, sizeof(S));
+        S.g = GeomScene{sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at};
+        S.normals = sd.normals;
+        S.tangents = sd.tangents;
+        S.uvs = sd.uvs;
+        S.lights = sd.lights;
+        S.mat = sd.materialsData;
+        S.tex = sd.texturesData;
+        S.oth = sd.otherResources;
+        S.faceLight = fl;
+        S.envLightPtr = envWords + s;
+        S.envLight = -1;
+        S.status = ctx->devStatus;
+    }
+    rc = hipCheck(ctx, hipMemcpyAsync(dTable, table.data(), sizeof(RScene) * numScenes, hipMemcpyHostToDevice, stream),
+                  "scene table copy");
+    // The copy reads pageable host memory that goes away with this call.
+    if (!rc) rc = hipCheck(ctx, hipStreamSynchronize(stream), "scene table sync");
+    if (rc) return rc;
+    launchResolveScenes(dTable, numScenes, stream);
+    rc = hipCheck(ctx, hipGetLastError(), "batch set-up launch");
+    if (rc) return rc;
+    WfArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.S = table[0];  // status word for the kernels' flags
+    A.table = dTable;
+    A.posPerScene = ((rp.width + 7) / 8) * ((rp.height + 7) / 8) * 64;
+    A.states = states;
+    A.pixels = reinterpret_cast<float4*>(pixels);
+    return runChunks(ctx, w, A, (uint64_t)A.posPerScene * numScenes, rp, stream);
 }
 
 int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,

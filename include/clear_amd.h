@@ -242,6 +242,19 @@ enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
 int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
                uint32_t* randStates, clr_float3* pixels, int mode, void* stream);
 
+/* Renders numScenes scenes with the same image size and params in one
+ * wavefront pass (CLR_RENDER_WAVEFRONT semantics, default ordered traversal):
+ * scenes[] is a HOST array of device scenes; randStates and pixels hold the
+ * scenes' images back to back (scene s: states[s*W*H*4 ...], pixels[s*W*H
+ * ...]). Every scene's image equals what clr_render(..., WAVEFRONT) gives
+ * for that scene alone. The reference renders one scene per process
+ * (main.cpp:475-494); this is the multi-scene form of that loop, paired with
+ * clr_lbvh_build_segmented. numScenes * W * H < 2^31;
+ * CLR_RENDER_REFERENCE_TRAVERSAL is not supported here. */
+int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                     const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
+                     void* stream);
+
 /* ---- differentiation ------------------------------------------------------
  * No counterpart in the reference. Differentiable parameters: the albedo
  * (RGB) and roughness sigma of materials made of one Oren-Nayar / Lambert

@@ -64,7 +64,16 @@ enum {
      * U[0.2,0.8]^3, roughness U[0.1,0.6] from `seed`, 4 when 0) and the
      * emission-15 ceiling light; 512x512. clr_scene_diff_params lists the
      * differentiable parameters. */
-    CLR_SCENE_ADJOINT = 4
+    CLR_SCENE_ADJOINT = 4,
+    /* Batch-render scene: numTriangles triangles of the Gaussian-cluster soup
+     * of clr_gen_cluster_soup (64 clusters, sigma 0.02, edges 1e-3..1e-2,
+     * `seed`) in the unit cube, triangle i using material i mod 8 (four
+     * Lambert, four Lambert under a Beckmann coat of alpha U[0.05,0.5], eta
+     * 1.5; albedo U[0.3,0.9]^3 from the host RNG), plus a 2-triangle area
+     * light (emission 10) facing down above the cube (faces numTriangles and
+     * numTriangles+1) and a white Lambert floor quad under it (faces
+     * numTriangles+2, +3); pinhole camera, 128x128 by default. */
+    CLR_SCENE_CLUSTERS = 5
 };
 
 enum {
@@ -81,6 +90,7 @@ typedef struct clr_scene_params {
     uint32_t buildCpuBvh;     /* also build the reference's CPU midpoint BVH */
     float albedo;             /* furnace sphere albedo */
     float fovY;               /* vertical field of view in degrees (0 = scene default) */
+    uint32_t numTriangles;    /* CLR_SCENE_CLUSTERS soup size (0 = 10,000) */
 } clr_scene_params;
 
 int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** out);

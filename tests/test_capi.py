@@ -78,3 +78,32 @@ def test_cluster_soup_shape_statistics():
     c = v.mean(axis=1)
     # 64 clusters of sigma 0.02: the centroid cloud spans roughly the unit cube
     assert c.min() > -0.2 and c.max() < 1.2
+
+
+def test_cluster_scene_layout_and_batch_soup():
+    """The batch scene holds the cluster soup of the same seed as faces
+    0..n-1 (then the light and floor quads), and soup_of restates each scene
+    as a triangle soup whose LBVH equals the scene's own."""
+    import oracle as orc
+
+    hosts = [ca.HostScene(ca.SCENE_CLUSTERS, 32, 24, seed=100 + i, num_triangles=n)
+             for i, n in enumerate([1000, 2500])]
+    for i, h in enumerate(hosts):
+        assert (h.width, h.height) == (32, 24)
+        assert h.num_faces == [1000, 2500][i] + 4 and h.num_lights == 2
+        soup = ca.gen_cluster_soup([1000, 2500][i], seed=100 + i).reshape(-1, 9)
+        p = h.faces["p"][: len(soup)].astype(np.int64)
+        np.testing.assert_array_equal(h.vertices[p][:, :, :3].reshape(-1, 9), soup)
+        lp = h.faces["lightPtr"]
+        assert (lp[-4:-2] != 0xFFFF).all() and (lp[:-4] == 0xFFFF).all() and (lp[-2:] == 0xFFFF).all()
+    offs = np.array([0, hosts[0].num_faces, hosts[0].num_faces + hosts[1].num_faces], np.uint32)
+    soup = ca.soup_of(hosts, offs)
+    for s, h in enumerate(hosts):
+        tri = soup[offs[s]:offs[s + 1]]
+        v = np.zeros((len(tri) * 3, 4), np.float32)
+        v[:, :3] = tri.reshape(-1, 3)
+        f = np.zeros(len(tri), ca.FACE_DTYPE)
+        f["p"] = np.arange(len(tri) * 3).reshape(-1, 3)
+        a, b = orc.lbvh_build(h.vertices, h.faces, 10), orc.lbvh_build(v, f, 10)
+        np.testing.assert_array_equal(a["leaves"], b["leaves"])
+        np.testing.assert_array_equal(a["inodes"], b["inodes"])
