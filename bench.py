@@ -216,6 +216,102 @@ def adjoint_leg(ca, np, torch, ctx):
     return out
 
 
+def batch_leg(ca, np, torch, ctx, args):
+    """256 cluster-soup scenes (log-uniform 1k..500k triangles, seeds
+    100..355): one segmented LBVH build against 256 single-scene builds, then
+    one batched wavefront render (128x128, 32 spp, depth 5, launch shapes
+    autotuned on the batch) against rendering the scenes one by one."""
+    n_scenes, W, H, spp, depth = args.batch_scenes, 128, 128, 32, 5
+    rng = np.random.default_rng(3)
+    counts = np.exp(rng.uniform(np.log(1000), np.log(500000), n_scenes)).astype(np.int64)
+    log(f"building {n_scenes} cluster scenes ({int(counts.sum())} soup triangles)")
+    hosts = [ca.HostScene(ca.SCENE_CLUSTERS, W, H, seed=100 + i, num_triangles=int(n))
+             for i, n in enumerate(counts)]
+    batch = ca.SceneBatch(ctx, hosts)
+    total = int(batch.offsets[-1])
+    out = {"workload": f"{n_scenes} Gaussian-cluster scenes, {int(counts.min())}..{int(counts.max())} soup "
+                       f"triangles (log-uniform 1k..500k, seeds 100..{99 + n_scenes}) + light and floor quads, "
+                       f"{total} triangles in all; {W}x{H}, {spp} spp, depth 5",
+           "scenes": n_scenes, "triangles": total}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    seg_ms = timed(lambda: batch.build(), 5)
+    b = ca.LBVHBuilder(ctx, int(counts.max()) + 4)
+    tp, ip, lp = batch.tris.data_ptr(), batch.inodes.data_ptr(), batch.leaves.data_ptr()
+
+    def per_scene_builds():
+        for s in range(n_scenes):
+            o, n = int(batch.offsets[s]), int(batch.offsets[s + 1] - batch.offsets[s])
+            b.perform_soup(tp + 36 * o, n, 10, ip + ca.INODE_BYTES * (o - s), lp + ca.LEAF_BYTES * o)
+
+    per_ms = timed(per_scene_builds, 3)
+    out["segmented_build"] = {"ms": round(seg_ms, 3), "Mtri_s": round(total / seg_ms / 1e3, 1)}
+    out["per_scene_builds"] = {"ms": round(per_ms, 3), "Mtri_s": round(total / per_ms / 1e3, 1)}
+    out["segmented_speedup"] = round(per_ms / seg_ms, 2)
+    log(f"batch builds: segmented {out['segmented_build']}, one by one {out['per_scene_builds']}")
+    assert ctx.device_status() == 0
+
+    npix = W * H
+    states = torch.empty(n_scenes * npix * 4, dtype=torch.int32, device="cuda")
+    pixels = torch.zeros(n_scenes * npix * 4, dtype=torch.float32, device="cuda")
+
+    def seed_all():
+        for s in range(n_scenes):
+            ca.seed_pixel_states(ctx, W, H, states[s * npix * 4:(s + 1) * npix * 4])
+
+    saved = ca.wavefront_config(ctx)
+    tune_t = time.perf_counter()
+    cfg = ca.wavefront_autotune_batch(ctx, batch, 4, depth)
+    out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
+    out["wavefront_config"] = cfg.as_dict()
+    seed_all()
+    ms = timed(lambda: ca.render_batch(ctx, batch, spp, depth, states, pixels), 1)
+    samples = n_scenes * npix * spp
+    out["batched_wavefront"] = {"ms": round(ms, 2), "Msamples_s": round(samples / ms / 1e3, 1)}
+    log(f"batched render: {out['batched_wavefront']}")
+    # Scene-by-scene rendering is timed on every k-th scene (k = scenes /
+    # batch_per_scene) and reported as a rate over those scenes' samples.
+    step = max(1, n_scenes // max(1, args.batch_per_scene))
+    picked = list(range(0, n_scenes, step))
+    psamples = len(picked) * npix * spp
+    out["per_scene_sample"] = f"every {step}th scene ({len(picked)} scenes)"
+    for name, mode, flags in ([] if args.batch_per_scene == 0 else [
+                              ("per_scene_wavefront", ca.RENDER_WAVEFRONT, 0),
+                              ("per_scene_megakernel", ca.RENDER_MEGAKERNEL, 0),
+                              ("per_scene_megakernel_plain", ca.RENDER_MEGAKERNEL,
+                               ca.RENDER_REFERENCE_TRAVERSAL)]):
+        ca.set_wavefront_config(ctx, saved)
+        seed_all()
+
+        def one_by_one():
+            for s in picked:
+                sl = slice(s * npix * 4, (s + 1) * npix * 4)
+                ca.ctx_render_dev(ctx, batch.devs[s], W, H, spp, depth, states[sl], pixels[sl], mode, flags)
+
+        ms = timed(one_by_one, 1)
+        out[name] = {"ms": round(ms, 2), "Msamples_s": round(psamples / ms / 1e3, 1)}
+        log(f"batch {name}: {out[name]}")
+    ca.set_wavefront_config(ctx, cfg)
+    assert ctx.device_status() == 0
+    if args.batch_per_scene:
+        out["batched_vs_per_scene_megakernel"] = round(
+            out["batched_wavefront"]["Msamples_s"] / out["per_scene_megakernel"]["Msamples_s"], 2)
+        out["batched_vs_per_scene_plain_megakernel"] = round(
+            out["batched_wavefront"]["Msamples_s"] / out["per_scene_megakernel_plain"]["Msamples_s"], 2)
+    del batch, hosts
+    return out
+
+
 def cpu_lbvh(ca, soup, bits):
     import oracle as orc
 
@@ -263,6 +359,10 @@ def main() -> int:
     ap.add_argument("--no-trace", action="store_true", help="skip the closest-hit leg")
     ap.add_argument("--no-render", action="store_true", help="skip the render leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-per-scene", type=int, default=32,
+                    help="scenes rendered one by one for the batch comparison (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (calibrates "
                          "HBM byte counters)")
@@ -290,6 +390,7 @@ def main() -> int:
     del tsoup
     render, host = (None, None) if args.no_render else render_leg(ca, np, torch, ctx, args)
     adjoint = None if args.no_adjoint else adjoint_leg(ca, np, torch, ctx)
+    batch = None if args.no_batch else batch_leg(ca, np, torch, ctx, args)
     cpu = cpu_r = None
     if not args.no_cpu:
         cpu = cpu_lbvh(ca, soup, args.bits)
@@ -321,6 +422,7 @@ def main() -> int:
         "trace": trace,
         "render": render,
         "adjoint": adjoint,
+        "batch": batch,
         "cpu_render_baseline": cpu_r,
     }
     print(json.dumps(line), flush=True)

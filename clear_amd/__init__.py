@@ -73,6 +73,7 @@ def lib():
         "clr_wavefront_get_config": (i32, [vp, vp]),
         "clr_wavefront_set_config": (i32, [vp, vp]),
         "clr_wavefront_autotune": (i32, [vp, vp, vp, vp, vp]),
+        "clr_wavefront_autotune_batch": (i32, [vp, vp, u32, vp, vp, vp]),
         "clr_scene_build": (i32, [i32, vp, ctypes.POINTER(vp)]),
         "clr_scene_free": (i32, [vp]),
         "clr_scene_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
@@ -431,6 +432,17 @@ class SceneBatch:
         return len(self.hosts)
 
 
+def wavefront_autotune_batch(ctx: Context, batch: SceneBatch, spp: int, max_depth: int,
+                             stream=None, flags: int = 0) -> WavefrontConfig:
+    """clr_wavefront_autotune_batch: the launch-shape search timed on batched
+    renders of `batch`; the result stays in the context."""
+    rp = RenderParams(batch.width, batch.height, spp, max_depth, flags)
+    c = WavefrontConfig()
+    ctx.check(lib().clr_wavefront_autotune_batch(ctx.handle, batch.devs, len(batch), ctypes.byref(rp),
+                                                 ctypes.byref(c), _stream(stream)))
+    return c
+
+
 def render_batch(ctx: Context, batch: SceneBatch, spp: int, max_depth: int, states, pixels,
                  stream=None, flags: int = 0) -> None:
     """clr_render_batch: every scene of `batch` in one wavefront pass; states
@@ -464,6 +476,14 @@ def render_adjoint(ctx: Context, scene: "DeviceScene", spp: int, max_depth: int,
     ctx.check(lib().clr_render_adjoint(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp),
                                        ctypes.byref(dp), _ptr(states), _ptr(d_image), _ptr(g_albedo),
                                        _ptr(g_sigma), _ptr(g_emission), _stream(stream)))
+
+
+def ctx_render_dev(ctx: Context, dev: SceneDev, width: int, height: int, spp: int, max_depth: int,
+                   states, pixels, mode: int = RENDER_MEGAKERNEL, flags: int = 0, stream=None) -> None:
+    """clr_render on a raw clr_scene_dev (e.g. one scene of a SceneBatch)."""
+    rp = RenderParams(width, height, spp, max_depth, flags)
+    ctx.check(lib().clr_render(ctx.handle, ctypes.byref(dev), ctypes.byref(rp), _ptr(states), _ptr(pixels),
+                               mode, _stream(stream)))
 
 
 def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, pixels,

@@ -482,7 +482,27 @@ int clr_wavefront_autotune(clr_ctx* ctx, const clr_scene_dev* scene, const clr_r
     if (params->samplesPerPixel == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "autotune needs samplesPerPixel >= 1");
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    return wavefrontAutotune(ctx, *scene, *params, config, static_cast<hipStream_t>(stream));
+    return wavefrontAutotune(ctx, scene, 1, false, *params, config, static_cast<hipStream_t>(stream));
+}
+
+int clr_wavefront_autotune_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                                 const clr_render_params* params, clr_wavefront_config* config, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!scenes || !params || numScenes == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "null or empty batch");
+    for (uint32_t s = 0; s < numScenes; ++s) {
+        int rc = checkRenderArgs(ctx, scenes + s, params);
+        if (rc) return rc;
+    }
+    if (params->maxDepth > 255) return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
+    if (params->flags & CLR_RENDER_REFERENCE_TRAVERSAL)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "batched render uses the ordered traversal only");
+    if (params->samplesPerPixel == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "autotune needs samplesPerPixel >= 1");
+    const uint64_t tiles = (uint64_t)((params->width + 7) / 8) * ((params->height + 7) / 8) * 64;
+    if ((uint64_t)params->width * params->height * numScenes >= (1ull << 31) || tiles * numScenes >= (1ull << 31))
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "batch too large (scenes x pixels >= 2^31)");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return wavefrontAutotune(ctx, scenes, numScenes, true, *params, config, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -103,6 +103,16 @@ __device__ __forceinline__ void slotScene(const WfArgs& A, uint32_t v, RScene& S
     }
 }
 
+// Scene a traversal walker starts bound to: the kernel's private copy in
+// single-scene mode, table entry 0 in batch mode (where every ray rebinds).
+// A pointer that could be either would be a generic address and force the
+// private copy into scratch memory.
+template <bool BATCH>
+__device__ __forceinline__ const RScene* firstScene(const WfArgs& A, const RScene& S) {
+    if constexpr (BATCH) return A.table;
+    else return &S;
+}
+
 size_t wfBytes(uint32_t P, uint32_t lanes) {
     size_t b = 0;
     auto add = [&](size_t x) { b += (x + 255) & ~size_t(255); };
@@ -329,8 +339,8 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     if constexpr (ORDERED) {
         // Persistent waves refilling idle lanes from a wave pool; hit / miss
         // queues are built by k_wf_compact.
-        const TexAlpha ta{&S};
-        Ordered<false, TexAlpha, SpillStack<LD>> w(S.g, stk, ta);
+        const RScene* S0 = firstScene<BATCH>(A, S);
+        Ordered<false, TexAlpha, SpillStack<LD>> w(S0->g, stk, TexAlpha{S0});
         WavePool pool(n);
         bool busy = false;
         uint32_t j = 0;
@@ -586,8 +596,8 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         // Persistent waves as in k_wf_extend; environment samples use the
         // occlusion walk, area samples the closest hit compared with the
         // light's face. Samples ending here are finished by k_wf_shadow_end.
-        const TexAlpha ta{&S};
-        Ordered<false, TexAlpha, SpillStack<LD>> w(S.g, stk, ta);
+        const RScene* S0 = firstScene<BATCH>(A, S);
+        Ordered<false, TexAlpha, SpillStack<LD>> w(S0->g, stk, TexAlpha{S0});
         WavePool pool(n);
         bool busy = false;
         uint32_t i = 0, target = kNone;
@@ -1041,26 +1051,36 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
     return runChunks(ctx, w, A, (uint64_t)A.posPerScene * numScenes, rp, stream);
 }
 
-int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
-                      clr_wavefront_config* out, hipStream_t stream) {
+// Times launch-shape candidates by coordinate descent on renders of the
+// caller's scene(s) (numScenes > 1 or batch: clr_render_batch renders) into
+// scratch images, and keeps the fastest in the context.
+int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes, bool batch,
+                      const clr_render_params& rp, clr_wavefront_config* out, hipStream_t stream) {
     int rc = ensureWfState(ctx);
     if (rc) return rc;
     WfState& w = *ctx->wf;
     const uint64_t numPix = (uint64_t)rp.width * rp.height;
+    const uint32_t images = batch ? numScenes : 1u;
     uint32_t* states = nullptr;
     clr_float3* pixels = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    rc = hipCheck(ctx, hipMalloc(&states, 16 * numPix), "autotune states");
-    if (!rc) rc = hipCheck(ctx, hipMalloc(&pixels, 16 * numPix), "autotune pixels");
+    rc = hipCheck(ctx, hipMalloc(&states, 16 * numPix * images), "autotune states");
+    if (!rc) rc = hipCheck(ctx, hipMalloc(&pixels, 16 * numPix * images), "autotune pixels");
     if (!rc) rc = hipCheck(ctx, hipEventCreate(&e0), "event");
     if (!rc) rc = hipCheck(ctx, hipEventCreate(&e1), "event");
+    auto renderAll = [&]() -> int {
+        return batch ? renderWavefrontBatch(ctx, scenes, numScenes, rp, states, pixels, stream)
+                     : renderWavefront(ctx, scenes[0], rp, states, pixels, stream);
+    };
     auto timeOne = [&](const clr_wavefront_config& c, float* ms) -> int {
         w.cfg = c;
-        int r = seedPixelStates(ctx, rp.width, rp.height, states, stream);
+        int r = CLR_OK;
+        for (uint32_t i = 0; i < images && !r; ++i)
+            r = seedPixelStates(ctx, rp.width, rp.height, states + 4 * numPix * i, stream);
         // One untimed render: pool growth and graph capture happen here.
-        if (!r) r = renderWavefront(ctx, sd, rp, states, pixels, stream);
+        if (!r) r = renderAll();
         if (!r) r = hipCheck(ctx, hipEventRecord(e0, stream), "event record");
-        if (!r) r = renderWavefront(ctx, sd, rp, states, pixels, stream);
+        if (!r) r = renderAll();
         if (!r) r = hipCheck(ctx, hipEventRecord(e1, stream), "event record");
         if (!r) r = hipCheck(ctx, hipEventSynchronize(e1), "event sync");
         if (!r) r = hipCheck(ctx, hipEventElapsedTime(ms, e0, e1), "event time");
@@ -1095,6 +1115,7 @@ int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_pa
     }, 8);
     const uint32_t iters[3] = {4, 8, 16}, checks[3] = {0, 2, 8};
     tryVariants([&](clr_wavefront_config& c, int v) { c.itersPerGraph = iters[v / 3]; c.checkEvery = checks[v % 3]; }, 9);
+    tryVariants([&](clr_wavefront_config& c, int v) { c.maxSlots = 1u << (20 + v); }, 3);
     w.cfg = best;
     w.tuned = true;
     if (out) *out = best;

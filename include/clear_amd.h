@@ -319,6 +319,11 @@ int clr_wavefront_set_config(clr_ctx* ctx, const clr_wavefront_config* config);
 int clr_wavefront_autotune(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
                            clr_wavefront_config* config, void* stream);
 
+/* clr_wavefront_autotune for clr_render_batch: the same search timed on
+ * batched renders of scenes[0..numScenes) (HOST array). */
+int clr_wavefront_autotune_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                                 const clr_render_params* params, clr_wavefront_config* config, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,3 +138,14 @@ def test_batch_rejects_reference_traversal_and_mixed_sizes(gpu_ctx, scenes):
     other = ca.HostScene(ca.SCENE_CLUSTERS, 32, 32, seed=7, num_triangles=1000)
     with pytest.raises(ValueError):
         ca.SceneBatch(gpu_ctx, [scenes[0], other])
+
+
+def test_batch_autotune_keeps_images(gpu_ctx, scenes):
+    saved = ca.wavefront_config(gpu_ctx)
+    try:
+        batch = ca.SceneBatch(gpu_ctx, scenes[:3])
+        cfg = ca.wavefront_autotune_batch(gpu_ctx, batch, 1, 5)
+        print(f"[batch] autotuned: {cfg.as_dict()}", flush=True)
+        _check_equal(gpu_ctx, scenes[:3], batch, 2, 5, "autotuned")
+    finally:
+        ca.set_wavefront_config(gpu_ctx, saved)
