@@ -21,6 +21,8 @@ FACE_DTYPE = np.dtype(
      ("alphaTexPtr", "<u4"), ("matPtr", "<u2"), ("lightPtr", "<u2")]
 )
 assert FACE_DTYPE.itemsize == 56
+BVH_NODE_DTYPE = np.dtype([("min", "<f4", 4), ("max", "<f4", 4), ("children", "<u4", 2), ("dum0", "<u4", 2)])
+assert BVH_NODE_DTYPE.itemsize == 48
 INODE_BYTES = 48
 LEAF_BYTES = 48
 
@@ -62,8 +64,10 @@ def lib():
         "clr_lbvh_build_segmented": (i32, [vp, vp, ctypes.POINTER(u32), u32, u32, vp, vp, vp]),
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_trace_closest_ordered": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
+        "clr_trace_closest_bvh": (i32, [vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, i32, vp]),
         "clr_gen_cluster_soup": (i32, [u32, i32, u32, f32, f32, f32, vp]),
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
+        "clr_bvh_build_midpoint": (i32, [vp, vp, u32, vp]),
         "clr_render": (i32, [vp, vp, vp, vp, vp, i32, vp]),
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
         "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -204,6 +208,15 @@ class LBVHBuilder:
             _stream(stream)))
 
 
+def trace_closest_bvh(ctx: Context, nodes, num_faces: int, vertices, faces, rays, num_rays: int, hit_face,
+                      hit_t, stats=None, stream=None, ordered: bool = False) -> None:
+    """clr_trace_closest_bvh: closest hit over a CPU midpoint BVH (reference
+    walk, or the ordered traversal with ordered=True)."""
+    ctx.check(lib().clr_trace_closest_bvh(ctx.handle, _ptr(nodes), num_faces, _ptr(vertices), _ptr(faces),
+                                          _ptr(rays), num_rays, _ptr(hit_face), _ptr(hit_t), _ptr(stats),
+                                          int(ordered), _stream(stream)))
+
+
 def trace_closest(ctx: Context, internal_nodes, leaf_nodes, num_faces: int, vertices, faces,
                   rays, num_rays: int, hit_face, hit_t, stats=None, stream=None,
                   ordered: bool = False) -> None:
@@ -234,6 +247,18 @@ def gen_random_rays(num_rays: int, seed: int = 2) -> np.ndarray:
     rc = lib().clr_gen_random_rays(num_rays, seed, _ptr(out))
     if rc:
         raise ClrError(rc, "clr_gen_random_rays")
+    return out
+
+
+def build_midpoint_bvh(vertices: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """clr_bvh_build_midpoint on host arrays: BVH_NODE_DTYPE [2n - 1]."""
+    n = len(faces)
+    out = np.zeros(max(2 * n - 1, 1), BVH_NODE_DTYPE)
+    v = np.ascontiguousarray(vertices, dtype=np.float32)
+    f = np.ascontiguousarray(faces)
+    rc = lib().clr_bvh_build_midpoint(v.ctypes.data, f.ctypes.data, n, out.ctypes.data)
+    if rc:
+        raise ClrError(rc, "clr_bvh_build_midpoint")
     return out
 
 
@@ -297,7 +322,7 @@ class SceneDev(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materialsData",
                  "texturesData", "otherResources", "internalNodes", "leafNodes"]] + \
-               [("numFaces", ctypes.c_uint32), ("numLights", ctypes.c_uint32)]
+               [("numFaces", ctypes.c_uint32), ("numLights", ctypes.c_uint32), ("bvhNodes", ctypes.c_void_p)]
 
 
 class RenderParams(ctypes.Structure):
@@ -355,23 +380,37 @@ class HostScene:
 
 
 class DeviceScene:
-    """Scene arrays resident on the GPU plus its LBVH (built by clr_lbvh_build)."""
+    """Scene arrays resident on the GPU plus its acceleration structure: the
+    LBVH built by clr_lbvh_build (accel="lbvh"), or the CPU midpoint BVH of
+    Scene::build uploaded as clr_scene_dev.bvhNodes (accel="midpoint")."""
 
-    def __init__(self, ctx: Context, host: HostScene, bits: int = 10, device="cuda", stream=None):
+    def __init__(self, ctx: Context, host: HostScene, bits: int = 10, device="cuda", stream=None,
+                 accel: str = "lbvh"):
         import torch
 
-        self.ctx, self.host = ctx, host
+        self.ctx, self.host, self.accel = ctx, host, accel
         self.t = host.to_device(device)
         n = host.num_faces
-        self.inodes, self.leaves = node_buffers(n, device)
-        LBVHBuilder(ctx, n, bits).perform(self.t["vertices"], self.t["faces"], n, bits, self.inodes,
-                                          self.leaves, stream)
+        bvh = None
+        self.inodes = self.leaves = self.bvh = None
+        if accel == "lbvh":
+            self.inodes, self.leaves = node_buffers(n, device)
+            LBVHBuilder(ctx, n, bits).perform(self.t["vertices"], self.t["faces"], n, bits, self.inodes,
+                                              self.leaves, stream)
+        elif accel == "midpoint":
+            nodes = host.arrays["bvh_nodes"]
+            if nodes.nbytes < (2 * n - 1) * 48:  # the host scene was built without its CPU BVH
+                nodes = build_midpoint_bvh(host.vertices, host.faces).view(np.uint8)
+            self.bvh = torch.from_numpy(np.ascontiguousarray(nodes[:(2 * n - 1) * 48])).to(device)
+            bvh = _ptr(self.bvh).value
+        else:
+            raise ValueError(f"unknown accel {accel!r}")
         self.dev = SceneDev(
             _ptr(self.t["vertices"]).value, _ptr(self.t["normals"]).value,
             _ptr(self.t["tangents"]).value, _ptr(self.t["uvs"]).value, _ptr(self.t["faces"]).value,
             _ptr(self.t["lights"]).value, _ptr(self.t["materials"]).value,
             _ptr(self.t["textures"]).value, _ptr(self.t["others"]).value,
-            _ptr(self.inodes).value, _ptr(self.leaves).value, n, host.num_lights)
+            _ptr(self.inodes).value, _ptr(self.leaves).value, n, host.num_lights, bvh)
         self.width, self.height = host.width, host.height
 
 

@@ -362,7 +362,7 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
         return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    return traceClosest(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
+    return traceClosest(ctx, internalNodes, leafNodes, nullptr, numFaces, vertices, faces, rays, numRays,
                         hitFace, hitT, stats, static_cast<hipStream_t>(stream));
 }
 
@@ -382,8 +382,29 @@ int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
         return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    return traceOrdered(ctx, internalNodes, leafNodes, numFaces, vertices, faces, rays, numRays,
+    return traceOrdered(ctx, internalNodes, leafNodes, nullptr, numFaces, vertices, faces, rays, numRays,
                         hitFace, hitT, stats, static_cast<hipStream_t>(stream));
+}
+
+int clr_trace_closest_bvh(clr_ctx* ctx, const clr_bvh_node* nodes, uint32_t numFaces, const clr_float3* vertices,
+                          const clr_face* faces, const float* rays, uint32_t numRays, uint32_t* hitFace,
+                          float* hitT, uint32_t* stats, int ordered, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (numFaces == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "numFaces must be >= 1");
+    if (numFaces >= (1u << 31)) return fail(ctx, CLR_ERR_UNSUPPORTED, "numFaces must be < 2^31");
+    if (!nodes || !vertices || !faces) return fail(ctx, CLR_ERR_INVALID_ARG, "null scene buffer");
+    if (numRays > 0 && (!rays || !hitFace || !hitT))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null ray buffer");
+    if (!aligned16(nodes) || !aligned16(vertices))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (ordered)
+        return traceOrdered(ctx, nullptr, nullptr, nodes, numFaces, vertices, faces, rays, numRays, hitFace, hitT,
+                            stats, st);
+    return traceClosest(ctx, nullptr, nullptr, nodes, numFaces, vertices, faces, rays, numRays, hitFace, hitT,
+                        stats, st);
 }
 
 }  // extern "C"
@@ -511,15 +532,18 @@ namespace clr {
 int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params) {
     if (!scene || !params) return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
     const clr_scene_dev& s = *scene;
-    if (s.numFaces == 0 || !s.vertices || !s.faces || !s.leafNodes || (s.numFaces > 1 && !s.internalNodes) ||
-        !s.materialsData || !s.texturesData || !s.otherResources || (s.numLights && !s.lights))
+    const bool accelOk = s.bvhNodes ? aligned16(s.bvhNodes) : (s.leafNodes && (s.numFaces == 1 || s.internalNodes));
+    if (s.numFaces == 0 || !s.vertices || !s.faces || !accelOk || !s.materialsData || !s.texturesData ||
+        !s.otherResources || (s.numLights && !s.lights))
         return fail(ctx, CLR_ERR_INVALID_ARG, "incomplete scene");
+    if (s.numFaces >= (1u << 31)) return fail(ctx, CLR_ERR_UNSUPPORTED, "numFaces must be < 2^31");
     if (s.numLights == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "scene has no light");
     if (params->width == 0 || params->height == 0 || params->maxDepth == 0)
         return fail(ctx, CLR_ERR_INVALID_ARG, "empty image or zero bounce cap");
     if ((uint64_t)params->width * params->height >= (1ull << 31))
         return fail(ctx, CLR_ERR_UNSUPPORTED, "image too large");
-    if (!aligned16(s.vertices) || !aligned16(s.leafNodes) || (s.internalNodes && !aligned16(s.internalNodes)) ||
+    if (!aligned16(s.vertices) || (s.leafNodes && !aligned16(s.leafNodes)) ||
+        (s.internalNodes && !aligned16(s.internalNodes)) ||
         !aligned16(s.otherResources) || !aligned16(s.texturesData) || !aligned16(s.materialsData))
         return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
     return CLR_OK;

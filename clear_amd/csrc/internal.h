@@ -134,10 +134,13 @@ int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_p
 
 // ---- traversal (traverse.hip) ----
 int ensureTraceSpill(clr_ctx* ctx, size_t bytes);
-// Launches the traversal-layout build into caller-given arrays.
-void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-                      const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
+// Launches the traversal-layout build into caller-given arrays (from the
+// LBVH pair, or from a CPU midpoint BVH when bvh is not null).
+void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+                      uint32_t n, const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
                       hipStream_t stream);
+// Bytes of the traversal layout's node array (256-B aligned).
+size_t accelNodeBytes(uint32_t n, bool midpoint);
 // Face -> first light map and first environment light of one scene.
 void launchFaceLights(const clr_light_info* lights, uint32_t numLights, uint32_t numFaces, uint32_t* faceLight,
                       int* envLight, hipStream_t stream);
@@ -145,13 +148,13 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
                          const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
 // Builds the ordered walk's traversal layout from the drop-in nodes on `stream`
 // into context memory and returns its two arrays.
-int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-               const clr_float3* vertices, const clr_face* faces, const float4** anodes, const float4** atris,
-               hipStream_t stream);
-int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-                 const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
+int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+               uint32_t n, const clr_float3* vertices, const clr_face* faces, const float4** anodes,
+               const float4** atris, hipStream_t stream);
+int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+                 uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
                  uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream);
-int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
+int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays,
                  uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
                  hipStream_t stream);

@@ -26,7 +26,7 @@ struct NoHook {
 
 // One camera path for pixel (gx, gy) (pathtracer.cl:48-196). noRR disables
 // Russian roulette (the bounce cap still applies).
-template <bool ORDERED, class Hook = NoHook>
+template <int WALK, class Hook = NoHook>
 __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, bool noRR, Rng& rng,
                         uint32_t* stack, int stride, Hook& hook) {
     V3 pix = v3z();
@@ -42,7 +42,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
     bool traceContinue = true;
     uint32_t sampledType = 0, depth = 0;
     V3 vout = v3z();
-    if (intersect<ORDERED>(S, org, dir, stack, stride, sp)) {
+    if (intersect<WALK>(S, org, dir, stack, stride, sp)) {
         uint32_t face = sp.face;
         clr_face F = S.g.faces[face];
         vout = -dir;
@@ -69,7 +69,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                 }
                 const V3 sorg = sp.p + vinL * kRayEps;
                 SurfPt lsp;
-                const bool hit = intersect<ORDERED>(S, sorg, vinL, stack, stride, lsp);
+                const bool hit = intersect<WALK>(S, sorg, vinL, stack, stride, lsp);
                 const bool visible = atInf ? !hit : (hit && lsp.face == lp.face);
                 // A zero or infinite light density (an environment sample at a
                 // pole, where sin(theta) = 0) contributes nothing.
@@ -97,7 +97,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
             dir = nd;
             org = sp.p + dir * kRayEps;
             ++depth;
-            if (!intersect<ORDERED>(S, org, dir, stack, stride, sp)) break;
+            if (!intersect<WALK>(S, org, dir, stack, stride, sp)) break;
             face = sp.face;
             F = S.g.faces[face];
             vout = -dir;
@@ -262,7 +262,7 @@ namespace {
 
 constexpr int kTile = 8;  // 8x8 pixels per wave
 
-template <bool ORDERED>
+template <int WALK>
 __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t width, uint32_t height,
                                                     uint32_t spp, uint32_t maxDepth, bool noRR,
                                                     uint32_t* __restrict__ states,
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
     dev::V3 acc = dev::v3z();
     dev::NoHook hook;
     for (uint32_t k = 0; k < spp; ++k)
-        acc += dev::tracePath<ORDERED>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64, hook);
+        acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64, hook);
     reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     float4 p = reinterpret_cast<float4*>(pixels)[pi];
     p.x += acc.x;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
 // Adjoint of k_render_mega: the same pixels, samples and random states, each
 // sample replayed with the adjoint hook; block sums of the gradients go to
 // global memory once per block.
-template <bool ORDERED>
+template <int WALK>
 __global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32_t width, uint32_t height,
                                                        uint32_t spp, uint32_t maxDepth, bool noRR,
                                                        const uint32_t* __restrict__ states,
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32
         const float inv = 1.0f / (float)spp;
         dev::AdjHook hook{S, table, lightProp, M, {a.x * inv, a.y * inv, a.z * inv}, &s_acc, 0};
         for (uint32_t k = 0; k < spp; ++k) {
-            const dev::V3 L = dev::tracePath<ORDERED>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64,
+            const dev::V3 L = dev::tracePath<WALK>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64,
                                                       hook);
             hook.finish(L);
         }
@@ -416,10 +416,11 @@ int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hi
         k_face_lights_fill<<<(sd.numLights + 255) / 256, 256, 0, stream>>>(sd.lights, sd.numLights,
                                                                            sd.numFaces, faceLight, envLight);
     const float4 *anodes = nullptr, *atris = nullptr;
-    rc = buildAccel(ctx, sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, &anodes, &atris,
-                    stream);
+    rc = buildAccel(ctx, sd.internalNodes, sd.leafNodes, sd.bvhNodes, sd.numFaces, sd.vertices, sd.faces, &anodes,
+                    &atris, stream);
     if (rc) return rc;
-    S.g = dev::GeomScene{sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, anodes, atris};
+    S.g = dev::GeomScene{sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, anodes, atris,
+                         reinterpret_cast<const float4*>(sd.bvhNodes)};
     S.normals = sd.normals;
     S.tangents = sd.tangents;
     S.uvs = sd.uvs;
@@ -443,12 +444,16 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     int rc = prepareRenderScene(ctx, sd, S, stream);
     if (rc) return rc;
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
-    if (rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)
-        k_render_mega<false><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                       (rp.flags & CLR_RENDER_NO_RR) != 0, states, pixels);
+    const bool noRR = (rp.flags & CLR_RENDER_NO_RR) != 0;
+    if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
+        k_render_mega<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
+                                                                   rp.maxDepth, noRR, states, pixels);
+    else if (sd.bvhNodes)
+        k_render_mega<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
+                                                                    rp.maxDepth, noRR, states, pixels);
     else
-        k_render_mega<true><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                      (rp.flags & CLR_RENDER_NO_RR) != 0, states, pixels);
+        k_render_mega<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
+                                                                rp.maxDepth, noRR, states, pixels);
     return hipCheck(ctx, hipGetLastError(), "render launch");
 }
 
@@ -478,14 +483,18 @@ int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
     const bool noRR = (rp.flags & CLR_RENDER_NO_RR) != 0;
     const float4* adj = reinterpret_cast<const float4*>(dLdI);
-    if (rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)
-        k_render_adjoint<false><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                          noRR, states, adj, table, dp.numMaterials, dp.lightProp,
-                                                          gAlb, gSig, gEm);
+    if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
+        k_render_adjoint<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            dp.lightProp, gAlb, gSig, gEm);
+    else if (sd.bvhNodes)
+        k_render_adjoint<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            dp.lightProp, gAlb, gSig, gEm);
     else
-        k_render_adjoint<true><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
-                                                         noRR, states, adj, table, dp.numMaterials, dp.lightProp,
-                                                         gAlb, gSig, gEm);
+        k_render_adjoint<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            dp.lightProp, gAlb, gSig, gEm);
     return hipCheck(ctx, hipGetLastError(), "adjoint launch");
 }
 

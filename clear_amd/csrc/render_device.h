@@ -346,21 +346,28 @@ struct TexAlpha {
     }
 };
 
-// rayIntersection with hit parameters (ORDERED: the ordered traversal).
-// Returns false (face = kNone) on a miss.
-template <bool ORDERED>
+// Traversal a render kernel is compiled for: the reference's walk over the
+// LBVH, the ordered traversal (over either structure's traversal layout), or
+// the reference's walk over the CPU midpoint BVH.
+enum : int { kWalkLbvh = 0, kWalkOrdered = 1, kWalkMidpoint = 2 };
+
+// rayIntersection with hit parameters, by WALK. Returns false (face = kNone)
+// on a miss.
+template <int WALK>
 __device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint32_t* stack, int stride,
                                           SurfPt& sp) {
     float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
     TraceStats st{0u, 0u};
     bool over = false;
     uint32_t face;
-    if constexpr (ORDERED)
+    if constexpr (WALK == kWalkOrdered)
         face = orderedHit<false, false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
                                                   TexAlpha{&S}, &over);
+    else if constexpr (WALK == kWalkMidpoint)
+        face = bvhWalk<false, false>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2, TexAlpha{&S}, &over);
     else
-        face = closestHit<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
-                                           TexAlpha{&S}, &over);
+        face = lbvhClosest<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
+                                            TexAlpha{&S}, &over);
     if (over) atomicOr(S.status, kStatusStackOverflow);
     if (face == kNone || isinf(t)) {
         sp.face = kNone;

@@ -648,6 +648,29 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
     return CLR_OK;
 }
 
+int clr_bvh_build_midpoint(const clr_float3* vertices, const clr_face* faces, uint32_t numFaces,
+                           clr_bvh_node* out) {
+    if (!vertices || !faces || !out || numFaces == 0) return CLR_ERR_INVALID_ARG;
+    try {
+        BVH bvh;
+        bvh.leaves.reserve(numFaces);
+        bvh.lIdx.reserve(numFaces);
+        for (uint32_t i = 0; i < numFaces; ++i) {
+            BBox b;
+            b.unionP(&vertices[faces[i].p0].x);
+            b.unionP(&vertices[faces[i].p1].x);
+            b.unionP(&vertices[faces[i].p2].x);
+            bvh.addLeaf(b);
+        }
+        bvh.build();
+        static_assert(sizeof(BVHNode) == sizeof(clr_bvh_node), "BVH node layout");
+        std::memcpy(out, bvh.nodes.data(), bvh.nodes.size() * sizeof(BVHNode));
+    } catch (const std::exception&) {
+        return CLR_ERR_OUT_OF_MEMORY;
+    }
+    return CLR_OK;
+}
+
 int clr_scene_free(clr_host_scene* s) {
     delete s;
     return CLR_OK;

@@ -28,7 +28,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
     uint32_t n, const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces,
     const float* __restrict__ rays, uint32_t numRays, uint32_t* __restrict__ hitFace,
-    float* __restrict__ hitT, uint32_t* __restrict__ stats, uint32_t* __restrict__ status) {
+    float* __restrict__ hitT, uint32_t* __restrict__ stats, uint32_t* __restrict__ status,
+    const float4* __restrict__ bvh) {
     __shared__ uint32_t s_stack[dev::kStackDepth * kTraceBlock];
     const uint32_t r = blockIdx.x * kTraceBlock + threadIdx.x;
     if (r >= numRays) return;
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     dev::V3 o = {rays[r], rays[R + r], rays[2 * R + r]};
     dev::V3 d = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
     float t = rays[6 * R + r];
-    dev::GeomScene g{inodes, leaves, n, verts, faces};
+    dev::GeomScene g{inodes, leaves, n, verts, faces, nullptr, nullptr, bvh};
     dev::TraceStats st{0u, 0u};
     bool over = false;
     uint32_t best = dev::closestHit<STATS>(g, o, d, t, dev::LdsStack{s_stack + threadIdx.x, kTraceBlock},
@@ -86,6 +87,45 @@ __global__ void k_accel_build(const clr_lbvh_inode* __restrict__ inodes, const c
     }
 }
 
+// Traversal layout of a CPU midpoint BVH, indexed by node number: internal
+// node i (children[1] != 0xFFFFFFFF) gets both children's boxes, a leaf child
+// as its face | kLeafRef, an internal child as its node number; leaf node
+// slots stay unused. Faces as in k_accel_build.
+__global__ void k_accel_build_bvh(const float4* __restrict__ bvh, uint32_t n, const clr_float3* __restrict__ verts,
+                                  const clr_face* __restrict__ faces, float4* __restrict__ anodes,
+                                  float4* __restrict__ atris) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numNodes = 2 * n - 1;
+    if (i < numNodes) {
+        const uint2 c = *reinterpret_cast<const uint2*>(bvh + 3 * (size_t)i + 2);
+        if (c.y != kEmpty && c.x < numNodes && c.y < numNodes) {
+            float4 box[2][2];
+            uint32_t ref[2];
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t ci = k ? c.y : c.x;
+                const float4* p = bvh + 3 * (size_t)ci;
+                box[k][0] = p[0];
+                box[k][1] = p[1];
+                const uint2 cc = *reinterpret_cast<const uint2*>(p + 2);
+                ref[k] = cc.y == kEmpty ? (cc.x | dev::kLeafRef) : ci;
+            }
+            float4* o = anodes + 4 * (size_t)i;
+            o[0] = make_float4(box[0][0].x, box[0][0].y, box[0][0].z, __uint_as_float(ref[0]));
+            o[1] = make_float4(box[0][1].x, box[0][1].y, box[0][1].z, __uint_as_float(ref[1]));
+            o[2] = make_float4(box[1][0].x, box[1][0].y, box[1][0].z, 0.0f);
+            o[3] = make_float4(box[1][1].x, box[1][1].y, box[1][1].z, 0.0f);
+        }
+    }
+    if (i < n) {
+        const uint32_t* f = reinterpret_cast<const uint32_t*>(faces + i);
+        float4* o = atris + 3 * (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            const float4 v = *reinterpret_cast<const float4*>(verts + f[k]);
+            o[k] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
+    }
+}
+
 // Ordered traversal, grid-stride over the rays with a fixed grid (waves per
 // CU x CUs) so that the traversal stack's overflow region is per lane:
 // kOrdLds slots in LDS ([slot][lane]), the rest in `spill`.
@@ -98,12 +138,12 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
     const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces, const float* __restrict__ rays,
     uint32_t numRays, uint32_t* __restrict__ hitFace, float* __restrict__ hitT, uint32_t* __restrict__ stats,
     uint32_t* __restrict__ status, uint32_t* __restrict__ spill, const float4* __restrict__ anodes,
-    const float4* __restrict__ atris) {
+    const float4* __restrict__ atris, const float4* __restrict__ bvh) {
     __shared__ uint32_t s_stack[kOrdLds * kOrdBlock];
     const uint32_t gid = blockIdx.x * kOrdBlock + threadIdx.x;
     const uint32_t lanes = gridDim.x * kOrdBlock;
     const dev::SpillStack<kOrdLds> stk{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
-    const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris};
+    const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris, bvh};
     const size_t R = numRays;
     bool over = false;
     for (uint32_t r = gid; r < numRays; r += lanes) {
@@ -126,10 +166,15 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
 
 }  // namespace
 
-int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-               const clr_float3* vertices, const clr_face* faces, const float4** anodes, const float4** atris,
-               hipStream_t stream) {
-    const size_t nodeBytes = (((size_t)std::max(n, 2u) - 1) * 64 + 255) & ~size_t(255);
+size_t accelNodeBytes(uint32_t n, bool midpoint) {
+    const size_t slots = midpoint ? 2 * (size_t)n - 1 : (size_t)std::max(n, 2u) - 1;
+    return (slots * 64 + 255) & ~size_t(255);
+}
+
+int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+               uint32_t n, const clr_float3* vertices, const clr_face* faces, const float4** anodes,
+               const float4** atris, hipStream_t stream) {
+    const size_t nodeBytes = accelNodeBytes(n, bvh != nullptr);
     const size_t need = nodeBytes + (size_t)n * 48;
     if (ctx->accelBytes < need) {
         int rc = hipCheck(ctx, hipDeviceSynchronize(), "accel regrow sync");
@@ -146,20 +191,24 @@ int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* 
     }
     float4* an = static_cast<float4*>(ctx->accel);
     float4* at = reinterpret_cast<float4*>(static_cast<char*>(ctx->accel) + nodeBytes);
-    k_accel_build<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, vertices, faces, an, at);
+    launchAccelBuild(inodes, leaves, bvh, n, vertices, faces, an, at, stream);
     *anodes = an;
     *atris = at;
     return hipCheck(ctx, hipGetLastError(), "traversal layout launch");
 }
 
-void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-                      const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
+void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+                      uint32_t n, const clr_float3* vertices, const clr_face* faces, float4* anodes, float4* atris,
                       hipStream_t stream) {
-    k_accel_build<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, vertices, faces, anodes, atris);
+    if (bvh)
+        k_accel_build_bvh<<<(2 * n - 1 + 255) / 256, 256, 0, stream>>>(reinterpret_cast<const float4*>(bvh), n,
+                                                                        vertices, faces, anodes, atris);
+    else
+        k_accel_build<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, vertices, faces, anodes, atris);
 }
 
-int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n,
-                 const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
+int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
+                 uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
                  uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream) {
     if (numRays == 0) return CLR_OK;
     int rc = ensureStatus(ctx);
@@ -173,20 +222,21 @@ int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     rc = ensureTraceSpill(ctx, lanes * (dev::kStackDepth - kOrdLds) * sizeof(uint32_t));
     if (rc) return rc;
     const float4 *anodes = nullptr, *atris = nullptr;
-    rc = buildAccel(ctx, inodes, leaves, n, vertices, faces, &anodes, &atris, stream);
+    rc = buildAccel(ctx, inodes, leaves, bvh, n, vertices, faces, &anodes, &atris, stream);
     if (rc) return rc;
+    const float4* b4 = reinterpret_cast<const float4*>(bvh);
     if (stats)
         k_trace_ordered<true><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
                                                               hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
-                                                              anodes, atris);
+                                                              anodes, atris, b4);
     else
         k_trace_ordered<false><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
                                                                hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
-                                                               anodes, atris);
+                                                               anodes, atris, b4);
     return hipCheck(ctx, hipGetLastError(), "trace launch");
 }
 
-int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
+int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays,
                  uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
                  hipStream_t stream) {
@@ -194,12 +244,13 @@ int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     int rc = ensureStatus(ctx);
     if (rc) return rc;
     const unsigned grid = (numRays + kTraceBlock - 1) / kTraceBlock;
+    const float4* b4 = reinterpret_cast<const float4*>(bvh);
     if (stats)
         k_trace_closest<true><<<grid, kTraceBlock, 0, stream>>>(
-            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus);
+            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus, b4);
     else
         k_trace_closest<false><<<grid, kTraceBlock, 0, stream>>>(
-            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus);
+            inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats, ctx->devStatus, b4);
     return hipCheck(ctx, hipGetLastError(), "trace launch");
 }
 

@@ -54,10 +54,16 @@ struct GeomScene {
     uint32_t n;
     const clr_float3* verts;
     const clr_face* faces;
-    const float4* anodes = nullptr;  // [n - 1][4]
+    const float4* anodes = nullptr;  // [n - 1][4] (LBVH) or [2n - 1][4] by node (midpoint BVH)
     const float4* atris = nullptr;   // [n][3]
+    // CPU midpoint BVH (clr_bvh_node[2n - 1], 3 float4 each). When set, the
+    // reference-order walks use it instead of inodes / leaves, and the
+    // traversal layout is indexed by its node numbers.
+    const float4* bvh = nullptr;
 };
 constexpr uint32_t kLeafRef = 0x80000000u;
+static_assert(sizeof(clr_lbvh_inode) == 48 && sizeof(clr_lbvh_leaf) == 48 && sizeof(clr_bvh_node) == 48,
+              "node records are three float4 with the box first");
 
 struct TraceStats {
     uint32_t visits, tests;
@@ -151,6 +157,67 @@ struct SpillStack {
     }
 };
 
+// rayIntersection over the CPU midpoint BVH (bvh_traversal.cl:256-285): pop a
+// node, test its box against the current t (boxTest); an internal node pushes
+// children[1] then children[0], a leaf (children[1] == 0xFFFFFFFF) tests
+// triangle children[0], replacing the hit when tt <= t. ANY: stop at the
+// first accepted triangle with a finite distance. Child or face indices out
+// of range, more pops than nodes, or a push beyond 64 entries stop the walk
+// or drop the push and set *overflow.
+template <bool STATS, bool ANY, class Alpha, class Stack>
+__device__ __forceinline__ uint32_t bvhWalk(const GeomScene& g, V3 o, V3 d, float& t, const Stack& stack,
+                                             TraceStats& st, float* b1Out, float* b2Out, const Alpha& alpha,
+                                             bool* overflow) {
+    uint32_t best = kNone;
+    float bb1 = 0.0f, bb2 = 0.0f;
+    const uint32_t numNodes = 2u * g.n - 1u;
+    int depth = 0;
+    bool over = false;
+    if (g.n) {
+        stack.put(0, 0u);
+        depth = 1;
+    }
+    uint32_t iters = 0;
+    while (depth > 0) {
+        if (++iters > numNodes + 1u) {
+            over = true;
+            break;
+        }
+        const uint32_t idx = stack.get(--depth);
+        const float4* np = g.bvh + 3 * (size_t)idx;
+        const float4 mn = np[0], mx = np[1];
+        if (!boxTest(mn, mx, o, d, t)) continue;
+        const uint2 c = *reinterpret_cast<const uint2*>(np + 2);
+        if (c.y != kNone) {
+            if constexpr (STATS) ++st.visits;
+            if (depth + 2 <= kStackDepth && c.x < numNodes && c.y < numNodes) {
+                stack.put(depth++, c.y);
+                stack.put(depth++, c.x);
+            } else {
+                over = true;
+            }
+        } else {
+            if constexpr (STATS) ++st.tests;
+            if (c.x >= g.n) {
+                over = true;
+                continue;
+            }
+            float tt = t, b1, b2;
+            if (!triTest(g, o, d, c.x, tt, b1, b2)) continue;
+            if (alpha.reject(c.x, b1, b2)) continue;
+            t = tt;
+            best = c.x;
+            bb1 = b1;
+            bb2 = b2;
+            if (ANY && !isinf(tt)) break;
+        }
+    }
+    if (overflow && over) *overflow = true;
+    if (b1Out) *b1Out = bb1;
+    if (b2Out) *b2Out = bb2;
+    return best;
+}
+
 // rayIntersection (bvh_traversal.cl:182-215). Returns the face or kNone; t,
 // b1, b2 describe the accepted hit. An alpha-rejected triangle leaves t
 // unchanged (the reference stores t before its alpha test, :87 vs :102-105,
@@ -158,11 +225,11 @@ struct SpillStack {
 // beyond 64 entries (the reference's array bound) is dropped and reported
 // through *overflow.
 template <bool STATS, class Alpha = NoAlpha, class Stack = LdsStack>
-__device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, float& t,
-                                               const Stack& stack, TraceStats& st,
-                                               float* b1Out = nullptr, float* b2Out = nullptr,
-                                               const Alpha& alpha = Alpha(),
-                                               bool* overflow = nullptr) {
+__device__ __forceinline__ uint32_t lbvhClosest(const GeomScene& g, V3 o, V3 d, float& t,
+                                                const Stack& stack, TraceStats& st,
+                                                float* b1Out = nullptr, float* b2Out = nullptr,
+                                                const Alpha& alpha = Alpha(),
+                                                bool* overflow = nullptr) {
     uint32_t best = kNone;
     float bb1 = 0.0f, bb2 = 0.0f;
     auto tryLeaf = [&](uint32_t leafIdx) {
@@ -213,6 +280,18 @@ __device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, f
     return best;
 }
 
+// The reference's walk over whichever structure the scene carries: the CPU
+// midpoint BVH when g.bvh is set, else the LBVH.
+template <bool STATS, class Alpha = NoAlpha, class Stack = LdsStack>
+__device__ __forceinline__ uint32_t closestHit(const GeomScene& g, V3 o, V3 d, float& t,
+                                               const Stack& stack, TraceStats& st,
+                                               float* b1Out = nullptr, float* b2Out = nullptr,
+                                               const Alpha& alpha = Alpha(),
+                                               bool* overflow = nullptr) {
+    if (g.bvh) return bvhWalk<STATS, false>(g, o, d, t, stack, st, b1Out, b2Out, alpha, overflow);
+    return lbvhClosest<STATS>(g, o, d, t, stack, st, b1Out, b2Out, alpha, overflow);
+}
+
 // Occlusion query with the same visiting order and tests as closestHit with
 // t = +inf, stopped at the first accepted triangle with a finite distance:
 // true exactly when closestHit would report a hit.
@@ -220,6 +299,11 @@ template <class Alpha = NoAlpha, class Stack = LdsStack>
 __device__ __forceinline__ bool anyHit(const GeomScene& g, V3 o, V3 d, const Stack& stack,
                                        const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
     float t = INFINITY;
+    if (g.bvh) {
+        TraceStats st{0u, 0u};
+        return bvhWalk<false, true>(g, o, d, t, stack, st, nullptr, nullptr, alpha, overflow) != kNone &&
+               !isinf(t);
+    }
     auto tryLeaf = [&](uint32_t leafIdx) -> bool {
         const float4* lp = reinterpret_cast<const float4*>(g.leaves + leafIdx);
         const float4 mn = lp[0], mx = lp[1];
@@ -371,11 +455,14 @@ struct Ordered {
         if (g.n == 0) {
             done = true;
         } else if (g.n == 1) {
-            const float4* lp = reinterpret_cast<const float4*>(g.leaves);
+            // The single leaf: LBVH leaf 0 (objIdx) or the midpoint root leaf
+            // (children[0]); both hold the box in their first 32 bytes and
+            // the face at byte 32.
+            const float4* lp = g.bvh ? g.bvh : reinterpret_cast<const float4*>(g.leaves);
             if (slabEnter(lp[0], lp[1], o, inv, t) <= t) leaf(reinterpret_cast<const uint32_t*>(lp + 2)[0]);
             done = true;
         } else {
-            const float4* np = reinterpret_cast<const float4*>(g.inodes);
+            const float4* np = g.bvh ? g.bvh : reinterpret_cast<const float4*>(g.inodes);
             if (slabEnter(np[0], np[1], o, inv, t) <= t) node = 0;
             else done = true;
         }
@@ -418,8 +505,10 @@ struct Ordered {
             }
             return;
         }
+        // The candidate's own box: first 32 bytes of its 48-byte record (LBVH
+        // internal node, or midpoint BVH node by node number).
         const uint32_t cand = stack.get(--depth);
-        const float4* cp = reinterpret_cast<const float4*>(g.inodes + cand);
+        const float4* cp = (g.bvh ? g.bvh : reinterpret_cast<const float4*>(g.inodes)) + 3 * (size_t)cand;
         if (slabEnter(cp[0], cp[1], o, inv, t) <= t) node = cand;
         else if (depth == 0) done = true;
     }

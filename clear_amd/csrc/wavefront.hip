@@ -990,7 +990,7 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
     size_t bytes = align(sizeof(RScene) * numScenes) + align(sizeof(int) * numScenes);
     for (uint32_t s = 0; s < numScenes; ++s) {
         const uint32_t n = scenes[s].numFaces;
-        bytes += align(64ull * (std::max(n, 2u) - 1)) + align(48ull * n) + align(4ull * n);
+        bytes += accelNodeBytes(n, scenes[s].bvhNodes != nullptr) + align(48ull * n) + align(4ull * n);
     }
     if (ctx->batchBytes < bytes) {
         rc = hipCheck(ctx, hipDeviceSynchronize(), "batch buffer regrow sync");
@@ -1013,14 +1013,15 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
     for (uint32_t s = 0; s < numScenes; ++s) {
         const clr_scene_dev& sd = scenes[s];
         const uint32_t n = sd.numFaces;
-        float4* an = c.take<float4>(4ull * (std::max(n, 2u) - 1));
+        float4* an = c.take<float4>(accelNodeBytes(n, sd.bvhNodes != nullptr) / sizeof(float4));
         float4* at = c.take<float4>(3ull * n);
         uint32_t* fl = c.take<uint32_t>(n);
-        launchAccelBuild(sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at, stream);
+        launchAccelBuild(sd.internalNodes, sd.leafNodes, sd.bvhNodes, n, sd.vertices, sd.faces, an, at, stream);
         launchFaceLights(sd.lights, sd.numLights, n, fl, envWords + s, stream);
         RScene& S = table[s];
         std::memset(&S, 0, sizeof(S));
-        S.g = GeomScene{sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at};
+        S.g = GeomScene{sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at,
+                        reinterpret_cast<const float4*>(sd.bvhNodes)};
         S.normals = sd.normals;
         S.tangents = sd.tangents;
         S.uvs = sd.uvs;

@@ -72,6 +72,17 @@ typedef struct clr_lbvh_leaf {
     uint8_t dum0[12];
 } clr_lbvh_leaf;
 
+/* BVHNode of the CPU midpoint BVH, 48 bytes (BVH.hpp:61-68;
+ * bvh_traversal.cl:33-37). Node 0 is the root; a leaf has children[1] ==
+ * 0xFFFFFFFF and children[0] = its face index. A tree over n faces has
+ * 2n - 1 nodes. */
+typedef struct clr_bvh_node {
+    float min[4];
+    float max[4];
+    uint32_t children[2];
+    uint32_t dum0[2];
+} clr_bvh_node;
+
 /* LightInfo, 8 bytes (Scene.h:23-27; scene.cl:32-35). */
 typedef struct clr_light_info {
     uint8_t atInfinity;
@@ -186,6 +197,19 @@ int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
                               const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
                               uint32_t* stats, void* stream);
 
+/* Closest hit over the CPU midpoint BVH (nodes: device clr_bvh_node[2 *
+ * numFaces - 1], e.g. from clr_bvh_build_midpoint). ordered == 0: the
+ * reference's walk over those nodes (rayIntersection without __USE_LBVH,
+ * bvh_traversal.cl:256-285: pop a node, test its box against the current t,
+ * push children[1] then children[0], test a leaf's triangle replacing the hit
+ * when tt <= t); ordered != 0: the ordered traversal of
+ * clr_trace_closest_ordered on these nodes, whose hits equal that call's on
+ * any BVH of the same faces. Other arguments as clr_trace_closest. */
+int clr_trace_closest_bvh(clr_ctx* ctx, const clr_bvh_node* nodes, uint32_t numFaces,
+                          const clr_float3* vertices, const clr_face* faces, const float* rays,
+                          uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,
+                          int ordered, void* stream);
+
 /* ---- path tracing ---------------------------------------------------------- */
 
 /* Device-resident scene: the arguments of the reference kernel `pathtracing`
@@ -205,6 +229,11 @@ typedef struct clr_scene_dev {
     const clr_lbvh_leaf* leafNodes;
     uint32_t numFaces;
     uint32_t numLights;
+    /* The reference's non-LBVH build (pathtracer.cl:25-33 with BVHNodes in
+     * place of the LBVH pair): when not NULL, rendering traverses this CPU
+     * midpoint BVH (2 * numFaces - 1 nodes) and the LBVH pointers are not
+     * read. */
+    const clr_bvh_node* bvhNodes;
 } clr_scene_dev;
 
 typedef struct clr_render_params {

@@ -14,6 +14,8 @@
 
 #include <stdint.h>
 
+#include "clear_amd.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -31,6 +33,14 @@ int clr_gen_cluster_soup(uint32_t numTris, int seed, uint32_t numClusters, float
  * (z = 1 - 2u, phi = 2 pi v), tmax = +inf. out: float[7][numRays] SoA planes
  * (ox, oy, oz, dx, dy, dz, tmax). */
 int clr_gen_random_rays(uint32_t numRays, int seed, float* out);
+
+/* The CPU midpoint BVH of Scene::build (BVH.hpp:94-252, built when the
+ * reference renders without __USE_LBVH): one leaf per face with the box of
+ * its three vertices, recursive split of the node's box at the centre of its
+ * widest axis by Hoare partition of leaf centres, nodes in depth-first
+ * creation order. HOST memory: vertices (16-B float3), faces, out[2n - 1]. */
+int clr_bvh_build_midpoint(const clr_float3* vertices, const clr_face* faces, uint32_t numFaces,
+                           clr_bvh_node* out);
 
 /* ---- render scenes ------------------------------------------------------
  * Built on the host with the drop-in Scene / MaterialCreator API

@@ -98,6 +98,25 @@ def trace_ordered(inodes, leaves, n, verts, faces, rays, any_hit=False):
     return hit, t, st
 
 
+def trace_bvh(nodes, verts, faces, rays):
+    """Midpoint-BVH walk restatement (reference order): (face, t, stats[R, 2])."""
+    L = lib()
+    fn = L.orc_trace_bvh
+    fn.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint32, vp, vp, vp]
+    R = rays.shape[1]
+    hit = np.zeros(R, dtype=np.uint32)
+    t = np.zeros(R, dtype=np.float32)
+    st = np.zeros((R, 2), dtype=np.uint32)
+    nodes = np.ascontiguousarray(nodes)
+    rc = fn(_p(nodes), nodes.nbytes // 48, _p(np.ascontiguousarray(verts)), _p(np.ascontiguousarray(faces)),
+            _p(np.ascontiguousarray(rays, dtype=np.float32)), R, _p(hit), _p(t), _p(st))
+    if rc:
+        raise RuntimeError(f"orc_trace_bvh failed: {rc}")
+    return hit, t, st
+
+
 def trace_bruteforce_minid(verts, faces, rays):
     """Brute force with the (t, smaller face) rule of the ordered traversal."""
     L = lib()
@@ -170,10 +189,29 @@ def seed_states(width, height):
     return st
 
 
+def _scene_ptrs(arrays, num_faces, inodes, leaves, bvh):
+    keep = [np.ascontiguousarray(arrays[k]) for k in
+            ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
+             "others"]]
+    keep.append(np.ascontiguousarray(inodes) if inodes is not None and num_faces > 1
+                else np.zeros((1, 48), np.uint8))
+    keep.append(np.ascontiguousarray(leaves) if leaves is not None else np.zeros((1, 48), np.uint8))
+    ptrs = [k.ctypes.data for k in keep]
+    if bvh is not None:
+        keep.append(np.ascontiguousarray(bvh))
+        ptrs.append(keep[-1].ctypes.data)
+    else:
+        ptrs.append(None)
+    return keep, (ctypes.c_void_p * 12)(*ptrs)
+
+
 def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, states, pixels,
-           pix_begin=0, pix_end=None, threads=0, flags=0):
+           pix_begin=0, pix_end=None, threads=0, flags=0, bvh=None):
     """Runs the render restatement over pixels [pix_begin, pix_end) (row-major),
-    accumulating into `pixels` (float32 [H*W*4]) and advancing `states`."""
+    accumulating into `pixels` (float32 [H*W*4]) and advancing `states`.
+    With `bvh` (CPU midpoint BVH nodes) the reference-order walk
+    (flags & RENDER_REFERENCE_TRAVERSAL) traverses it; the ordered traversal,
+    whose hits do not depend on the structure, always uses the LBVH."""
     L = lib()
     if not hasattr(L.orc_render, "_typed"):
         L.orc_render.restype = ctypes.c_int
@@ -181,12 +219,7 @@ def render(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, sta
                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]
         L.orc_render._typed = True
-    keep = [np.ascontiguousarray(arrays[k]) for k in
-            ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
-             "others"]]
-    keep.append(np.ascontiguousarray(inodes) if num_faces > 1 else np.zeros((1, 48), np.uint8))
-    keep.append(np.ascontiguousarray(leaves))
-    ptrs = (ctypes.c_void_p * 11)(*[k.ctypes.data for k in keep])
+    keep, ptrs = _scene_ptrs(arrays, num_faces, inodes, leaves, bvh)
     end = width * height if pix_end is None else pix_end
     rc = L.orc_render(ptrs, num_faces, width, height, spp, max_depth, _p(states), _p(pixels),
                       pix_begin, end, threads, flags)
@@ -230,19 +263,14 @@ def disk_probe(u1, u2):
 
 
 def render_adjoint(arrays, num_faces, inodes, leaves, width, height, spp, max_depth, flags, mat_offsets,
-                   light_prop, states, d_image, threads=0):
+                   light_prop, states, d_image, threads=0, bvh=None):
     """CPU adjoint: (g_albedo [M,3], g_sigma [M], g_emission [3])."""
     L = lib()
     fn = L.orc_render_adjoint
     fn.restype = ctypes.c_int
     vp, u32 = ctypes.c_void_p, ctypes.c_uint32
     fn.argtypes = [vp, u32, u32, u32, u32, u32, u32, u32, vp, u32, vp, vp, vp, vp, vp, ctypes.c_int]
-    keep = [np.ascontiguousarray(arrays[k]) for k in
-            ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials", "textures",
-             "others"]]
-    keep.append(np.ascontiguousarray(inodes) if num_faces > 1 else np.zeros((1, 48), np.uint8))
-    keep.append(np.ascontiguousarray(leaves))
-    ptrs = (ctypes.c_void_p * 11)(*[k.ctypes.data for k in keep])
+    keep, ptrs = _scene_ptrs(arrays, num_faces, inodes, leaves, bvh)
     m = len(mat_offsets)
     mo = np.ascontiguousarray(mat_offsets, dtype=np.uint32)
     ga = np.zeros((max(m, 1), 3), np.float32)

@@ -101,6 +101,9 @@ struct SceneC {
     const uint8_t* environment;
     const uint8_t* lightDist;
     bool ordered;  // ordered traversal (ordered_traversal.h) instead of the reference's walk
+    // CPU midpoint BVH: the reference's walk uses it instead of the LBVH
+    // (the ordered traversal's hits do not depend on the structure).
+    const clr_bvh_node* bvh;
 };
 
 inline V3 vtx(const clr_float3* a, uint32_t i) { return {a[i].x, a[i].y, a[i].z}; }
@@ -451,7 +454,24 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
         if (!boxTest(L.min, L.max, org, dir, t)) return;
         if (triTest(s, org, dir, L.objIdx, &t, &cur)) best = cur;
     };
-    if (s.numFaces == 1) {
+    if (s.bvh) {
+        // rayIntersection without __USE_LBVH (bvh_traversal.cl:256-285).
+        uint32_t stack[64];
+        uint32_t depth = 0;
+        stack[depth++] = 0;
+        while (depth > 0) {
+            const clr_bvh_node& N = s.bvh[stack[--depth]];
+            if (!boxTest(N.min, N.max, org, dir, t)) continue;
+            if (N.children[1] != 0xFFFFFFFFu) {
+                if (depth + 2 <= 64) {
+                    stack[depth++] = N.children[1];
+                    stack[depth++] = N.children[0];
+                }
+            } else if (triTest(s, org, dir, N.children[0], &t, &cur)) {
+                best = cur;
+            }
+        }
+    } else if (s.numFaces == 1) {
         tryLeaf(0);
     } else {
         uint32_t stack[64];
@@ -1270,7 +1290,8 @@ int orc_seed_states(uint32_t width, uint32_t height, uint32_t* states) {
 // Renders `spp` samples for pixels [pixBegin, pixEnd) (row-major index),
 // accumulating into pixels (clr_float3 per pixel) and advancing states.
 // arrays: vertices, normals, tangents, uvs, faces, lights, materials, textures,
-// others, inodes, leaves. threads <= 0 uses all hardware threads.
+// others, inodes, leaves, midpoint BVH nodes (or null). threads <= 0 uses all
+// hardware threads.
 // Adjoint of orc_render over pixels [0, width * height): gradients (float out)
 // of sum_p dLdI[p] . (radiance added for p) / spp, states not modified.
 int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t width, uint32_t height,
@@ -1289,6 +1310,7 @@ int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t wi
     s.others = (const uint8_t*)arrays[8];
     s.inodes = (const clr_lbvh_inode*)arrays[9];
     s.leaves = (const clr_lbvh_leaf*)arrays[10];
+    s.bvh = (const clr_bvh_node*)arrays[11];
     s.numFaces = numFaces;
     s.camera = s.others + rd<uint32_t>(s.others + 0);
     s.environment = s.others + rd<uint32_t>(s.others + 4);
@@ -1352,6 +1374,7 @@ int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uin
     s.others = (const uint8_t*)arrays[8];
     s.inodes = (const clr_lbvh_inode*)arrays[9];
     s.leaves = (const clr_lbvh_leaf*)arrays[10];
+    s.bvh = (const clr_bvh_node*)arrays[11];
     s.numFaces = numFaces;
     s.camera = s.others + rd<uint32_t>(s.others + 0);
     s.environment = s.others + rd<uint32_t>(s.others + 4);

@@ -141,6 +141,49 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
     return 0;
 }
 
+// rayIntersection over the CPU midpoint BVH (bvh_traversal.cl:256-285):
+// explicit stack of 64, node 0 first; pop, test the node's box against the
+// current t; an internal node pushes children[1] then children[0] (so
+// children[0] is visited first); a leaf (children[1] == 0xFFFFFFFF) tests
+// triangle children[0], replacing the hit when tt <= t. stats: (internal
+// nodes whose box passed, triangle tests) per ray.
+int orc_trace_bvh(const clr_bvh_node* nodes, uint32_t numNodes, const clr_float3* verts,
+                  const clr_face* faces, const float* rays, uint32_t numRays, uint32_t* hitFace,
+                  float* hitT, uint32_t* stats) {
+    for (uint32_t r = 0; r < numRays; ++r) {
+        const size_t R = numRays;
+        const V3 org = {rays[r], rays[R + r], rays[2 * R + r]};
+        const V3 dir = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+        float t = rays[6 * R + r];
+        uint32_t best = 0xFFFFFFFFu, visits = 0, tests = 0;
+        uint32_t stack[64];
+        uint32_t depth = 0;
+        if (numNodes) stack[depth++] = 0;
+        while (depth > 0) {
+            const clr_bvh_node& N = nodes[stack[--depth]];
+            if (!boxTest(N.min, N.max, org, dir, t)) continue;
+            if (N.children[1] != 0xFFFFFFFFu) {
+                ++visits;
+                if (depth + 2 > 64) return 2;  // stack bound exceeded
+                if (N.children[0] >= numNodes || N.children[1] >= numNodes) return 3;
+                stack[depth++] = N.children[1];
+                stack[depth++] = N.children[0];
+            } else {
+                ++tests;
+                if (triTest(verts, faces, org, dir, N.children[0], &t)) best = N.children[0];
+            }
+        }
+        const bool hit = best != 0xFFFFFFFFu && !std::isinf(t);
+        hitFace[r] = hit ? best : 0xFFFFFFFFu;
+        hitT[r] = t;
+        if (stats) {
+            stats[2 * (size_t)r + 0] = visits;
+            stats[2 * (size_t)r + 1] = tests;
+        }
+    }
+    return 0;
+}
+
 // Ordered traversal (ordered_traversal.h) with the geometric triangle test:
 // face, t and per-ray (visits, tests); anyHit != 0 stops at the first
 // accepted triangle (hitFace then only tells hit / miss).

@@ -152,3 +152,46 @@ def test_render_rejects_bad_arguments(gpu_ctx):
     with pytest.raises(ca.ClrError):
         ca.render(gpu_ctx, ds, 1, 0, states, pixels)
     torch.cuda.synchronize()
+
+
+def test_midpoint_bvh_scene(gpu_ctx):
+    """clr_scene_dev.bvhNodes (the CPU midpoint BVH of Scene::build): with the
+    ordered traversal the image and states equal the LBVH render's bit for bit
+    (megakernel and wavefront); with the reference's walk over the midpoint
+    nodes the megakernel matches the oracle's midpoint render at the render
+    bar."""
+    import torch
+
+    W = H = 96
+    spp, depth = 8, 5
+    host = ca.HostScene(ca.SCENE_SHOWCASE, W, H, cpu_bvh=True)
+    print(f"[render parity] midpoint BVH showcase {W}x{H} spp={spp}", flush=True)
+    lb = ca.DeviceScene(gpu_ctx, host)
+    mp = ca.DeviceScene(gpu_ctx, host, accel="midpoint")
+    outs = {}
+    for name, ds, mode, flags in [("lbvh_mega", lb, ca.RENDER_MEGAKERNEL, 0),
+                                  ("mid_mega", mp, ca.RENDER_MEGAKERNEL, 0),
+                                  ("mid_wave", mp, ca.RENDER_WAVEFRONT, 0),
+                                  ("mid_ref", mp, ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL)]:
+        st, px = ca.new_image(W, H)
+        ca.seed_pixel_states(gpu_ctx, W, H, st)
+        ca.render(gpu_ctx, ds, spp, depth, st, px, mode=mode, flags=flags)
+        torch.cuda.synchronize()
+        outs[name] = (px.cpu().numpy(), st.cpu().numpy().view(np.uint32))
+    assert gpu_ctx.device_status() == 0
+    for name in ("mid_mega", "mid_wave"):
+        np.testing.assert_array_equal(outs[name][0], outs["lbvh_mega"][0])
+        np.testing.assert_array_equal(outs[name][1], outs["lbvh_mega"][1])
+    nodes = host.arrays["bvh_nodes"][: (2 * host.num_faces - 1) * 48]
+    ost = orc.seed_states(W, H)
+    opx = np.zeros(W * H * 4, np.float32)
+    orc.render(host.arrays, host.num_faces, None, None, W, H, spp, depth, ost, opx, threads=THREADS,
+               flags=ca.RENDER_REFERENCE_TRAVERSAL, bvh=nodes)
+    gi = outs["mid_ref"][0].reshape(-1, 4)[:, :3]
+    oi = opx.reshape(-1, 4)[:, :3]
+    ok = (np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-6 * spp).all(axis=1)
+    same = (outs["mid_ref"][1].reshape(-1, 4) == ost.reshape(-1, 4)).all(axis=1)
+    print(f"[render parity] midpoint reference walk: pixels within tol {ok.mean():.6f}, "
+          f"identical states {same.mean():.6f}, mean {oi.mean():.5f}", flush=True)
+    assert oi.mean() > 1e-4
+    assert ok.mean() >= 0.999 and same.mean() >= 0.999

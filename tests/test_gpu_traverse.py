@@ -106,3 +106,64 @@ def test_ordered_1m_tris_256k_rays_match_reference_walk(gpu_ctx):
     o = orc.lbvh_build(v, f, 10)
     gh, gt, _ = run_gpu(gpu_ctx, o["inodes"], o["leaves"], 1 << 20, v, f, rays)
     assert (gh == h).mean() > 0.99999
+
+
+# ---- CPU midpoint BVH (clr_trace_closest_bvh) --------------------------------
+
+def run_gpu_bvh(ctx, nodes, v, f, rays, ordered):
+    import torch
+
+    R = rays.shape[1]
+    tn = torch.from_numpy(np.ascontiguousarray(nodes).view(np.uint8).reshape(-1)).cuda()
+    tv = torch.from_numpy(v).cuda()
+    tf = torch.from_numpy(f.view(np.uint8)).cuda()
+    tr = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    hit = torch.empty(R, dtype=torch.int32, device="cuda")
+    t = torch.empty(R, dtype=torch.float32, device="cuda")
+    st = torch.empty(2 * R, dtype=torch.int32, device="cuda")
+    ca.trace_closest_bvh(ctx, tn, len(f), tv, tf, tr, R, hit, t, st, ordered=ordered)
+    torch.cuda.synchronize()
+    return (hit.cpu().numpy().view(np.uint32), t.cpu().numpy(), st.cpu().numpy().view(np.uint32).reshape(R, 2))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 65, 4097, 20000])
+def test_midpoint_bvh_walks(gpu_ctx, n):
+    """Reference-order walk over the midpoint BVH: face, t and (visits, tests)
+    equal the oracle's walk bit for bit; ordered traversal over the same nodes:
+    equal to the brute-force lexicographic minimum of (t, face)."""
+    soup = ca.gen_cluster_soup(n, seed=n + 30, num_clusters=8, sigma=0.05, e_min=0.005, e_max=0.03)
+    v, f = ca.soup_to_mesh(soup)
+    nodes = ca.build_midpoint_bvh(v, f)
+    rays = ca.gen_random_rays(8192, seed=n + 31)
+    rays[3:6, :2048] = 0.0
+    rays[3 + (np.arange(2048) % 3), np.arange(2048)] = np.where(np.arange(2048) % 2, 1.0, -1.0)
+    rays[6, 6144:] = np.float32(0.3)
+    print(f"[trace midpoint] n={n} rays={rays.shape[1]}", flush=True)
+    h, t, st = orc.trace_bvh(nodes, v, f, rays)
+    gh, gt, gst = run_gpu_bvh(gpu_ctx, nodes, v, f, rays, ordered=False)
+    np.testing.assert_array_equal(gh, h)
+    np.testing.assert_array_equal(gt.view(np.uint32), t.view(np.uint32))
+    np.testing.assert_array_equal(gst, st)
+    bh, bt = orc.trace_bruteforce_minid(v, f, rays)
+    oh, ot, _ = run_gpu_bvh(gpu_ctx, nodes, v, f, rays, ordered=True)
+    np.testing.assert_array_equal(oh, bh)
+    np.testing.assert_array_equal(ot.view(np.uint32), bt.view(np.uint32))
+    assert gpu_ctx.device_status() == 0
+    if n >= 4097:
+        assert (oh != 0xFFFFFFFF).sum() > 200
+
+
+def test_midpoint_and_lbvh_ordered_hits_agree_1m(gpu_ctx):
+    """The ordered traversal returns the same (face, t) over the LBVH and the
+    midpoint BVH of a 1M-triangle soup (both equal the lexicographic minimum)."""
+    soup = ca.gen_cluster_soup(1 << 20, seed=1)
+    v, f = ca.soup_to_mesh(soup)
+    nodes = ca.build_midpoint_bvh(v, f)
+    rays = ca.gen_random_rays(1 << 18, seed=2)
+    o = orc.lbvh_build(v, f, 10)
+    print("[trace midpoint] 1M soup, ordered over both structures", flush=True)
+    lh, lt, _ = run_gpu(gpu_ctx, o["inodes"], o["leaves"], 1 << 20, v, f, rays, ordered=True)
+    mh, mt, _ = run_gpu_bvh(gpu_ctx, nodes, v, f, rays, ordered=True)
+    np.testing.assert_array_equal(mh, lh)
+    np.testing.assert_array_equal(mt.view(np.uint32), lt.view(np.uint32))
+    assert gpu_ctx.device_status() == 0
