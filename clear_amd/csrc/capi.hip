@@ -186,6 +186,10 @@ int clr_create(clr_ctx** out, int device) {
                 delete ctx;
                 return CLR_ERR_HIP;
             }
+    if (hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->numCUs = 0;
+    }
     *out = ctx;
     return CLR_OK;
 }

@@ -14,6 +14,7 @@ struct WfState;
 
 struct clr_ctx {
     int device = 0;
+    int numCUs = 0;  // compute units of the device (launch sizing)
     std::string lastError;
     // Scratch pool owned by the context, regrown on demand (the reference's
     // m_bufferGenericPool, LBVHBuilder.cpp:130).

@@ -54,7 +54,7 @@ template <typename KeyT>
 struct SortCfg;
 template <>
 struct SortCfg<uint32_t> {
-    static constexpr int kItems = 16;
+    static constexpr int kItems = 32;
 };
 template <>
 struct SortCfg<uint64_t> {
@@ -159,7 +159,31 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
                                                    uint32_t* __restrict__ sceneOrd) {
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    // A 16-B aligned soup is read as float4 groups of three (12 floats: four
+    // vertices x, y, z, x, y, z, ...), so axis k of a group is always lanes k,
+    // k + 3, k + 6, k + 9; the scene box is the min / max over all vertices,
+    // which equals the union of the triangle boxes.
+    const bool vec = SOUP && (reinterpret_cast<uintptr_t>(tris) & 15u) == 0;
+    if (vec) {
+        const float4* t4 = reinterpret_cast<const float4*>(tris);
+        const size_t floats = 9 * (size_t)n;
+        const size_t groups = floats / 12;
+        for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += (size_t)gridDim.x * kBlock) {
+            const float4 a = t4[3 * g], b = t4[3 * g + 1], c = t4[3 * g + 2];
+            const float v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+            for (int j = 0; j < 12; ++j) {
+                lo[j % 3] = omin(lo[j % 3], v[j]);
+                hi[j % 3] = omax(hi[j % 3], v[j]);
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (size_t f = groups * 12; f < floats; ++f) {
+                lo[f % 3] = omin(lo[f % 3], tris[f]);
+                hi[f % 3] = omax(hi[f % 3], tris[f]);
+            }
+    }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; !vec && i < n; i += gridDim.x * kBlock) {
         Box b;
         if constexpr (SOUP) {
             b = soupBox(tris, i);
@@ -868,11 +892,13 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     ev(0);
     k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0,
              stream>>>(L.hist, L.clearWords, L.sceneOrd);
-    k_bounds<SOUP><<<gridFor(n, 4096), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n,
-                                                              L.boxes, L.sceneOrd);
+    // One set of same-address atomics per block: a few blocks per CU.
+    const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
+    k_bounds<SOUP><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n,
+                                                                    L.boxes, L.sceneOrd);
     ev(1);
     PrimSrc src{in.tris, L.boxes};
-    k_morton<SOUP, KeyT><<<gridFor(n, 4096), kBlock, 0, stream>>>(
+    k_morton<SOUP, KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
         src, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.otherEnd, L.hist, P);
     ev(2);
     KeyT* kin = static_cast<KeyT*>(L.keysA);

@@ -195,3 +195,35 @@ def test_midpoint_bvh_scene(gpu_ctx):
           f"identical states {same.mean():.6f}, mean {oi.mean():.5f}", flush=True)
     assert oi.mean() > 1e-4
     assert ok.mean() >= 0.999 and same.mean() >= 0.999
+
+
+def test_converged_4096spp_rmse(gpu_ctx):
+    """Converged images: 4096 samples per pixel of the Cornell box (48x48,
+    depth 5) from the megakernel, the wavefront and the reference traversal
+    each match the oracle's 4096-spp image to RMSE <= 1e-3 (image = sum /
+    spp; per-pixel paths that diverge on a last-ulp decision average out)."""
+    import torch
+
+    W = H = 48
+    spp, depth = 4096, 5
+    host = ca.HostScene(ca.SCENE_CORNELL, W, H)
+    print(f"[render parity] converged cornell {W}x{H} spp={spp}", flush=True)
+    o = orc.lbvh_build(host.vertices, host.faces, 10)
+    ost = orc.seed_states(W, H)
+    opx = np.zeros(W * H * 4, np.float32)
+    orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, spp, depth, ost, opx,
+               threads=THREADS)
+    ref = opx.reshape(-1, 4)[:, :3] / spp
+    ds = ca.DeviceScene(gpu_ctx, host)
+    for name, mode, flags in [("megakernel", ca.RENDER_MEGAKERNEL, 0), ("wavefront", ca.RENDER_WAVEFRONT, 0),
+                              ("reference traversal", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL)]:
+        st, px = ca.new_image(W, H)
+        ca.seed_pixel_states(gpu_ctx, W, H, st)
+        for _ in range(4):  # 4 calls of 1024 samples accumulate like one call of 4096
+            ca.render(gpu_ctx, ds, spp // 4, depth, st, px, mode=mode, flags=flags)
+        torch.cuda.synchronize()
+        img = px.cpu().numpy().reshape(-1, 4)[:, :3] / spp
+        rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
+        print(f"[render parity] converged {name}: RMSE {rmse:.3g} (image mean {ref.mean():.4f})", flush=True)
+        assert np.isfinite(img).all() and rmse <= 1e-3
+    assert gpu_ctx.device_status() == 0
