@@ -72,6 +72,13 @@ def lib():
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
         "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "clr_render_batch": (i32, [vp, vp, u32, vp, vp, vp, vp]),
+        "clr_post_clear": (i32, [vp, u32, u32, vp, vp]),
+        "clr_post_scaling": (i32, [vp, u32, u32, u32, vp, vp, vp]),
+        "clr_post_gaussian_scattering": (i32, [vp, u32, u32, u32, vp, vp, vp]),
+        "clr_post_bloom": (i32, [vp, u32, u32, u32, vp, vp, vp]),
+        "clr_post_tone_mapping": (i32, [vp, u32, u32, u32, vp, vp, vp]),
+        "clr_bmp_row_bytes": (u32, [u32]),
+        "clr_save_bmp": (i32, [ctypes.c_char_p, vp, u32, u32]),
         "clr_scene_diff_params": (i32, [vp, ctypes.POINTER(u32), vp, vp, vp, ctypes.POINTER(u32),
                                         ctypes.POINTER(u32)]),
         "clr_wavefront_get_config": (i32, [vp, vp]),
@@ -561,6 +568,44 @@ def wavefront_autotune(ctx: Context, scene: "DeviceScene", spp: int, max_depth: 
     ctx.check(lib().clr_wavefront_autotune(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp),
                                            ctypes.byref(c), _stream(stream)))
     return c
+
+
+# ---- post-processing (post_processing.cl, BMPExporter.cpp) ----------------
+
+def post_clear(ctx: Context, width: int, height: int, buf, stream=None) -> None:
+    ctx.check(lib().clr_post_clear(ctx.handle, width, height, _ptr(buf), _stream(stream)))
+
+
+def post_scaling(ctx: Context, width: int, height: int, spp: int, src, out, stream=None) -> None:
+    ctx.check(lib().clr_post_scaling(ctx.handle, width, height, spp, _ptr(src), _ptr(out), _stream(stream)))
+
+
+def post_gaussian_scattering(ctx: Context, width: int, height: int, spp: int, src, out, stream=None) -> None:
+    ctx.check(lib().clr_post_gaussian_scattering(ctx.handle, width, height, spp, _ptr(src), _ptr(out),
+                                                 _stream(stream)))
+
+
+def post_bloom(ctx: Context, width: int, height: int, spp: int, src, out, stream=None) -> None:
+    ctx.check(lib().clr_post_bloom(ctx.handle, width, height, spp, _ptr(src), _ptr(out), _stream(stream)))
+
+
+def post_tone_mapping(ctx: Context, width: int, height: int, row_bytes: int, src, dst, stream=None) -> None:
+    ctx.check(lib().clr_post_tone_mapping(ctx.handle, width, height, row_bytes, _ptr(src), _ptr(dst),
+                                          _stream(stream)))
+
+
+def bmp_row_bytes(width: int) -> int:
+    return int(lib().clr_bmp_row_bytes(width))
+
+
+def save_bmp(path: str, pixels: np.ndarray, width: int, height: int) -> None:
+    """clr_save_bmp of host tone-mapped rows (bmp_row_bytes(width) * height bytes)."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint8)
+    if px.size < bmp_row_bytes(width) * height:
+        raise ValueError("pixel buffer smaller than the BMP data")
+    rc = lib().clr_save_bmp(str(path).encode(), px.ctypes.data, width, height)
+    if rc:
+        raise ClrError(rc, f"clr_save_bmp({path})")
 
 
 def new_image(width: int, height: int, device="cuda"):

@@ -130,22 +130,6 @@ int ensureStatus(clr_ctx* ctx) {
     return hipCheck(ctx, hipDeviceSynchronize(), "status clear");
 }
 
-// Sets the context's device for the scope of one call.
-struct DeviceScope {
-    int prev = -1;
-    hipError_t err = hipSuccess;
-    explicit DeviceScope(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) err = hipSetDevice(dev);
-    }
-    ~DeviceScope() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
-
-inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
-
 }  // namespace clr
 
 using namespace clr;

@@ -85,6 +85,23 @@ int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut);
 int ensureStatus(clr_ctx* ctx);
 
 // Bump allocator over the pool (256-B aligned sub-buffers).
+// Makes the context's device current for the duration of a C-ABI call and
+// restores the caller's device afterwards.
+struct DeviceScope {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceScope(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) err = hipSetDevice(dev);
+    }
+    ~DeviceScope() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 struct Carve {
     char* base;
     size_t off = 0;

@@ -284,6 +284,48 @@ int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScen
                      const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
                      void* stream);
 
+/* ---- post-processing (post_processing.cl, BMPExporter.cpp) ----------------
+ * Device buffers of clr_float3 per pixel (row-major), stream-ordered. src and
+ * output must not overlap. The reference's gaussianScattering and bloom add
+ * with float atomics in arrival order; these add the same terms per output
+ * pixel in a fixed order (deterministic; equal to the reference up to the
+ * order of float additions). */
+
+/* clear (post_processing.cl:16-18): buffer = 0. */
+int clr_post_clear(clr_ctx* ctx, uint32_t width, uint32_t height, clr_float3* buffer, void* stream);
+
+/* scaling (post_processing.cl:20-26): output = src * (1 / spp). */
+int clr_post_scaling(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t spp, const clr_float3* src,
+                     clr_float3* output, void* stream);
+
+/* gaussianScattering (post_processing.cl:47-66): output += 9x9 Gaussian of
+ * src * (1 / spp), taps beyond the image clamped onto its border. */
+int clr_post_gaussian_scattering(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t spp,
+                                 const clr_float3* src, clr_float3* output, void* stream);
+
+/* bloom (post_processing.cl:70-101): per channel, v = src * (1 / spp); a
+ * value above 30 adds 30 to its pixel and exp(-|d|) * (v - 30) to the pixels
+ * at offsets d of the square of radius min(7, (int)(v - 30)) (clamped onto
+ * the border); other values add v to their pixel. output += the result. */
+int clr_post_bloom(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t spp, const clr_float3* src,
+                   clr_float3* output, void* stream);
+
+/* toneMapping (post_processing.cl:103-117): channels saturated at 30, luminance
+ * Y = 0.2989 R + 0.5866 G + 0.1144 B, scale (1 - exp(-Y)) / Y (1 for Y = 0),
+ * byte = trunc(clamp(255 * (scale * c)^(1/2.2), 0, 255)), written B, G, R into
+ * bottom-up rows of rowBytes (>= 3 * width) bytes of the device buffer dst;
+ * the row padding is written as 0. Luminance and scale are computed in float. */
+int clr_post_tone_mapping(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t rowBytes, const clr_float3* src,
+                          uint8_t* dst, void* stream);
+
+/* Bytes per BMP row as the reference computes them: 3 * width + width % 4
+ * (main.cpp:521, BMPExporter.cpp:15). */
+uint32_t clr_bmp_row_bytes(uint32_t width);
+
+/* saveBMP (BMPExporter.cpp:14-53): writes a 24-bit uncompressed BMP (54-byte
+ * header, 1 pixel per metre) of HOST pixels in the tone-mapped layout. */
+int clr_save_bmp(const char* filename, const void* pixels, uint32_t width, uint32_t height);
+
 /* ---- differentiation ------------------------------------------------------
  * No counterpart in the reference. Differentiable parameters: the albedo
  * (RGB) and roughness sigma of materials made of one Oren-Nayar / Lambert

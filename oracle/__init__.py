@@ -283,3 +283,46 @@ def render_adjoint(arrays, num_faces, inodes, leaves, width, height, spp, max_de
     if rc:
         raise RuntimeError(f"orc_render_adjoint failed: {rc}")
     return ga[:m], gs[:m], ge
+
+
+# ---- post-processing restatement (post_oracle.cpp) --------------------------
+
+def _post(name, *args):
+    fn = getattr(lib(), name)
+    fn.restype = ctypes.c_int
+    return fn(*args)
+
+
+def post_scaling(src, spp):
+    """src float32 [H, W, 4] -> src * (1 / spp) (channel 3 untouched: 0)."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    out = np.zeros_like(src)
+    _post("orc_post_scaling", ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_uint32(spp), _p(src), _p(out))
+    return out
+
+
+def post_gaussian(src, spp, out=None):
+    """out (zeros by default) += the reference's clamped 9x9 Gaussian scatter."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    out = np.zeros_like(src) if out is None else out
+    _post("orc_post_gaussian", ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_uint32(spp), _p(src), _p(out))
+    return out
+
+
+def post_bloom(src, spp, out=None):
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    out = np.zeros_like(src) if out is None else out
+    _post("orc_post_bloom", ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_uint32(spp), _p(src), _p(out))
+    return out
+
+
+def post_tone(src, row_bytes):
+    """Tone-mapped bottom-up BGR rows, uint8 [H * row_bytes]."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    dst = np.full(h * row_bytes, 0xAB, np.uint8)
+    _post("orc_post_tone", ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_uint32(row_bytes), _p(src), _p(dst))
+    return dst

@@ -225,5 +225,6 @@ def test_converged_4096spp_rmse(gpu_ctx):
         img = px.cpu().numpy().reshape(-1, 4)[:, :3] / spp
         rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
         print(f"[render parity] converged {name}: RMSE {rmse:.3g} (image mean {ref.mean():.4f})", flush=True)
-        assert np.isfinite(img).all() and rmse <= 1e-3
+        # the bar is absolute RMSE <= 1e-3; this image is dim, so also 1e-3 of its mean
+        assert np.isfinite(img).all() and rmse <= 1e-3 and rmse <= 1e-3 * float(ref.mean())
     assert gpu_ctx.device_status() == 0
