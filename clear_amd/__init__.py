@@ -72,6 +72,7 @@ def lib():
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
         "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "clr_render_batch": (i32, [vp, vp, u32, vp, vp, vp, vp]),
+        "clr_bvh_visualizer": (i32, [vp, vp, vp, vp, vp]),
         "clr_post_clear": (i32, [vp, u32, u32, vp, vp]),
         "clr_post_scaling": (i32, [vp, u32, u32, u32, vp, vp, vp]),
         "clr_post_gaussian_scattering": (i32, [vp, u32, u32, u32, vp, vp, vp]),
@@ -530,6 +531,13 @@ def ctx_render_dev(ctx: Context, dev: SceneDev, width: int, height: int, spp: in
     rp = RenderParams(width, height, spp, max_depth, flags)
     ctx.check(lib().clr_render(ctx.handle, ctypes.byref(dev), ctypes.byref(rp), _ptr(states), _ptr(pixels),
                                mode, _stream(stream)))
+
+
+def bvh_visualizer(ctx: Context, scene: DeviceScene, pixels, flags: int = 0, stream=None) -> None:
+    """clr_bvh_visualizer: pixels += base colour + 0.025 * box hits per pixel."""
+    rp = RenderParams(scene.width, scene.height, 1, 1, flags)
+    ctx.check(lib().clr_bvh_visualizer(ctx.handle, ctypes.byref(scene.dev), ctypes.byref(rp), _ptr(pixels),
+                                       _stream(stream)))
 
 
 def render(ctx: Context, scene: DeviceScene, spp: int, max_depth: int, states, pixels,

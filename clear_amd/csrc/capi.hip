@@ -418,6 +418,20 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
     return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
 }
 
+int clr_bvh_visualizer(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                       clr_float3* pixels, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (!pixels || !params) return fail(ctx, CLR_ERR_INVALID_ARG, "null visualizer argument");
+    clr_render_params rp = *params;
+    rp.maxDepth = 1;  // not used here
+    int rc = checkRenderArgs(ctx, scene, &rp);
+    if (rc) return rc;
+    if (!aligned16(pixels)) return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return renderBvhVisualizer(ctx, *scene, rp, pixels, static_cast<hipStream_t>(stream));
+}
+
 int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
                      const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
                      void* stream) {

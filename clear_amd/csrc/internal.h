@@ -147,6 +147,8 @@ int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg);
 int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes, bool batch,
                       const clr_render_params& rp, clr_wavefront_config* out, hipStream_t stream);
 void destroyWfState(clr_ctx* ctx);
+int renderBvhVisualizer(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, clr_float3* pixels,
+                        hipStream_t stream);
 // Argument checks shared by clr_render and the autotuner.
 int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params);
 

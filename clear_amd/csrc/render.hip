@@ -289,6 +289,57 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
     reinterpret_cast<float4*>(pixels)[pi] = p;
 }
 
+// bvh_visualizer (bvh_visualizer.cl:25-84): one ray through each pixel
+// centre from the lens centre; pixel += the base colour of the hit surface
+// (sample_We's weight is 1), then += 0.025 * the ray's box hits. Box hits are
+// the reference walk's count (rayIntersectionVis, bvh_traversal.cl:216-319:
+// internal and leaf boxes passed for the LBVH, node boxes passed for the
+// midpoint BVH), or for the ordered traversal its entered nodes plus
+// triangle tests.
+template <int WALK>
+__global__ __launch_bounds__(64) void k_bvh_visualizer(dev::RScene scene, uint32_t width, uint32_t height,
+                                                       clr_float3* __restrict__ pixels) {
+    __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+    dev::RScene S = scene;
+    dev::finishScene(S);
+    const uint32_t tilesX = (width + kTile - 1) / kTile;
+    const uint32_t gx = (blockIdx.x % tilesX) * kTile + (threadIdx.x % kTile);
+    const uint32_t gy = (blockIdx.x / tilesX) * kTile + (threadIdx.x / kTile);
+    if (gx >= width || gy >= height) return;
+    dev::V3 org, dir;
+    dev::cameraRayAt(S, 0.5f, 0.5f, gx + 0.5f, gy + 0.5f, org, dir);
+    float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
+    dev::TraceStats st{0u, 0u};
+    bool over = false;
+    const dev::LdsStack stk{s_stack + threadIdx.x, 64};
+    uint32_t face;
+    if constexpr (WALK == dev::kWalkOrdered)
+        face = dev::orderedHit<true, false, dev::TexAlpha>(S.g, org, dir, t, stk, st, &b1, &b2, dev::TexAlpha{&S},
+                                                          &over);
+    else if constexpr (WALK == dev::kWalkMidpoint)
+        face = dev::bvhWalk<true, false>(S.g, org, dir, t, stk, st, &b1, &b2, dev::TexAlpha{&S}, &over);
+    else
+        face = dev::lbvhClosest<true, dev::TexAlpha>(S.g, org, dir, t, stk, st, &b1, &b2, dev::TexAlpha{&S}, &over);
+    if (over) atomicOr(S.status, kStatusStackOverflow);
+    const size_t pi = (size_t)gy * width + gx;
+    float4 p = reinterpret_cast<float4*>(pixels)[pi];
+    if (face != dev::kNone && !isinf(t)) {
+        dev::SurfPt sp;
+        dev::hitParams(S, org, dir, t, face, b1, b2, sp);
+        dev::BSDF B;
+        dev::bsdfAlloc(S, S.g.faces[face].matPtr, sp, B);
+        const dev::V3 c = dev::baseColor(B);
+        p.x += c.x;
+        p.y += c.y;
+        p.z += c.z;
+    }
+    const float k = (float)(st.visits + st.tests) * 0.025f;
+    p.x += k;
+    p.y += k;
+    p.z += k;
+    reinterpret_cast<float4*>(pixels)[pi] = p;
+}
+
 // Adjoint of k_render_mega: the same pixels, samples and random states, each
 // sample replayed with the adjoint hook; block sums of the gradients go to
 // global memory once per block.
@@ -455,6 +506,21 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
         k_render_mega<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
                                                                 rp.maxDepth, noRR, states, pixels);
     return hipCheck(ctx, hipGetLastError(), "render launch");
+}
+
+int renderBvhVisualizer(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, clr_float3* pixels,
+                        hipStream_t stream) {
+    dev::RScene S;
+    int rc = prepareRenderScene(ctx, sd, S, stream);
+    if (rc) return rc;
+    const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
+    if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
+        k_bvh_visualizer<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, pixels);
+    else if (sd.bvhNodes)
+        k_bvh_visualizer<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, pixels);
+    else
+        k_bvh_visualizer<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, pixels);
+    return hipCheck(ctx, hipGetLastError(), "visualizer launch");
 }
 
 int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, const clr_diff_params& dp,

@@ -953,5 +953,39 @@ __device__ __forceinline__ void cameraRay(const RScene& Sc, uint32_t gx, uint32_
     weight = fabsf(dot(idfDir, dir)) / dirPDF;
 }
 
+// Primary ray from given samples: lens sample (c0, c1) and raster position
+// (px, py) (sampleLensPos and sample_We, camera.cl:60-113).
+__device__ __forceinline__ void cameraRayAt(const RScene& Sc, float c0, float c1, float px, float py, V3& org,
+                                            V3& dir) {
+    const uint8_t* cam = Sc.camera;
+    const uint8_t* l2w = cam + 64;
+    const float lensR = ld<float>(cam + 136), objPD = ld<float>(cam + 140);
+    float ux, uy;
+    concentricDisk(c0, c1, ux, uy);
+    const V3 lensLocal = v3(ux * lensR, uy * lensR, 0.0f);
+    org = mulPoint(l2w, lensLocal);
+    V3 d = normalize3(mulRaster(cam + 192, px, py));
+    if (lensR > 0.0f) {
+        const float ft = -objPD / d.z;
+        const V3 pf = v3z() + ft * d;
+        d = normalize3(pf - v3(lensLocal.x, lensLocal.y, 0.0f));
+    }
+    dir = mulVec(l2w, d);
+}
+
+// getBaseColor_fs (reflection.cl:730-772): the sum over the BSDF's elements
+// of R (diffuse, specular reflection, Ward), Rs (Ashikhmin specular) and Rd
+// (Ashikhmin diffuse); transmission and other elements add nothing.
+__device__ __forceinline__ V3 baseColor(const BSDF& B) {
+    V3 c = v3z();
+    for (int i = 0; i < 4; ++i) {
+        if (i >= B.n) break;
+        const BxDF& f = B.c[i];
+        if (f.id == 0 || f.id == 1 || f.id == 3 || f.id == 4) c = c + f.R;
+        else if (f.id == 5) c = c + f.R2;
+    }
+    return c;
+}
+
 }  // namespace dev
 }  // namespace clr

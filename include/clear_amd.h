@@ -284,6 +284,18 @@ int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScen
                      const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
                      void* stream);
 
+/* bvh_visualizer (bvh_visualizer.cl:25-84), a traversal-cost view: for each
+ * pixel one ray through its centre from the lens centre; pixels += the base
+ * colour of the surface hit (getBaseColor_fs, reflection.cl:730-772), then
+ * += 0.025 * the ray's box hits. With params->flags &
+ * CLR_RENDER_REFERENCE_TRAVERSAL the count is the reference's
+ * (rayIntersectionVis, bvh_traversal.cl:216-319: internal and leaf boxes
+ * passed on the LBVH, node boxes passed on scene->bvhNodes); otherwise it is
+ * the ordered traversal's entered nodes plus triangle tests. Uses
+ * params->width / height / flags; samplesPerPixel and maxDepth are ignored. */
+int clr_bvh_visualizer(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params* params,
+                       clr_float3* pixels, void* stream);
+
 /* ---- post-processing (post_processing.cl, BMPExporter.cpp) ----------------
  * Device buffers of clr_float3 per pixel (row-major), stream-ordered. src and
  * output must not overlap. The reference's gaussianScattering and bloom add

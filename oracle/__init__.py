@@ -326,3 +326,29 @@ def post_tone(src, row_bytes):
     dst = np.full(h * row_bytes, 0xAB, np.uint8)
     _post("orc_post_tone", ctypes.c_uint32(w), ctypes.c_uint32(h), ctypes.c_uint32(row_bytes), _p(src), _p(dst))
     return dst
+
+
+def bvh_visualizer(arrays, num_faces, inodes, leaves, width, height, flags=0, bvh=None):
+    """Visualizer restatement: float32 [H*W*4] (added onto zeros)."""
+    L = lib()
+    fn = L.orc_bvh_visualizer
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.c_void_p]
+    keep, ptrs = _scene_ptrs(arrays, num_faces, inodes, leaves, bvh)
+    px = np.zeros(width * height * 4, np.float32)
+    rc = fn(ptrs, num_faces, width, height, flags, _p(px))
+    if rc:
+        raise RuntimeError(f"orc_bvh_visualizer failed: {rc}")
+    return px
+
+
+def centre_rays(arrays, width, height):
+    """The visualizer's rays (lens centre through pixel centres), float32 [7, W*H]."""
+    L = lib()
+    fn = L.orc_centre_rays
+    fn.restype = ctypes.c_int
+    keep, ptrs = _scene_ptrs(arrays, 1, None, None, None)
+    rays = np.zeros((7, width * height), np.float32)
+    fn(ptrs, ctypes.c_uint32(width), ctypes.c_uint32(height), _p(rays))
+    return rays

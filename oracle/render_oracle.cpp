@@ -424,7 +424,18 @@ void hitParams(const SceneC& s, V3 org, V3 dir, float t, const Isect& is, Surfac
     }
 }
 // rayIntersection (bvh_traversal.cl:182-215), or the ordered traversal.
-bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
+// boxHits (optional): rayIntersectionVis's count (bvh_traversal.cl:216-319:
+// internal and leaf boxes passed on the LBVH, node boxes passed on the
+// midpoint BVH), or the ordered traversal's entered nodes plus tests.
+bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp, uint32_t* boxHits = nullptr) {
+    uint32_t hits = 0;
+    struct Commit {
+        uint32_t* out;
+        uint32_t& n;
+        ~Commit() {
+            if (out) *out = n;
+        }
+    } commit{boxHits, hits};
     float t = INFINITY;
     Isect best{kNoFace, 0, 0}, cur{};
     if (s.ordered) {
@@ -438,6 +449,7 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
         };
         orc_ordered::Counts cnt;
         const uint32_t f = orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr);
+        hits = cnt.visits + cnt.tests;
         const bool hit = f != kNoFace && !std::isinf(t);
         if (hit) {
             // The accepted triangle's barycentrics: the same test again.
@@ -452,6 +464,7 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
     auto tryLeaf = [&](uint32_t li) {
         const clr_lbvh_leaf& L = s.leaves[li];
         if (!boxTest(L.min, L.max, org, dir, t)) return;
+        ++hits;
         if (triTest(s, org, dir, L.objIdx, &t, &cur)) best = cur;
     };
     if (s.bvh) {
@@ -462,6 +475,7 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
         while (depth > 0) {
             const clr_bvh_node& N = s.bvh[stack[--depth]];
             if (!boxTest(N.min, N.max, org, dir, t)) continue;
+            ++hits;
             if (N.children[1] != 0xFFFFFFFFu) {
                 if (depth + 2 <= 64) {
                     stack[depth++] = N.children[1];
@@ -480,6 +494,7 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp) {
         while (depth > 0) {
             const clr_lbvh_inode& N = s.inodes[stack[--depth]];
             if (!boxTest(N.min, N.max, org, dir, t)) continue;
+            ++hits;
             for (int i = 0; i < 2; ++i) {
                 if (!N.isChild[i]) {
                     if (depth < 64) stack[depth++] = N.c[i];
@@ -1355,6 +1370,100 @@ int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t wi
         else if (k < 4 * numMat) gSig[k - 3 * numMat] = (float)v;
         else if (gEm) gEm[k - 4 * numMat] = (float)v;
     }
+    return 0;
+}
+
+// The ray from the lens centre (lens sample 0.5, 0.5) through the centre of
+// pixel (gx, gy) (sampleLensPos, sample_We; camera.cl:60-113).
+void centreRay(const SceneC& s, uint32_t gx, uint32_t gy, V3* org, V3* rdir) {
+    const uint8_t* cam = s.camera;
+    const uint8_t* l2w = cam + 64;
+    const float lensRadius = rd<float>(cam + 136), objPD = rd<float>(cam + 140);
+    float ux, uy;
+    concentricSampleDisk(0.5f, 0.5f, &ux, &uy);
+    const V3 lensLocal = mk(ux * lensRadius, uy * lensRadius, 0.0f);
+    *org = mulP3(l2w, lensLocal);
+    V3 dir = normalize(mulP4xyz(cam + 192, gx + 0.5f, gy + 0.5f));
+    if (lensRadius > 0.0f) {
+        const float ft = -objPD / dir.z;
+        const V3 pf = mk(0, 0, 0) + ft * dir;
+        dir = normalize(pf - mk(lensLocal.x, lensLocal.y, 0.0f));
+    }
+    *rdir = mulV3(l2w, dir);
+}
+
+// Those rays for every pixel as float[7][W * H] SoA planes (tmax = inf), so
+// tests can trace them with the traversal restatement.
+int orc_centre_rays(const void* const* arrays, uint32_t width, uint32_t height, float* rays) {
+    SceneC s{};
+    s.others = (const uint8_t*)arrays[8];
+    s.camera = s.others + rd<uint32_t>(s.others + 0);
+    const size_t R = (size_t)width * height;
+    for (uint32_t gy = 0; gy < height; ++gy)
+        for (uint32_t gx = 0; gx < width; ++gx) {
+            V3 o, d;
+            centreRay(s, gx, gy, &o, &d);
+            const size_t i = (size_t)gy * width + gx;
+            rays[i] = o.x;
+            rays[R + i] = o.y;
+            rays[2 * R + i] = o.z;
+            rays[3 * R + i] = d.x;
+            rays[4 * R + i] = d.y;
+            rays[5 * R + i] = d.z;
+            rays[6 * R + i] = INFINITY;
+        }
+    return 0;
+}
+
+// bvh_visualizer (bvh_visualizer.cl:25-84) over every pixel: the ray from
+// the lens centre (sample 0.5, 0.5) through the pixel centre; pixel += the
+// base colour (getBaseColor_fs, reflection.cl:730-772) of the surface hit,
+// then += 0.025 * box hits (rayIntersection's count).
+int orc_bvh_visualizer(const void* const* arrays, uint32_t numFaces, uint32_t width, uint32_t height,
+                       uint32_t flags, clr_float3* pixels) {
+    SceneC s{};
+    s.ordered = (flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
+    s.vertices = (const clr_float3*)arrays[0];
+    s.normals = (const clr_float3*)arrays[1];
+    s.tangents = (const clr_float3*)arrays[2];
+    s.uvs = (const float*)arrays[3];
+    s.faces = (const clr_face*)arrays[4];
+    s.lights = (const clr_light_info*)arrays[5];
+    s.materials = (const uint8_t*)arrays[6];
+    s.textures = (const uint8_t*)arrays[7];
+    s.others = (const uint8_t*)arrays[8];
+    s.inodes = (const clr_lbvh_inode*)arrays[9];
+    s.leaves = (const clr_lbvh_leaf*)arrays[10];
+    s.bvh = (const clr_bvh_node*)arrays[11];
+    s.numFaces = numFaces;
+    s.camera = s.others + rd<uint32_t>(s.others + 0);
+    s.environment = s.others + rd<uint32_t>(s.others + 4);
+    s.lightDist = s.others + rd<uint32_t>(s.others + 8);
+    for (uint32_t gy = 0; gy < height; ++gy)
+        for (uint32_t gx = 0; gx < width; ++gx) {
+            V3 org, rdir;
+            centreRay(s, gx, gy, &org, &rdir);
+            SurfacePoint sp;
+            uint32_t hits = 0;
+            clr_float3& p = pixels[(size_t)gy * width + gx];
+            if (rayIntersection(s, org, rdir, &sp, &hits)) {
+                BSDF B;
+                bsdfAlloc(s, s.faces[sp.faceID].matPtr, sp, &B);
+                V3 c = mk(0, 0, 0);
+                for (int i = 0; i < B.n; ++i) {
+                    const BxDF& f = B.c[i];
+                    if (f.id == 0 || f.id == 1 || f.id == 3 || f.id == 4) c = c + f.R;
+                    else if (f.id == 5) c = c + f.Rs;  // Ashikhmin diffuse: Rd
+                }
+                p.x += c.x;
+                p.y += c.y;
+                p.z += c.z;
+            }
+            const float k = (float)hits * 0.025f;
+            p.x += k;
+            p.y += k;
+            p.z += k;
+        }
     return 0;
 }
 

@@ -228,3 +228,29 @@ def test_converged_4096spp_rmse(gpu_ctx):
         # the bar is absolute RMSE <= 1e-3; this image is dim, so also 1e-3 of its mean
         assert np.isfinite(img).all() and rmse <= 1e-3 and rmse <= 1e-3 * float(ref.mean())
     assert gpu_ctx.device_status() == 0
+
+
+@pytest.mark.parametrize("walk", ["lbvh", "midpoint", "ordered"])
+@pytest.mark.parametrize("kind", [ca.SCENE_CORNELL, ca.SCENE_SHOWCASE])
+def test_bvh_visualizer(gpu_ctx, walk, kind):
+    """clr_bvh_visualizer against the oracle: base colours and box-hit counts,
+    equal on >= 99.9 % of pixels (within 1e-6 relative)."""
+    import torch
+
+    W, H = 64, 48
+    host = ca.HostScene(kind, W, H, cpu_bvh=True)
+    ds = ca.DeviceScene(gpu_ctx, host, accel="midpoint" if walk == "midpoint" else "lbvh")
+    flags = 0 if walk == "ordered" else ca.RENDER_REFERENCE_TRAVERSAL
+    px = torch.zeros(W * H * 4, dtype=torch.float32, device="cuda")
+    ca.bvh_visualizer(gpu_ctx, ds, px, flags=flags)
+    torch.cuda.synchronize()
+    assert gpu_ctx.device_status() == 0
+    o = orc.lbvh_build(host.vertices, host.faces, 10)
+    nodes = host.arrays["bvh_nodes"][: (2 * host.num_faces - 1) * 48]
+    ref = orc.bvh_visualizer(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, flags,
+                             bvh=nodes if walk == "midpoint" else None).reshape(-1, 4)[:, :3]
+    got = px.cpu().numpy().reshape(-1, 4)[:, :3]
+    ok = (np.abs(got - ref) <= 1e-6 * np.abs(ref) + 1e-7).all(axis=1)
+    print(f"[render parity] visualizer kind={kind} {walk}: equal pixels {ok.mean():.5f}, "
+          f"mean {ref.mean():.4f}", flush=True)
+    assert ok.mean() >= 0.999

@@ -122,3 +122,35 @@ def test_ordered_visits_fewer_nodes_on_clusters():
     _, _, st_ref = orc.trace_closest(o["inodes"], o["leaves"], 20000, v, f, rays, stats=True)
     _, _, st_ord = orc.trace_ordered(o["inodes"], o["leaves"], 20000, v, f, rays)
     assert st_ord[:, 1].sum() < st_ref[:, 1].sum()  # fewer triangle tests
+
+
+# ---- BVH visualizer (orc_bvh_visualizer) ---------------------------------------
+
+@pytest.mark.parametrize("walk", ["lbvh", "midpoint", "ordered"])
+def test_bvh_visualizer_counts_and_base_colours(walk):
+    """pixel = base colour of the surface hit + 0.025 * box hits, where the box
+    hits equal the traversal restatement's counts for the same ray (visits +
+    tests of orc_trace_closest / orc_trace_bvh / orc_trace_ordered) and the base
+    colour is one of the Cornell box's materials (0 on a miss)."""
+    W, H = 40, 32
+    host = ca.HostScene(ca.SCENE_CORNELL, W, H, cpu_bvh=True)
+    o = orc.lbvh_build(host.vertices, host.faces, 10)
+    nodes = host.arrays["bvh_nodes"][: (2 * host.num_faces - 1) * 48].view(ca.BVH_NODE_DTYPE)
+    flags = 0 if walk == "ordered" else ca.RENDER_REFERENCE_TRAVERSAL
+    px = orc.bvh_visualizer(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, flags,
+                            bvh=nodes if walk == "midpoint" else None).reshape(-1, 4)[:, :3]
+    rays = orc.centre_rays(host.arrays, W, H)
+    if walk == "lbvh":
+        hit, _, st = orc.trace_closest(o["inodes"], o["leaves"], host.num_faces, host.vertices, host.faces, rays,
+                                       stats=True)
+    elif walk == "midpoint":
+        hit, _, st = orc.trace_bvh(nodes, host.vertices, host.faces, rays)
+    else:
+        hit, _, st = orc.trace_ordered(o["inodes"], o["leaves"], host.num_faces, host.vertices, host.faces, rays)
+    k = st.sum(axis=1).astype(np.float32) * np.float32(0.025)
+    assert (st.sum(axis=1) > 0).all()
+    base = px - k[:, None]
+    palette = np.array([[0.75, 0.75, 0.75], [0.6, 0, 0], [0, 0.6, 0]], np.float32)
+    d = np.abs(base[:, None, :] - palette[None]).max(axis=2).min(axis=1)
+    assert (d[hit != 0xFFFFFFFF] < 1e-5).all()
+    assert (np.abs(base[hit == 0xFFFFFFFF]) < 1e-6).all()
