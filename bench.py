@@ -163,7 +163,7 @@ def render_leg(ca, np, torch, ctx, args):
                        f"max depth {depth}, thin lens 0.02", "triangles": host.num_faces,
            "spp": spp}
     tune_t = time.perf_counter()
-    cfg = ca.wavefront_autotune(ctx, ds, 1, depth)
+    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 1, depth)
     out["wavefront_config"] = cfg.as_dict()
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     legs = [("megakernel_plain", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL),
@@ -271,7 +271,7 @@ def batch_leg(ca, np, torch, ctx, args):
 
     saved = ca.wavefront_config(ctx)
     tune_t = time.perf_counter()
-    cfg = ca.wavefront_autotune_batch(ctx, batch, 4, depth)
+    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune_batch(ctx, batch, 4, depth)
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     out["wavefront_config"] = cfg.as_dict()
     seed_all()
@@ -361,6 +361,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="render legs use the default wavefront configuration (for kernel profiles)")
     ap.add_argument("--batch-per-scene", type=int, default=32,
                     help="scenes rendered one by one for the batch comparison (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,

@@ -177,10 +177,15 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
                 hi[j % 3] = omax(hi[j % 3], v[j]);
             }
         }
+        // The tail (fewer than four vertices) vertex by vertex: constant
+        // indices keep lo / hi in registers.
         if (blockIdx.x == 0 && threadIdx.x == 0)
-            for (size_t f = groups * 12; f < floats; ++f) {
-                lo[f % 3] = omin(lo[f % 3], tris[f]);
-                hi[f % 3] = omax(hi[f % 3], tris[f]);
+            for (size_t f = groups * 12; f < floats; f += 3) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = omin(lo[k], tris[f + k]);
+                    hi[k] = omax(hi[k], tris[f + k]);
+                }
             }
     }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; !vec && i < n; i += gridDim.x * kBlock) {
