@@ -151,8 +151,8 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
         dev::V3 d = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
         float t = rays[6 * R + r];
         dev::TraceStats st{0u, 0u};
-        const uint32_t best = dev::orderedHit<STATS, false>(g, o, d, t, stk, st, nullptr, nullptr, dev::NoAlpha(),
-                                                            &over);
+        const uint32_t best = dev::orderedHit<STATS, false, dev::NoAlpha, dev::SpillStack<kOrdLds>, true>(
+            g, o, d, t, stk, st, nullptr, nullptr, dev::NoAlpha(), &over);
         const bool hit = best != kEmpty && !isinf(t);
         hitFace[r] = hit ? best : kEmpty;
         hitT[r] = t;
@@ -219,7 +219,7 @@ int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     // 16 waves per CU (LDS: 16 KiB per 128-thread block).
     const uint32_t grid = std::max(1u, std::min((uint32_t)cus * 8u, (numRays + kOrdBlock - 1) / kOrdBlock));
     const size_t lanes = (size_t)grid * kOrdBlock;
-    rc = ensureTraceSpill(ctx, lanes * (dev::kStackDepth - kOrdLds) * sizeof(uint32_t));
+    rc = ensureTraceSpill(ctx, lanes * (dev::kStackPairs - kOrdLds) * sizeof(uint32_t));
     if (rc) return rc;
     const float4 *anodes = nullptr, *atris = nullptr;
     rc = buildAccel(ctx, inodes, leaves, bvh, n, vertices, faces, &anodes, &atris, stream);

@@ -406,7 +406,15 @@ __device__ __forceinline__ bool triTestPacked(const float4* tris, V3 o, V3 d, ui
 // callers can interleave rays (refill idle lanes) without changing results.
 // any (set by begin): stop at the first accepted triangle with a finite
 // distance (occlusion queries).
-template <bool STATS, class Alpha, class Stack>
+// PAIRS (for stacks with a spill region of kStackPairs slots): the farther
+// child is pushed with its entry distance, and a visit that enters no child
+// pops at once, entering the first candidate whose entry is still <= t. That
+// is the same decision as testing its box again (the entry does not depend on
+// t), without the box fetch and without a separate pop step, so hits and
+// visit / test counts are unchanged.
+constexpr int kStackPairs = 128;  // stack slots of a PAIRS walk: 64 (ref, entry) pairs
+
+template <bool STATS, class Alpha, class Stack, bool PAIRS = false>
 struct Ordered {
     GeomScene g;  // by value: rebind() switches scenes between rays
     const Stack& stack;
@@ -494,15 +502,39 @@ struct Ordered {
             }
             const bool go0 = !l0 && e0 <= t, go1 = !l1 && e1 <= t;
             if (go0 && go1) {
-                if (depth < kStackDepth) stack.put(depth++, first1 ? r0 : r1);
-                else over = true;
+                if constexpr (PAIRS) {
+                    if (depth + 2 <= kStackPairs) {
+                        stack.put(depth, first1 ? r0 : r1);
+                        stack.put(depth + 1, __float_as_uint(first1 ? e0 : e1));
+                        depth += 2;
+                    } else {
+                        over = true;
+                    }
+                } else {
+                    if (depth < kStackDepth) stack.put(depth++, first1 ? r0 : r1);
+                    else over = true;
+                }
                 node = first1 ? r1 : r0;
             } else if (go0 || go1) {
                 node = go0 ? r0 : r1;
             } else {
                 node = kNone;
-                if (depth == 0) done = true;
+                if constexpr (PAIRS) {
+                    while (depth > 0) {
+                        depth -= 2;
+                        const uint32_t cand = stack.get(depth);
+                        if (__uint_as_float(stack.get(depth + 1)) <= t) {
+                            node = cand;
+                            break;
+                        }
+                    }
+                }
+                if (node == kNone && depth == 0) done = true;
             }
+            return;
+        }
+        if constexpr (PAIRS) {  // not reached: PAIRS visits pop inline
+            done = true;
             return;
         }
         // The candidate's own box: first 32 bytes of its 48-byte record (LBVH
@@ -516,11 +548,11 @@ struct Ordered {
 
 // The ordered walk run to completion for one ray. Returns the face (kNone
 // when nothing was accepted) and updates t, b1, b2.
-template <bool STATS, bool ANY, class Alpha = NoAlpha, class Stack = LdsStack>
+template <bool STATS, bool ANY, class Alpha = NoAlpha, class Stack = LdsStack, bool PAIRS = false>
 __device__ __forceinline__ uint32_t orderedHit(const GeomScene& g, V3 o, V3 d, float& t, const Stack& stack,
                                                TraceStats& st, float* b1Out = nullptr, float* b2Out = nullptr,
                                                const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
-    Ordered<STATS, Alpha, Stack> w(g, stack, alpha);
+    Ordered<STATS, Alpha, Stack, PAIRS> w(g, stack, alpha);
     w.begin(o, d, t, ANY);
     while (!w.done) w.step();
     t = w.t;

@@ -71,7 +71,7 @@ struct WfBufs {
     float4* sqDir;
     float4* sqC;     // weighted contribution
     uint32_t* counts;
-    uint32_t* spill; // traversal stack overflow region, (64 - kMinLdsDepth) x lanes
+    uint32_t* spill; // traversal stack overflow region, (kStackPairs - kMinLdsDepth) x lanes
     uint32_t lanes;
     uint32_t slots;  // P: capacity of every queue
 };
@@ -121,7 +121,7 @@ size_t wfBytes(uint32_t P, uint32_t lanes) {
     add(16ull * P); add(4ull * P); add(4ull * P);
     add(4ull * P); add(16ull * P); add(16ull * P); add(16ull * P);
     add(256);
-    add(4ull * (kStackDepth - kMinLdsDepth) * lanes);
+    add(4ull * (kStackPairs - kMinLdsDepth) * lanes);
     return b;
 }
 
@@ -146,7 +146,7 @@ WfBufs carveWf(void* pool, uint32_t P, uint32_t lanes) {
     B.sqDir = c.take<float4>(P);
     B.sqC = c.take<float4>(P);
     B.counts = c.take<uint32_t>(64);
-    B.spill = c.take<uint32_t>((size_t)(kStackDepth - kMinLdsDepth) * lanes);
+    B.spill = c.take<uint32_t>((size_t)(kStackPairs - kMinLdsDepth) * lanes);
     B.lanes = lanes;
     B.slots = P;
     return B;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
         // Persistent waves refilling idle lanes from a wave pool; hit / miss
         // queues are built by k_wf_compact.
         const RScene* S0 = firstScene<BATCH>(A, S);
-        Ordered<false, TexAlpha, SpillStack<LD>> w(S0->g, stk, TexAlpha{S0});
+        Ordered<false, TexAlpha, SpillStack<LD>, true> w(S0->g, stk, TexAlpha{S0});
         WavePool pool(n);
         bool busy = false;
         uint32_t j = 0;
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         // occlusion walk, area samples the closest hit compared with the
         // light's face. Samples ending here are finished by k_wf_shadow_end.
         const RScene* S0 = firstScene<BATCH>(A, S);
-        Ordered<false, TexAlpha, SpillStack<LD>> w(S0->g, stk, TexAlpha{S0});
+        Ordered<false, TexAlpha, SpillStack<LD>, true> w(S0->g, stk, TexAlpha{S0});
         WavePool pool(n);
         bool busy = false;
         uint32_t i = 0, target = kNone;
