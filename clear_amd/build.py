@@ -25,7 +25,8 @@ ARCH = "gfx950"
 
 # -ffp-contract=off keeps every a*b+c as two rounded operations, as the CPU
 # oracle computes them; IEEE division/sqrt are requested explicitly.
-# Packed FP32 instructions are disabled ("Compiler flags").
+# Packed FP32 instructions (v_pk_*_f32) are disabled: some of their forms
+# have given wrong results on this hardware without any error.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

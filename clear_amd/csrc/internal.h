@@ -75,6 +75,7 @@ int ensurePool(clr_ctx* ctx, size_t bytes);
 constexpr uint32_t kStatusLookbackSpin = 1u << 0;
 constexpr uint32_t kStatusExchangeSpin = 1u << 1;
 constexpr uint32_t kStatusStackOverflow = 1u << 2;
+constexpr uint32_t kStatusHierarchy = 1u << 3;
 // Upper bound on any device-side wait loop (iterations of one atomic load).
 constexpr uint32_t kSpinLimit = 1u << 22;
 

@@ -90,10 +90,10 @@ __device__ __forceinline__ Box boxOf3(const float* p0, const float* p1, const fl
     return b;
 }
 
-// Primitive source: a 36-B triangle soup, plus a dense box array in input
-// order (32-B records: lo.xyz, hi.x | hi.yz, 0, 0) written by the first pass
-// that sees the triangles. The hierarchy pass gathers boxes by sorted index
-// from that array: one aligned 32-B sector per leaf.
+// Primitive source: a 36-B triangle soup, plus a box array of 32-B records
+// (lo.xyz, hi.x | hi.yz, triangle index, 0) in input order, written by the
+// first pass that sees the triangles. The hierarchy pass gathers one record
+// per leaf by the sorted index.
 struct PrimSrc {
     const float* tris;  // soup (null on the drop-in path)
     float4* boxes;      // box array
@@ -107,16 +107,17 @@ __device__ __forceinline__ Box soupBox(const float* tris, uint32_t i) {
     return boxOf3(p0, p1, p2);
 }
 
-__device__ __forceinline__ void storeBoxRec(float4* boxes, uint32_t i, const Box& b) {
-    boxes[2 * (size_t)i] = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
-    boxes[2 * (size_t)i + 1] = make_float4(b.hi[1], b.hi[2], 0.0f, 0.0f);
+__device__ __forceinline__ void storeBoxRec(float4* boxes, uint32_t pos, const Box& b, uint32_t obj) {
+    boxes[2 * (size_t)pos] = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
+    boxes[2 * (size_t)pos + 1] = make_float4(b.hi[1], b.hi[2], __uint_as_float(obj), 0.0f);
 }
 
-__device__ __forceinline__ Box loadBoxRec(const float4* boxes, uint32_t i) {
-    const float4 a = boxes[2 * (size_t)i], c = boxes[2 * (size_t)i + 1];
+__device__ __forceinline__ Box loadBoxRec(const float4* boxes, uint32_t pos, uint32_t* obj = nullptr) {
+    const float4 a = boxes[2 * (size_t)pos], c = boxes[2 * (size_t)pos + 1];
     Box r;
     r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z;
     r.hi[0] = a.w; r.hi[1] = c.x; r.hi[2] = c.y;
+    if (obj) *obj = __float_as_uint(c.z);
     return r;
 }
 
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
             float p0[3] = {v0.x, v0.y, v0.z}, p1[3] = {v1.x, v1.y, v1.z},
                   p2[3] = {v2.x, v2.y, v2.z};
             b = boxOf3(p0, p1, p2);
-            storeBoxRec(boxesOut, i, b);
+            storeBoxRec(boxesOut, i, b, i);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -265,6 +266,25 @@ __device__ __forceinline__ uint32_t quantize(float v, uint32_t numDivs) {
     return q < numDivs - 1u ? q : numDivs - 1u;
 }
 
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* s_tmp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; ++w) pre += s_tmp[w];
+    __syncthreads();
+    return pre + x - v;
+}
+
+// Keys, digit histograms of every pass and (soup input) the box records, in
+// input order.
 template <bool SOUP, typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint32_t bits,
                                                    const uint32_t* __restrict__ sceneOrd,
@@ -285,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint
     const float fDivs = (float)numDivs;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         Box b = loadBox<SOUP>(src, i);
-        if constexpr (SOUP) storeBoxRec(src.boxes, i, b);
+        if constexpr (SOUP) storeBoxRec(src.boxes, i, b, i);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -305,23 +325,6 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint
 }
 
 // ------------------------------------------------------------ onesweep ----
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* s_tmp) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) s_tmp[wave] = x;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (int w = 0; w < wave; ++w) pre += s_tmp[w];
-    __syncthreads();
-    return pre + x - v;
-}
-
 __device__ __forceinline__ void writeLeaf(clr_lbvh_leaf* leaves, uint32_t i, const Box& b,
                                           uint32_t obj) {
     uint4* o = reinterpret_cast<uint4*>(leaves + i);
@@ -514,6 +517,28 @@ __device__ __forceinline__ int deltaAt(const KeyT* __restrict__ k, int64_t x, ui
     }
 }
 
+// deltaAt with unconditional loads (an index outside [0, n - 1) reads
+// position 0 and the result is -1), so that a thread's loads for several x
+// can be in flight together.
+template <typename KeyT, bool SEG>
+__device__ __forceinline__ int deltaAtLoaded(const KeyT* __restrict__ k, int64_t x, uint32_t n, int keyBits,
+                                             const SegView& sv) {
+    const bool valid = x >= 0 && x + 1 < (int64_t)n;
+    const uint32_t x0 = valid ? (uint32_t)x : 0u;
+    const uint32_t x1 = valid ? x0 + 1u : 0u;
+    const KeyT a = k[x0], b = k[x1];
+    uint32_t base = 0;
+    bool cross = false;
+    if constexpr (SEG) {
+        const uint32_t s0 = sv.segOf[x0], s1 = sv.segOf[x1];
+        cross = s0 != s1;
+        base = sv.segOff[s0];
+    }
+    const int dk = clzKey<KeyT>(a ^ b) - (int)(8 * sizeof(KeyT)) + keyBits;
+    const int di = keyBits + __clz((int)((x0 - base) ^ (x1 - base)));
+    return (!valid || cross) ? -1 : (a != b ? dk : di);
+}
+
 // Node record; children are split - base and split + 1 - base (scene-local in
 // a segmented build).
 __device__ __forceinline__ void writeNode(clr_lbvh_inode* nodes, uint32_t idx, const Box& b,
@@ -589,165 +614,271 @@ __device__ __forceinline__ Box awaitBox(const uint64_t* slot, uint32_t epoch,
 // Bottom-up topology + refit in two phases.
 //
 // Phase 1 (k_hier_local): block b owns leaves [bs, be) (kLocalLeaves per
-// block) and completes every node whose range lies inside them: arrival slots
-// for splits bs..be-2, leaf boxes, internal boxes and boundary deltas all live
-// in LDS, and the hand-off is a workgroup-scope exchange (s_waitcnt around
-// ds_wrxchg, no cache maintenance). A climb whose parent splits at a block
-// boundary (bs-1 or be-1) stops and leaves a pending record (at most two per
-// block). After a barrier the block exports its half-filled slots (one child
-// arrived) to the global arrival words, publishing that child's box.
+// block, kHierThreads threads) and completes every node whose range lies
+// inside them. Arrival slots for splits bs..be-2, boxes and boundary deltas
+// live in LDS. The climb runs in synchronous steps over a queue of climbers
+// (a climber holds a completed node [l, r]; at first every leaf): in a step
+// each climber finds its parent (split r if delta(r) > delta(l-1), else l-1),
+// exchanges its far end into the parent's slot, and the second arrival
+// completes the parent (box = left child's box joined with the right's),
+// writes its record at its index (r if the parent is itself a left child,
+// else l; the root of the range at 0) and enqueues it. The queue is compacted
+// every step, so lanes are spent on live climbs only (a lane per leaf would
+// keep each wave stepping as long as its longest chain). Boxes: slot k holds
+// leaf k's box until internal node k is completed (leaf k lies inside node k
+// and was joined into its parent before). A climb whose parent splits at a
+// block boundary (bs-1 or be-1) stops and leaves a pending record (at most two
+// per block); after the last step the block exports its half-filled slots
+// (one child arrived) to the global arrival words, publishing that child's box.
 // Phase 2 (k_hier_global): one thread per pending record continues with the
 // global epoch-tagged protocol. Only nodes whose range crosses a block
 // boundary (a small fraction) are finished there.
 constexpr int kLocalLeaves = 1024;
+constexpr int kHierThreads = 256;
+constexpr int kHierPer = kLocalLeaves / kHierThreads;
+static_assert(kLocalLeaves % kHierThreads == 0 && kLocalLeaves <= 65536, "hierarchy block shape");
 constexpr uint32_t kDone = 0xFFFFFFFEu;
 
 struct Pending {
-    uint32_t valid, l, r, split, internal;
+    uint32_t valid, l, r, split, internal;  // split: unused (a pending node is never a root)
     int dl, dr;
     float box[6];
     uint32_t pad[3];
 };
 static_assert(sizeof(Pending) == 64, "pending record layout");
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup g runs on
+// XCD g % 8). This numbering gives each XCD one contiguous range of blocks,
+// so the records and nodes of neighbouring leaves meet in one L2.
+__device__ __forceinline__ uint32_t xcdContiguous(uint32_t g, uint32_t G) {
+    constexpr uint32_t X = 8;
+    const uint32_t x = g % X, k = g / X, q = G / X, r = G % X;
+    return x * q + min(x, r) + k;
+}
+
 // SEG: segmented build (sv describes the scenes): deltas are -1 across
 // scenes, a scene's root is the node covering its whole range (stored at its
 // first inode slot), inodes of scene s are numbered locally and stored at
 // segOff[s] - s + local, children and leaf object indices are scene-local.
+// A range [l, r] is a scene's root exactly when delta(l-1) and delta(r) are
+// both -1.
 template <bool SOUP, typename KeyT, bool SEG = false>
-__global__ __launch_bounds__(kLocalLeaves) void k_hier_local(
+__global__ __launch_bounds__(kHierThreads) void k_hier_local(
     PrimSrc src, const KeyT* __restrict__ skeys, const uint32_t* __restrict__ sidx, uint32_t n,
     int keyBits, uint32_t* __restrict__ otherEnd, uint64_t* __restrict__ xchg, uint32_t epoch,
     clr_lbvh_inode* __restrict__ inodes, clr_lbvh_leaf* __restrict__ leaves,
-    Pending* __restrict__ pend, SegView sv = SegView{nullptr, nullptr}) {
+    Pending* __restrict__ pend, uint32_t* __restrict__ devStatus, SegView sv = SegView{nullptr, nullptr}) {
     constexpr int B = kLocalLeaves;
     __shared__ uint32_t s_other[B];
-    __shared__ float s_lbox[6][B];
-    __shared__ float s_ibox[6][B];
+    __shared__ float s_box[6][B];
     __shared__ int s_delta[B + 1];  // delta(x) for x = bs-1 .. be-1 at [x - bs + 1]
+    __shared__ uint32_t s_q[2][B];  // climbers: l | r << 16 (block-local)
+    __shared__ uint32_t s_aux[B];   // node completed here at slot j: split - bs | flags << 16
+    __shared__ uint32_t s_qc[3];
+    __shared__ uint32_t s_seg[SEG ? B : 1];
 
     const uint32_t t = threadIdx.x;
-    const uint32_t bs = blockIdx.x * (uint32_t)B;
+    const uint32_t blk = xcdContiguous(blockIdx.x, gridDim.x);
+    const uint32_t bs = blk * (uint32_t)B;
     const uint32_t be = min(bs + (uint32_t)B, n);
-    const uint32_t i = bs + t;
-    s_other[t] = kEmpty;
-    uint32_t seg = 0, segS = 0, segE = n;
-    if (SEG && i < be) {
-        seg = sv.segOf[i];
-        segS = sv.segOff[seg];
-        segE = sv.segOff[seg + 1];
-    }
-    if (i < be) {
-        // Leaf i: gather the sorted primitive, write its LeafNode record.
-        const uint32_t obj = sidx[i];
-        const Box b = loadBoxRec(src.boxes, obj);
-        writeLeaf(leaves, i, b, obj - segS);
+
+    // Leaves: gather each sorted primitive's record, write its LeafNode. The
+    // loads are unconditional (indices clamped into the block) so that every
+    // gather of a thread is in flight at once.
+    uint32_t rp[kHierPer];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            s_lbox[k][t] = b.lo[k];
-            s_lbox[3 + k][t] = b.hi[k];
+    for (int k = 0; k < kHierPer; ++k) {
+        const uint32_t j = (uint32_t)k * kHierThreads + t;
+        s_other[j] = kEmpty;
+        s_aux[j] = kEmpty;
+        rp[k] = sidx[min(bs + j, be - 1u)];
+    }
+    float4 ra[kHierPer], rb[kHierPer];
+#pragma unroll
+    for (int k = 0; k < kHierPer; ++k) {
+        ra[k] = src.boxes[2 * (size_t)rp[k]];
+        rb[k] = src.boxes[2 * (size_t)rp[k] + 1];
+    }
+    uint32_t segS[kHierPer];
+#pragma unroll
+    for (int k = 0; k < kHierPer; ++k) {
+        segS[k] = 0;
+        if constexpr (SEG) {
+            const uint32_t j = (uint32_t)k * kHierThreads + t;
+            const uint32_t seg = sv.segOf[min(bs + j, be - 1u)];
+            s_seg[j] = seg;
+            segS[k] = sv.segOff[seg];
         }
     }
-    for (uint32_t j = t; j <= (uint32_t)B; j += B) {
-        const int64_t x = (int64_t)bs - 1 + j;
-        s_delta[j] = x < (int64_t)be ? deltaAt<KeyT, SEG>(skeys, x, n, keyBits, sv) : -1;
+#pragma unroll
+    for (int k = 0; k < kHierPer; ++k) {
+        const uint32_t j = (uint32_t)k * kHierThreads + t, i = bs + j;
+        if (i < be) {
+            s_box[0][j] = ra[k].x; s_box[1][j] = ra[k].y; s_box[2][j] = ra[k].z;
+            s_box[3][j] = ra[k].w; s_box[4][j] = rb[k].x; s_box[5][j] = rb[k].y;
+            s_q[0][j] = __float_as_uint(rb[k].z) - segS[k];  // the leaf's object index
+        }
     }
-    if (t < 2) pend[2 * blockIdx.x + t].valid = 0u;
+    // delta(x) for x = bs-1 .. be-1 (B + 1 values; thread 0 takes the last).
+    {
+        int dv[kHierPer + 1];
+#pragma unroll
+        for (int k = 0; k <= kHierPer; ++k) {
+            const uint32_t j = k < kHierPer ? (uint32_t)k * kHierThreads + t : (uint32_t)B;
+            const int64_t x = (int64_t)bs - 1 + j;
+            dv[k] = deltaAtLoaded<KeyT, SEG>(skeys, x, n, keyBits, sv);
+        }
+#pragma unroll
+        for (int k = 0; k < kHierPer; ++k) s_delta[(uint32_t)k * kHierThreads + t] = dv[k];
+        if (t == 0) s_delta[B] = dv[kHierPer];
+    }
+    if (t < 2) pend[2 * blk + t].valid = 0u;
+    if (t < 3) s_qc[t] = 0u;
+    __syncthreads();
+    // Leaf records [bs, be) as consecutive 16-B parts (whole lines per store).
+    const uint32_t nl = be - bs;
+    for (uint32_t q = t; q < 3u * nl; q += kHierThreads) {
+        const uint32_t j = q / 3u, part = q - 3u * j;
+        uint4 v;
+        if (part == 2u)
+            v = make_uint4(s_q[0][j], 0u, 0u, 0u);
+        else
+            v = make_uint4(__float_as_uint(s_box[3 * part][j]), __float_as_uint(s_box[3 * part + 1][j]),
+                           __float_as_uint(s_box[3 * part + 2][j]), 0u);
+        reinterpret_cast<uint4*>(leaves + bs)[q] = v;
+    }
     __syncthreads();
 
-    // delta(x, x+1) for x in [bs-1, be-1]; x = bs-1 may be -1 (then -1).
-    auto D = [&](int64_t x) -> int { return s_delta[x - (int64_t)bs + 1]; };
-    if (i < be && (SEG || n > 1)) {
-        Box box;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            box.lo[k] = s_lbox[k][t];
-            box.hi[k] = s_lbox[3 + k][t];
+    const int lane = (int)(t & 63u);
+    const uint64_t ltMask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t cnt = be > bs ? be - bs : 0u;  // step 0: climber c is leaf c
+    for (uint32_t it = 0; cnt > 0; ++it) {
+        if (it >= (uint32_t)B) {  // a valid climb completes at least one node per step
+            if (t == 0) atomicOr(devStatus, kStatusHierarchy);
+            break;
         }
-        uint32_t l = i, r = i, split = 0;
-        int dl = D((int64_t)l - 1), dr = D(r);  // delta(l-1, l), delta(r, r+1)
-        bool internal = false;
-        while (true) {
-            if (l == segS && r == segE - 1) {
-                if (internal) writeNode(inodes, segS - seg, box, split, l, r, segS);
-                break;
+        if (t == 0) s_qc[(it + 2) % 3] = 0u;  // the counter of step it + 1's pushes
+        const uint32_t* qin = s_q[it & 1u];
+        uint32_t* qout = s_q[(it + 1u) & 1u];
+        uint32_t* qcount = &s_qc[(it + 1) % 3];
+        for (uint32_t c = t; c < cnt; c += kHierThreads) {
+            uint32_t l = c, r = c;
+            if (it > 0) {
+                const uint32_t q = qin[c];
+                l = q & 0xFFFFu;
+                r = q >> 16;
             }
-            const bool isLeft = dr > dl;
-            const uint32_t parentSplit = isLeft ? r : l - 1;
-            if (internal) {
-                const uint32_t self = isLeft ? r : l;
-                writeNode(inodes, self - seg, box, split, l, r, segS);
+            const uint32_t L = bs + l, R = bs + r;
+            const int dl = s_delta[l], dr = s_delta[r + 1];  // delta(L-1), delta(R)
+            bool push = false;
+            uint32_t packed = 0;
+            if (dl != -1 || dr != -1) {  // else a one-leaf scene: nothing above it
+                const bool isLeft = dr > dl;
+                const uint32_t ps = isLeft ? R : L - 1u;  // the parent's split
+                const uint32_t self = isLeft ? r : l;     // this node's box slot
+                if (ps < bs || ps + 1u >= be) {
+                    // Split on a block boundary: hand the climb to phase 2.
+                    Pending& P = pend[2 * blk + (isLeft ? 1 : 0)];
+                    P.l = L; P.r = R; P.split = 0u; P.internal = l != r ? 1u : 0u;
+                    P.dl = dl; P.dr = dr;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    s_ibox[k][self - bs] = box.lo[k];
-                    s_ibox[3 + k][self - bs] = box.hi[k];
+                    for (int k = 0; k < 6; ++k) P.box[k] = s_box[k][self];
+                    P.valid = 1u;
+                } else {
+                    const uint32_t other =
+                        __hip_atomic_exchange(&s_other[ps - bs], isLeft ? L : R, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (other != kEmpty && (other < bs || other >= be)) {
+                        // A third arrival: the keys were not sorted. Stop this climb.
+                        atomicOr(devStatus, kStatusHierarchy);
+                    } else if (other != kEmpty) {
+                        s_other[ps - bs] = kDone;  // both children arrived
+                        const uint32_t sib = (isLeft ? ps + 1u : ps) - bs;
+                        const uint32_t lo = isLeft ? self : sib, hi = isLeft ? sib : self;
+                        Box pb;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            pb.lo[k] = omin(s_box[k][lo], s_box[k][hi]);
+                            pb.hi[k] = omax(s_box[3 + k][lo], s_box[3 + k][hi]);
+                        }
+                        const uint32_t pL = isLeft ? L : other, pR = isLeft ? other : R;
+                        const int pdl = isLeft ? dl : s_delta[other - bs];
+                        const int pdr = isLeft ? s_delta[other - bs + 1] : dr;
+                        // The parent's slot: its index r or l, or for a scene's
+                        // root the scene's first position (stored at the scene's
+                        // first inode). Its record is written after the climb.
+                        const bool root = pdl == -1 && pdr == -1;
+                        const uint32_t pSelf = root ? pL : (pdr > pdl ? pR : pL);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            s_box[k][pSelf - bs] = pb.lo[k];
+                            s_box[3 + k][pSelf - bs] = pb.hi[k];
+                        }
+                        s_aux[pSelf - bs] = (ps - bs) | (ps == pL ? 1u << 16 : 0u) | (ps + 1u == pR ? 1u << 17 : 0u);
+                        if (!root) {
+                            push = true;
+                            packed = (pL - bs) | ((pR - bs) << 16);
+                        }
+                    }
                 }
             }
-            if (parentSplit < bs || parentSplit + 1 >= be) {
-                // Split on a block boundary: hand the climb to phase 2.
-                Pending& P = pend[2 * blockIdx.x + (isLeft ? 1 : 0)];
-                P.l = l; P.r = r; P.split = split; P.internal = internal ? 1u : 0u;
-                P.dl = dl; P.dr = dr;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    P.box[k] = box.lo[k];
-                    P.box[3 + k] = box.hi[k];
-                }
-                P.valid = 1u;
-                break;
+            // Wave-aggregated enqueue.
+            const uint64_t pm = __ballot(push);
+            if (pm) {
+                const int leader = __ffsll((unsigned long long)pm) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(qcount, (uint32_t)__popcll(pm));
+                base = __shfl(base, leader);
+                if (push) qout[base + (uint32_t)__popcll(pm & ltMask)] = packed;
             }
-            const uint32_t other =
-                __hip_atomic_exchange(&s_other[parentSplit - bs], isLeft ? l : r,
-                                      __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (other == kEmpty) break;
-            s_other[parentSplit - bs] = kDone;  // both children arrived
-            const uint32_t sIdx = (isLeft ? parentSplit + 1u : parentSplit) - bs;
-            const bool sLeaf = (sIdx + bs) == other;
-            Box sb;
+        }
+        __syncthreads();
+        cnt = *qcount;
+    }
+    // The loop ends with a barrier. Export slots where exactly one child
+    // arrived: phase 2 brings the other.
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                sb.lo[k] = sLeaf ? s_lbox[k][sIdx] : s_ibox[k][sIdx];
-                sb.hi[k] = sLeaf ? s_lbox[3 + k][sIdx] : s_ibox[3 + k][sIdx];
-            }
-            if (isLeft) {
+    for (int k = 0; k < kHierPer; ++k) {
+        const uint32_t j = (uint32_t)k * kHierThreads + t;
+        if (bs + j + 1 < be) {
+            const uint32_t v = s_other[j];
+            if (v != kEmpty && v != kDone) {
+                const uint32_t ps = bs + j;
+                otherEnd[ps] = v;
+                const bool leftArrived = v <= ps;
+                const uint32_t idx = leftArrived ? ps : ps + 1u;
+                const bool isLeaf = leftArrived ? (v == ps) : (v == ps + 1u);
+                if (!isLeaf) {
+                    const uint64_t tag = (uint64_t)epoch << 32;
+                    uint64_t* slot = xchg + 6 * (size_t)idx;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    box.lo[k] = omin(box.lo[k], sb.lo[k]);
-                    box.hi[k] = omax(box.hi[k], sb.hi[k]);
+                    for (int c = 0; c < 6; ++c) slot[c] = tag | __float_as_uint(s_box[c][idx - bs]);
                 }
-                r = other;
-                dr = D(other);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    box.lo[k] = omin(sb.lo[k], box.lo[k]);
-                    box.hi[k] = omax(sb.hi[k], box.hi[k]);
-                }
-                l = other;
-                dl = D((int64_t)other - 1);
             }
-            split = parentSplit;
-            internal = true;
         }
     }
-    __syncthreads();
-    // Export slots where exactly one child arrived: phase 2 brings the other.
-    if (bs + t + 1 < be) {
-        const uint32_t v = s_other[t];
-        if (v != kEmpty && v != kDone) {
-            const uint32_t ps = bs + t;
-            otherEnd[ps] = v;
-            const bool leftArrived = v <= ps;
-            const uint32_t idx = leftArrived ? ps : ps + 1u;
-            const bool isLeaf = leftArrived ? (v == ps) : (v == ps + 1u);
-            if (!isLeaf) {
-                const uint64_t tag = (uint64_t)epoch << 32;
-                uint64_t* slot = xchg + 6 * (size_t)idx;
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    slot[k] = tag | __float_as_uint(s_ibox[k][idx - bs]);
-            }
+    // Records of the nodes completed here, as consecutive 16-B parts. A node
+    // at slot j is stored at j minus its scene index; children are split and
+    // split + 1 relative to the scene's first position.
+    for (uint32_t q = t; q < 3u * nl; q += kHierThreads) {
+        const uint32_t j = q / 3u, part = q - 3u * j;
+        const uint32_t aux = s_aux[j];
+        if (aux == kEmpty) continue;
+        uint32_t seg = 0, segS = 0;
+        if constexpr (SEG) {
+            seg = s_seg[j];
+            segS = sv.segOff[seg];
         }
+        uint4 v;
+        if (part == 2u) {
+            const uint32_t split = bs + (aux & 0xFFFFu);
+            const uint32_t flags = ((aux >> 16) & 1u) | (((aux >> 17) & 1u) << 8);
+            v = make_uint4(flags, split - segS, split + 1u - segS, 0u);
+        } else {
+            v = make_uint4(__float_as_uint(s_box[3 * part][j]), __float_as_uint(s_box[3 * part + 1][j]),
+                           __float_as_uint(s_box[3 * part + 2][j]), 0u);
+        }
+        reinterpret_cast<uint4*>(inodes + (bs + j - seg))[part] = v;
     }
 }
 
@@ -925,8 +1056,8 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     }
     ev(3);
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
-    k_hier_local<SOUP, KeyT><<<localBlocks, kLocalLeaves, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend);
+    k_hier_local<SOUP, KeyT><<<localBlocks, kHierThreads, 0, stream>>>(
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, ctx->devStatus);
     ev(4);
     k_hier_global<KeyT><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
         kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend,
@@ -1015,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__
     const uint64_t tag = (uint64_t)ch.x << (3 * bits);
     for (uint32_t i = ch.y + threadIdx.x; i < ch.z; i += kBlock) {
         const Box b = soupBox(tris, i);
-        storeBoxRec(boxes, i, b);
+        storeBoxRec(boxes, i, b, i);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1136,8 +1267,8 @@ int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegO
     const SegView sv{S.segOf, S.segOff};
     PrimSrc src{tris, L.boxes};
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
-    k_hier_local<true, uint64_t, true><<<localBlocks, kLocalLeaves, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, sv);
+    k_hier_local<true, uint64_t, true><<<localBlocks, kHierThreads, 0, stream>>>(
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, ctx->devStatus, sv);
     k_hier_global<uint64_t, true><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
         kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend, 2 * localBlocks, sv);
     return hipCheck(ctx, hipGetLastError(), "segmented LBVH build launch");

@@ -105,7 +105,8 @@ const char* clr_last_error(const clr_ctx* ctx);
 int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes);
 /* Device-side failure flags raised by kernels since the last clear: bit 0 a
  * radix-sort look-back wait, bit 1 a hierarchy hand-off wait exceeded its
- * bound (the result is then wrong), bit 2 a traversal stack overflowed.
+ * bound (the result is then wrong), bit 2 a traversal stack overflowed, bit 3
+ * the hierarchy met a node with more than two children (unsorted keys).
  * Synchronises the device. No reference counterpart: the reference has no
  * failure detection (its atomic refit has no bound, lbvh_construction.cl:465). */
 int clr_device_status(clr_ctx* ctx, uint32_t* flags, int clear);
