@@ -42,8 +42,9 @@ namespace clr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSortBlock = 256;
+constexpr int kSortBlock = 512;
 constexpr int kWaves = kSortBlock / 64;
+static_assert(kSortBlock >= 256 && kSortBlock % 64 == 0, "a sort block owns all 256 digits");
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kFlagMask = 3u << 30;
@@ -54,11 +55,11 @@ template <typename KeyT>
 struct SortCfg;
 template <>
 struct SortCfg<uint32_t> {
-    static constexpr int kItems = 32;
+    static constexpr int kItems = 16;
 };
 template <>
 struct SortCfg<uint64_t> {
-    static constexpr int kItems = 12;
+    static constexpr int kItems = 6;
 };
 
 // OpenCL fmin/fmax with a fixed answer for +-0 ties (left operand) and NaN
@@ -160,33 +161,55 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
                                                    uint32_t* __restrict__ sceneOrd) {
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    // A 16-B aligned soup is read as float4 groups of three (12 floats: four
-    // vertices x, y, z, x, y, z, ...), so axis k of a group is always lanes k,
-    // k + 3, k + 6, k + 9; the scene box is the min / max over all vertices,
-    // which equals the union of the triangle boxes.
-    const bool vec = SOUP && (reinterpret_cast<uintptr_t>(tris) & 15u) == 0;
+    // A 16-B aligned soup is read as a flat float4 array, lane k of a wave
+    // reading float4 k (whole lines per load). Component c of float4 f holds
+    // axis (f + c) % 3; with a grid stride that is a multiple of 3, f % 3 is
+    // the same for every float4 a thread reads, so four accumulators per bound
+    // are folded into the three axes once at the end. The scene box is the
+    // min / max over all vertices, which equals the union of the triangle
+    // boxes.
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const bool vec = SOUP && (reinterpret_cast<uintptr_t>(tris) & 15u) == 0 && stride % 3 == 0;
     if (vec) {
         const float4* t4 = reinterpret_cast<const float4*>(tris);
         const size_t floats = 9 * (size_t)n;
-        const size_t groups = floats / 12;
-        for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += (size_t)gridDim.x * kBlock) {
-            const float4 a = t4[3 * g], b = t4[3 * g + 1], c = t4[3 * g + 2];
-            const float v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
-#pragma unroll
-            for (int j = 0; j < 12; ++j) {
-                lo[j % 3] = omin(lo[j % 3], v[j]);
-                hi[j % 3] = omax(hi[j % 3], v[j]);
-            }
+        const size_t nf4 = floats / 4;
+        const size_t f0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+        float l4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+        float h4[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        auto acc = [&](const float4 v) {
+            l4[0] = omin(l4[0], v.x); l4[1] = omin(l4[1], v.y); l4[2] = omin(l4[2], v.z); l4[3] = omin(l4[3], v.w);
+            h4[0] = omax(h4[0], v.x); h4[1] = omax(h4[1], v.y); h4[2] = omax(h4[2], v.z); h4[3] = omax(h4[3], v.w);
+        };
+        size_t f = f0;
+        for (; f + 3 * stride < nf4; f += 4 * stride) {
+            const float4 a = t4[f], b = t4[f + stride], c = t4[f + 2 * stride], d = t4[f + 3 * stride];
+            acc(a);
+            acc(b);
+            acc(c);
+            acc(d);
         }
-        // The tail (fewer than four vertices) vertex by vertex: constant
-        // indices keep lo / hi in registers.
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            for (size_t f = groups * 12; f < floats; f += 3) {
+        for (; f < nf4; f += stride) acc(t4[f]);
+        const uint32_t r = (uint32_t)(f0 % 3);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    lo[k] = omin(lo[k], tris[f + k]);
-                    hi[k] = omax(hi[k], tris[f + k]);
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if ((r + (uint32_t)c) % 3u == (uint32_t)ax) {
+                    lo[ax] = omin(lo[ax], l4[c]);
+                    hi[ax] = omax(hi[ax], h4[c]);
                 }
+        // The last floats % 4 floats; constant indices keep lo / hi in registers.
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (size_t q = nf4 * 4; q < floats; ++q) {
+                const float v = tris[q];
+                const uint32_t ax = (uint32_t)(q % 3);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if (ax == (uint32_t)k) {
+                        lo[k] = omin(lo[k], v);
+                        hi[k] = omax(hi[k], v);
+                    }
             }
     }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; !vec && i < n; i += gridDim.x * kBlock) {
@@ -266,8 +289,10 @@ __device__ __forceinline__ uint32_t quantize(float v, uint32_t numDivs) {
     return q < numDivs - 1u ? q : numDivs - 1u;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* s_tmp) {
+// Exclusive scan of one value per digit over threads 0..255 (digit t in
+// thread t); every thread of the block takes part in the barriers, and the
+// result of threads from 256 on is meaningless.
+__device__ __forceinline__ uint32_t digitExclusiveScan(uint32_t v, uint32_t* s_tmp) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -275,10 +300,10 @@ __device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* 
         uint32_t y = __shfl_up(x, off);
         if (lane >= off) x += y;
     }
-    if (lane == 63) s_tmp[wave] = x;
+    if (lane == 63 && wave < 4) s_tmp[wave] = x;
     __syncthreads();
     uint32_t pre = 0;
-    for (int w = 0; w < wave; ++w) pre += s_tmp[w];
+    for (int w = 0; w < wave && w < 4; ++w) pre += s_tmp[w];
     __syncthreads();
     return pre + x - v;
 }
@@ -407,9 +432,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
     __syncthreads();
 
     // Per digit: wave offsets, tile count, look-back for the tile's global
-    // offset. Thread t owns digit t.
+    // offset. Thread t < 256 owns digit t.
+    const bool owner = tid < 256;
     uint32_t tileCount = 0;
-    {
+    if (owner) {
         uint32_t sum = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
@@ -419,39 +445,39 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
         }
         tileCount = sum;
     }
+    // Decoupled look-back, kWin predecessors per round trip. The first
+    // window is requested before the tile-local scatter so that its latency
+    // overlaps that work. Earlier partitions took their ids before this one,
+    // so they are resident and will publish; the bound only guards against a
+    // bug.
+    constexpr int kWin = 8;
     uint32_t exclusive = 0;
-    if (part == 0) {
-        __hip_atomic_store(status + tid, kFlagInc | tileCount, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagAgg | tileCount,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // Earlier partitions took their ids before this one, so they are
-        // resident and will publish; the bound only guards against a bug.
-        int64_t look = (int64_t)part - 1;
-        uint32_t spins = 0;
-        while (true) {
-            uint32_t v = __hip_atomic_load(status + (size_t)look * 256 + tid, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t flag = v & kFlagMask;
-            if (flag == 0u) {
-                if (++spins < kSpinLimit) continue;
-                atomicOr(devStatus, kStatusLookbackSpin);
-                break;
-            }
-            exclusive += v & kCountMask;
-            if (flag == kFlagInc) break;
-            --look;
+    uint32_t win[kWin];
+    int64_t look = (int64_t)part - 1;
+    auto loadWindow = [&]() {
+#pragma unroll
+        for (int k = 0; k < kWin; ++k) {
+            const int64_t q = look - k;
+            const uint32_t v = __hip_atomic_load(status + (size_t)(q < 0 ? 0 : q) * 256 + tid, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            win[k] = q < 0 ? kFlagInc : v;  // never reached: partition 0 is inclusive
         }
-        __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagInc | (exclusive + tileCount),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if (owner) {
+        if (part == 0) {
+            __hip_atomic_store(status + tid, kFlagInc | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagAgg | tileCount, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            loadWindow();
+        }
     }
-    const uint32_t localStart = blockExclusiveScan256(tileCount, s_tmp);
-    const uint32_t globalStart = blockExclusiveScan256(passHist[tid], s_tmp);
-    s_localStart[tid] = localStart;
-    s_digitBase[tid] = globalStart + exclusive;
+    const uint32_t localStart = digitExclusiveScan(tileCount, s_tmp);
+    const uint32_t globalStart = digitExclusiveScan(owner ? passHist[tid] : 0u, s_tmp);
+    if (owner) s_localStart[tid] = localStart;
     __syncthreads();
 
+    // Tile-local order in LDS (does not depend on the global offsets).
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const uint32_t d = (uint32_t)(keys[i] >> shift) & 0xFFu;
@@ -459,6 +485,38 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
         s_keys[pos] = keys[i];
         s_vals[pos] = vals[i];
     }
+
+    if (owner && part > 0) {
+        uint32_t spins = 0;
+        while (true) {
+            int stop = kWin;  // first entry not yet published
+            bool inc = false;
+            uint32_t add = 0;
+#pragma unroll
+            for (int k = 0; k < kWin; ++k) {
+                if (stop == kWin && !inc) {
+                    const uint32_t v = win[k], flag = v & kFlagMask;
+                    if (flag == 0u) {
+                        stop = k;
+                    } else {
+                        add += v & kCountMask;
+                        inc = flag == kFlagInc;
+                    }
+                }
+            }
+            exclusive += add;
+            if (inc) break;
+            look -= stop;
+            if (stop < kWin && ++spins >= kSpinLimit) {
+                atomicOr(devStatus, kStatusLookbackSpin);
+                break;
+            }
+            loadWindow();
+        }
+        __hip_atomic_store(status + (size_t)part * 256 + tid, kFlagInc | (exclusive + tileCount),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (owner) s_digitBase[tid] = globalStart + exclusive;
     __syncthreads();
     const uint32_t valid = min((uint32_t)kTile, n - tileBase);
     for (uint32_t j = tid; j < valid; j += kSortBlock) {
@@ -1030,8 +1088,9 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
              stream>>>(L.hist, L.clearWords, L.sceneOrd);
     // One set of same-address atomics per block: a few blocks per CU.
     const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
-    k_bounds<SOUP><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n,
-                                                                    L.boxes, L.sceneOrd);
+    // The soup path reads float4s with a grid stride that is a multiple of 3.
+    const unsigned boundsGrid = SOUP ? std::max(3u, reduceGrid / 3u * 3u) : gridFor(n, reduceGrid);
+    k_bounds<SOUP><<<boundsGrid, kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n, L.boxes, L.sceneOrd);
     ev(1);
     PrimSrc src{in.tris, L.boxes};
     k_morton<SOUP, KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
