@@ -1089,7 +1089,7 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     // One set of same-address atomics per block: a few blocks per CU.
     const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
     // The soup path reads float4s with a grid stride that is a multiple of 3.
-    const unsigned boundsGrid = SOUP ? std::max(3u, reduceGrid / 3u * 3u) : gridFor(n, reduceGrid);
+    const unsigned boundsGrid = SOUP ? std::max(3u, gridFor(n, reduceGrid) / 3u * 3u) : gridFor(n, reduceGrid);
     k_bounds<SOUP><<<boundsGrid, kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n, L.boxes, L.sceneOrd);
     ev(1);
     PrimSrc src{in.tris, L.boxes};
