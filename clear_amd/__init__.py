@@ -551,7 +551,7 @@ class WavefrontConfig(ctypes.Structure):
     """clr_wavefront_config: launch shapes of the wavefront renderer."""
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ["traceBlock", "traceWavesPerCU", "shadeBlock", "shadeWavesPerCU", "itersPerGraph",
-                 "checkEvery", "maxSlots", "useGraph", "refillMin", "ldsDepth"]]
+                 "checkEvery", "maxSlots", "useGraph", "refillMin", "ldsDepth", "raysPerLane"]]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -127,10 +127,14 @@ __global__ void k_accel_build_bvh(const float4* __restrict__ bvh, uint32_t n, co
 }
 
 // Ordered traversal, grid-stride over the rays with a fixed grid (waves per
-// CU x CUs) so that the traversal stack's overflow region is per lane:
-// kOrdLds slots in LDS ([slot][lane]), the rest in `spill`.
+// CU x CUs). Each lane walks two rays at once, stepping them in turn, so one
+// ray's node fetch is in flight while the other's boxes and triangles are
+// tested (the walk is bound by dependent load latency). Each walker's stack
+// keeps kOrdLds / 2 slots in LDS ([slot][lane]) and the rest in its part of
+// `spill`.
 constexpr int kOrdLds = 32;
 constexpr int kOrdBlock = 128;
+constexpr int kOrdHalf = kOrdLds / 2;
 
 template <bool STATS>
 __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
@@ -142,23 +146,54 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
     __shared__ uint32_t s_stack[kOrdLds * kOrdBlock];
     const uint32_t gid = blockIdx.x * kOrdBlock + threadIdx.x;
     const uint32_t lanes = gridDim.x * kOrdBlock;
-    const dev::SpillStack<kOrdLds> stk{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
+    using S = dev::SpillStack<kOrdHalf>;
+    const S stk0{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
+    const S stk1{s_stack + kOrdHalf * kOrdBlock + threadIdx.x, kOrdBlock,
+                 spill + (size_t)(dev::kStackPairs - kOrdHalf) * lanes + gid, lanes};
     const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris, bvh};
+    using W = dev::Ordered<STATS, dev::NoAlpha, S, true>;
+    W w0(g, stk0, dev::NoAlpha()), w1(g, stk1, dev::NoAlpha());
     const size_t R = numRays;
     bool over = false;
-    for (uint32_t r = gid; r < numRays; r += lanes) {
-        dev::V3 o = {rays[r], rays[R + r], rays[2 * R + r]};
-        dev::V3 d = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
-        float t = rays[6 * R + r];
-        dev::TraceStats st{0u, 0u};
-        const uint32_t best = dev::orderedHit<STATS, false, dev::NoAlpha, dev::SpillStack<kOrdLds>, true>(
-            g, o, d, t, stk, st, nullptr, nullptr, dev::NoAlpha(), &over);
-        const bool hit = best != kEmpty && !isinf(t);
-        hitFace[r] = hit ? best : kEmpty;
-        hitT[r] = t;
+    uint32_t next = gid, r0 = kEmpty, r1 = kEmpty;  // this lane's rays: gid, gid + lanes, ...
+    auto start = [&](W& w, uint32_t& r) {
+        if (next < numRays) {
+            r = next;
+            next += lanes;
+            const dev::V3 o = {rays[r], rays[R + r], rays[2 * R + r]};
+            const dev::V3 d = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+            w.begin(o, d, rays[6 * R + r], false);
+        } else {
+            r = kEmpty;
+            w.done = true;
+        }
+    };
+    auto finish = [&](const W& w, uint32_t r) {
+        const bool hit = w.best != kEmpty && !isinf(w.t);
+        hitFace[r] = hit ? w.best : kEmpty;
+        hitT[r] = w.t;
+        if (w.over) over = true;
         if constexpr (STATS) {
-            stats[2 * (size_t)r] = st.visits;
-            stats[2 * (size_t)r + 1] = st.tests;
+            stats[2 * (size_t)r] = w.st.visits;
+            stats[2 * (size_t)r + 1] = w.st.tests;
+        }
+    };
+    start(w0, r0);
+    start(w1, r1);
+    while (r0 != kEmpty || r1 != kEmpty) {
+        if (r0 != kEmpty) {
+            if (!w0.done) w0.step();
+            if (w0.done) {
+                finish(w0, r0);
+                start(w0, r0);
+            }
+        }
+        if (r1 != kEmpty) {
+            if (!w1.done) w1.step();
+            if (w1.done) {
+                finish(w1, r1);
+                start(w1, r1);
+            }
         }
     }
     if (over) atomicOr(status, kStatusStackOverflow);
@@ -219,7 +254,7 @@ int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     // 16 waves per CU (LDS: 16 KiB per 128-thread block).
     const uint32_t grid = std::max(1u, std::min((uint32_t)cus * 8u, (numRays + kOrdBlock - 1) / kOrdBlock));
     const size_t lanes = (size_t)grid * kOrdBlock;
-    rc = ensureTraceSpill(ctx, lanes * (dev::kStackPairs - kOrdLds) * sizeof(uint32_t));
+    rc = ensureTraceSpill(ctx, lanes * 2 * (dev::kStackPairs - kOrdHalf) * sizeof(uint32_t));
     if (rc) return rc;
     const float4 *anodes = nullptr, *atris = nullptr;
     rc = buildAccel(ctx, inodes, leaves, bvh, n, vertices, faces, &anodes, &atris, stream);

@@ -71,7 +71,7 @@ struct WfBufs {
     float4* sqDir;
     float4* sqC;     // weighted contribution
     uint32_t* counts;
-    uint32_t* spill; // traversal stack overflow region, (kStackPairs - kMinLdsDepth) x lanes
+    uint32_t* spill; // traversal stack overflow regions, 2 x (kStackPairs - kMinLdsDepth) x lanes
     uint32_t lanes;
     uint32_t slots;  // P: capacity of every queue
 };
@@ -121,7 +121,7 @@ size_t wfBytes(uint32_t P, uint32_t lanes) {
     add(16ull * P); add(4ull * P); add(4ull * P);
     add(4ull * P); add(16ull * P); add(16ull * P); add(16ull * P);
     add(256);
-    add(4ull * (kStackPairs - kMinLdsDepth) * lanes);
+    add(4ull * 2 * (kStackPairs - kMinLdsDepth) * lanes);
     return b;
 }
 
@@ -146,7 +146,7 @@ WfBufs carveWf(void* pool, uint32_t P, uint32_t lanes) {
     B.sqDir = c.take<float4>(P);
     B.sqC = c.take<float4>(P);
     B.counts = c.take<uint32_t>(64);
-    B.spill = c.take<uint32_t>((size_t)(kStackPairs - kMinLdsDepth) * lanes);
+    B.spill = c.take<uint32_t>((size_t)2 * (kStackPairs - kMinLdsDepth) * lanes);
     B.lanes = lanes;
     B.slots = P;
     return B;
@@ -320,7 +320,7 @@ __global__ void k_wf_init(WfArgs A, uint32_t chunk, uint32_t count, uint32_t spp
 
 // Closest hit of extension queue `in`; resets the counters the shade / miss /
 // shadow kernels of this iteration append to.
-template <bool ORDERED, int LD, bool BATCH>
+template <bool ORDERED, int LD, bool BATCH, int K = 1>
 __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     extern __shared__ uint32_t s_stack[];
     const WfBufs& B = A.B;
@@ -338,32 +338,49 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
     bool over = false;
     if constexpr (ORDERED) {
         // Persistent waves refilling idle lanes from a wave pool; hit / miss
-        // queues are built by k_wf_compact.
+        // queues are built by k_wf_compact. With K = 2 each lane walks two
+        // rays at once, stepping them in turn, so that one ray's node fetch
+        // overlaps the other's tests; each walker has its own stack.
         const RScene* S0 = firstScene<BATCH>(A, S);
-        Ordered<false, TexAlpha, SpillStack<LD>, true> w(S0->g, stk, TexAlpha{S0});
+        const SpillStack<LD> stk1{s_stack + LD * blockDim.x + threadIdx.x, (int)blockDim.x,
+                                  B.spill + (size_t)(kStackPairs - kMinLdsDepth) * lanes + gid, lanes};
+        using W = Ordered<false, TexAlpha, SpillStack<LD>, true>;
+        W w0(S0->g, stk, TexAlpha{S0}), w1(S0->g, stk1, TexAlpha{S0});
         WavePool pool(n);
-        bool busy = false;
-        uint32_t j = 0;
+        bool busy0 = false, busy1 = false;
+        uint32_t j0 = 0, j1 = 0;
+        auto start = [&](W& w, uint32_t j) {
+            if constexpr (BATCH) {
+                const RScene* Sp = A.table + (chunk + B.rqSlot[in][j]) / A.posPerScene;
+                w.rebind(Sp->g, TexAlpha{Sp});
+            }
+            w.begin(xyz(B.rqOrg[in][j]), xyz(B.rqDir[in][j]), INFINITY);
+        };
+        auto advance = [&](W& w, uint32_t j, bool& busy) {
+            if (!w.done) w.step();
+            if (w.done) {
+                over |= w.over;
+                const bool isHit = w.best != kNone && !isinf(w.t);
+                B.hit[j] = make_uint4(isHit ? w.best : kNone, __float_as_uint(w.t), __float_as_uint(w.b1),
+                                      __float_as_uint(w.b2));
+                busy = false;
+            }
+        };
         while (true) {
-            if (pool.refill(A.refillMin, busy, j)) {
-                if constexpr (BATCH) {
-                    const RScene* Sp = A.table + (chunk + B.rqSlot[in][j]) / A.posPerScene;
-                    w.rebind(Sp->g, TexAlpha{Sp});
-                }
-                w.begin(xyz(B.rqOrg[in][j]), xyz(B.rqDir[in][j]), INFINITY);
-                busy = true;
+            if (pool.refill(A.refillMin, busy0, j0)) {
+                start(w0, j0);
+                busy0 = true;
             }
-            if (__ballot(busy) == 0 && pool.drained()) break;
-            if (busy) {
-                if (!w.done) w.step();
-                if (w.done) {
-                    over |= w.over;
-                    const bool isHit = w.best != kNone && !isinf(w.t);
-                    B.hit[j] = make_uint4(isHit ? w.best : kNone, __float_as_uint(w.t), __float_as_uint(w.b1),
-                                          __float_as_uint(w.b2));
-                    busy = false;
+            if constexpr (K == 2) {
+                if (pool.refill(A.refillMin, busy1, j1)) {
+                    start(w1, j1);
+                    busy1 = true;
                 }
             }
+            if (__ballot(busy0 || busy1) == 0 && pool.drained()) break;
+            if (busy0) advance(w0, j0, busy0);
+            if constexpr (K == 2)
+                if (busy1) advance(w1, j1, busy1);
         }
         if (over) atomicOr(S.status, kStatusStackOverflow);
         return;
@@ -575,7 +592,7 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
 
 // Shadow rays of this iteration; resets the hit / miss counters for the next
 // iteration's extend kernel.
-template <bool ORDERED, int LD, bool BATCH>
+template <bool ORDERED, int LD, bool BATCH, int K = 1>
 __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
     RScene S = A.S;
     if (!BATCH) finishScene(S);
@@ -596,41 +613,58 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         // Persistent waves as in k_wf_extend; environment samples use the
         // occlusion walk, area samples the closest hit compared with the
         // light's face. Samples ending here are finished by k_wf_shadow_end.
+        // K walkers per lane, as in k_wf_extend (a path slot has at most one
+        // shadow ray per iteration, so the pixel updates never collide).
         const RScene* S0 = firstScene<BATCH>(A, S);
-        Ordered<false, TexAlpha, SpillStack<LD>, true> w(S0->g, stk, TexAlpha{S0});
+        const SpillStack<LD> stk1{s_stack + LD * blockDim.x + threadIdx.x, (int)blockDim.x,
+                                  B.spill + (size_t)(kStackPairs - kMinLdsDepth) * lanes + gid, lanes};
+        using W = Ordered<false, TexAlpha, SpillStack<LD>, true>;
+        W w0(S0->g, stk, TexAlpha{S0}), w1(S0->g, stk1, TexAlpha{S0});
         WavePool pool(n);
-        bool busy = false;
-        uint32_t i = 0, target = kNone;
+        bool busy0 = false, busy1 = false;
+        uint32_t i0 = 0, i1 = 0, target0 = kNone, target1 = kNone;
+        auto start = [&](W& w, uint32_t i, uint32_t& target) {
+            if constexpr (BATCH) {
+                const RScene* Sp = A.table + (chunk + B.sqSlot[i]) / A.posPerScene;
+                w.rebind(Sp->g, TexAlpha{Sp});
+            }
+            const float4 o4 = B.sqOrg[i];
+            target = __float_as_uint(o4.w);
+            w.begin(xyz(o4), xyz(B.sqDir[i]), INFINITY, target == kNone);
+        };
+        auto advance = [&](W& w, uint32_t i, uint32_t target, bool& busy) {
+            if (!w.done) w.step();
+            if (w.done) {
+                over |= w.over;
+                const bool hit = w.best != kNone && !isinf(w.t);
+                const bool visible = target == kNone ? !hit : (hit && w.best == target);
+                if (visible) {
+                    const uint32_t slot = B.sqSlot[i];
+                    const float4 c = B.sqC[i];
+                    float4 p4 = B.pix[slot];
+                    p4.x += c.x;
+                    p4.y += c.y;
+                    p4.z += c.z;
+                    B.pix[slot] = p4;
+                }
+                busy = false;
+            }
+        };
         while (true) {
-            if (pool.refill(A.refillMin, busy, i)) {
-                if constexpr (BATCH) {
-                    const RScene* Sp = A.table + (chunk + B.sqSlot[i]) / A.posPerScene;
-                    w.rebind(Sp->g, TexAlpha{Sp});
-                }
-                const float4 o4 = B.sqOrg[i];
-                target = __float_as_uint(o4.w);
-                w.begin(xyz(o4), xyz(B.sqDir[i]), INFINITY, target == kNone);
-                busy = true;
+            if (pool.refill(A.refillMin, busy0, i0)) {
+                start(w0, i0, target0);
+                busy0 = true;
             }
-            if (__ballot(busy) == 0 && pool.drained()) break;
-            if (busy) {
-                if (!w.done) w.step();
-                if (w.done) {
-                    over |= w.over;
-                    const bool hit = w.best != kNone && !isinf(w.t);
-                    const bool visible = target == kNone ? !hit : (hit && w.best == target);
-                    if (visible) {
-                        const uint32_t slot = B.sqSlot[i];
-                        const float4 c = B.sqC[i];
-                        float4 p4 = B.pix[slot];
-                        p4.x += c.x;
-                        p4.y += c.y;
-                        p4.z += c.z;
-                        B.pix[slot] = p4;
-                    }
-                    busy = false;
+            if constexpr (K == 2) {
+                if (pool.refill(A.refillMin, busy1, i1)) {
+                    start(w1, i1, target1);
+                    busy1 = true;
                 }
             }
+            if (__ballot(busy0 || busy1) == 0 && pool.drained()) break;
+            if (busy0) advance(w0, i0, target0, busy0);
+            if constexpr (K == 2)
+                if (busy1) advance(w1, i1, target1, busy1);
         }
         if (over) atomicOr(S.status, kStatusStackOverflow);
         return;
@@ -758,6 +792,7 @@ clr_wavefront_config defaultConfig() {
     c.useGraph = 1;
     c.refillMin = 16;
     c.ldsDepth = 32;
+    c.raysPerLane = 1;
     return c;
 }
 
@@ -766,7 +801,8 @@ int validConfig(const clr_wavefront_config& c) {
     return okBlock(c.traceBlock) && okBlock(c.shadeBlock) && c.traceWavesPerCU >= 1 &&
            c.traceWavesPerCU <= 32 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 32 &&
            c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64 &&
-           c.refillMin >= 1 && c.refillMin <= 64 && (c.ldsDepth == 16 || c.ldsDepth == 32);
+           c.refillMin >= 1 && c.refillMin <= 64 && (c.ldsDepth == 16 || c.ldsDepth == 32) &&
+           (c.raysPerLane == 1 || c.raysPerLane == 2);
 }
 
 int ensureWfState(clr_ctx* ctx) {
@@ -793,12 +829,12 @@ Launch shadeLaunch(const WfState& w) {
     return {std::max(1u, (uint32_t)w.numCUs * w.cfg.shadeWavesPerCU * 64u / b), b};
 }
 
-template <bool BATCH>
+template <bool BATCH, int K>
 void launchIteration(const WfArgs& A, const Launch& tl, const Launch& sl, size_t lds, bool ld16, bool ordered,
                      uint32_t in, hipStream_t st) {
-    if (ordered) {
-        if (ld16) k_wf_extend<true, 16, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
-        else k_wf_extend<true, 32, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+    if (ordered) {  // K walkers per lane: K stacks
+        if (ld16) k_wf_extend<true, 16, BATCH, K><<<tl.grid, tl.block, K * lds, st>>>(A, in);
+        else k_wf_extend<true, 32, BATCH, K><<<tl.grid, tl.block, K * lds, st>>>(A, in);
         k_wf_compact<<<sl.grid, sl.block, 0, st>>>(A, in);
     } else if constexpr (!BATCH) {
         if (ld16) k_wf_extend<false, 16, false><<<tl.grid, tl.block, lds, st>>>(A, in);
@@ -807,8 +843,8 @@ void launchIteration(const WfArgs& A, const Launch& tl, const Launch& sl, size_t
     k_wf_shade<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
     k_wf_miss<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
     if (ordered) {
-        if (ld16) k_wf_shadow<true, 16, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
-        else k_wf_shadow<true, 32, BATCH><<<tl.grid, tl.block, lds, st>>>(A, in);
+        if (ld16) k_wf_shadow<true, 16, BATCH, K><<<tl.grid, tl.block, K * lds, st>>>(A, in);
+        else k_wf_shadow<true, 32, BATCH, K><<<tl.grid, tl.block, K * lds, st>>>(A, in);
         k_wf_shadow_end<BATCH><<<sl.grid, sl.block, 0, st>>>(A, in);
     } else if constexpr (!BATCH) {
         if (ld16) k_wf_shadow<false, 16, false><<<tl.grid, tl.block, lds, st>>>(A, in);
@@ -821,9 +857,15 @@ void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStr
     const bool ordered = (A.flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
     const bool ld16 = w.cfg.ldsDepth == 16;
     const size_t lds = sizeof(uint32_t) * (ld16 ? 16 : 32) * tl.block;
+    const bool two = w.cfg.raysPerLane == 2;
     for (uint32_t it = 0; it < iters; ++it) {
-        if (A.table) launchIteration<true>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
-        else launchIteration<false>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+        if (A.table) {
+            if (two) launchIteration<true, 2>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+            else launchIteration<true, 1>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+        } else {
+            if (two) launchIteration<false, 2>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+            else launchIteration<false, 1>(A, tl, sl, lds, ld16, ordered, it & 1u, st);
+        }
     }
 }
 
@@ -1110,6 +1152,10 @@ int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numSce
     tryVariants([&](clr_wavefront_config& c, int v) { c.shadeBlock = blocks[v / 4]; c.shadeWavesPerCU = waves[v % 4]; }, 12);
     const uint32_t refills[4] = {8, 16, 32, 64};
     tryVariants([&](clr_wavefront_config& c, int v) { c.refillMin = refills[v]; }, 4);
+    tryVariants([&](clr_wavefront_config& c, int v) {
+        c.raysPerLane = c.raysPerLane == 1 ? 2u : 1u;
+        c.traceWavesPerCU = waves[v];
+    }, 4);
     tryVariants([&](clr_wavefront_config& c, int v) {
         c.ldsDepth = v < 4 ? 16u : 32u;
         c.traceWavesPerCU = (v % 4 + 1) * 8u;

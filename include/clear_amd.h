@@ -390,8 +390,11 @@ typedef struct clr_wavefront_config {
     uint32_t useGraph;        /* 1: replay the captured graph, 0: launch kernels directly */
     uint32_t refillMin;       /* traversal kernels hand new rays to a wave's idle lanes once at
                                  least this many are idle (1..64; 64: only when all are) */
-    uint32_t ldsDepth;        /* traversal stack entries per lane kept in LDS (16 or 32; the
-                                 rest of the 64 spill to a per-lane global region) */
+    uint32_t ldsDepth;        /* traversal stack words per walker kept in LDS (16 or 32; the
+                                 rest of the 128 spill to a per-lane global region) */
+    uint32_t raysPerLane;     /* rays each lane of the ordered extend / shadow kernels walks at
+                                 once (1 or 2): 2 overlaps one ray's node fetch with the other's
+                                 tests (incoherent rays), 1 keeps more waves resident */
 } clr_wavefront_config;
 
 int clr_wavefront_get_config(clr_ctx* ctx, clr_wavefront_config* config);

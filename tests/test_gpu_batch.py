@@ -94,6 +94,7 @@ def test_batch_chunks_straddle_scenes_and_direct_launches(gpu_ctx, scenes):
         cfg.maxSlots = 5000  # not a multiple of the per-scene positions (64 * 8 * 6 tiles)
         cfg.useGraph = 0
         cfg.checkEvery = 1
+        cfg.raysPerLane = 2  # two walkers per lane, rebinding scenes independently
         ca.set_wavefront_config(gpu_ctx, cfg)
         print("[batch] chunks of 5000 slots, direct launches", flush=True)
         batch = ca.SceneBatch(gpu_ctx, scenes[:4])

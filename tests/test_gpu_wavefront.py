@@ -85,11 +85,22 @@ def test_chunks_direct_launches_and_calls(gpu_ctx):
 
 def test_graph_shapes(gpu_ctx):
     host = ca.HostScene(ca.SCENE_CORNELL, 61, 13, detail=2)
-    for tb, sb, g, chk, ld, rf in [(64, 256, 2, 1, 16, 1), (256, 64, 16, 0, 32, 64),
-                                   (128, 128, 64, 3, 16, 24)]:
+    for tb, sb, g, chk, ld, rf, k in [(64, 256, 2, 1, 16, 1, 1), (256, 64, 16, 0, 32, 64, 2),
+                                      (128, 128, 64, 3, 16, 24, 2), (64, 64, 4, 0, 32, 8, 2)]:
         cfg = default_cfg(gpu_ctx, traceBlock=tb, shadeBlock=sb, itersPerGraph=g, checkEvery=chk,
-                          traceWavesPerCU=4, shadeWavesPerCU=4, ldsDepth=ld, refillMin=rf)
+                          traceWavesPerCU=4, shadeWavesPerCU=4, ldsDepth=ld, refillMin=rf, raysPerLane=k)
         same_as_megakernel(gpu_ctx, host, 4, 5, cfg=cfg)
+
+
+@pytest.mark.parametrize("ld", [16, 32])
+def test_two_rays_per_lane(gpu_ctx, ld):
+    """Two walkers per lane in the extend / shadow kernels: same image as the
+    megakernel, with textured alpha and both environment and area-light shadow
+    rays (showcase scene), graph and direct launches."""
+    host = ca.HostScene(ca.SCENE_SHOWCASE, 96, 64)
+    for graph in (1, 0):
+        cfg = default_cfg(gpu_ctx, raysPerLane=2, ldsDepth=ld, useGraph=graph, maxSlots=4096)
+        same_as_megakernel(gpu_ctx, host, 4, 6, cfg=cfg)
 
 
 def test_autotune_leaves_caller_buffers_alone(gpu_ctx):
@@ -118,5 +129,8 @@ def test_rejects_bad_config(gpu_ctx):
     with pytest.raises(ca.ClrError):
         ca.set_wavefront_config(gpu_ctx, cfg)
     cfg = default_cfg(gpu_ctx, ldsDepth=24)
+    with pytest.raises(ca.ClrError):
+        ca.set_wavefront_config(gpu_ctx, cfg)
+    cfg = default_cfg(gpu_ctx, raysPerLane=3)
     with pytest.raises(ca.ClrError):
         ca.set_wavefront_config(gpu_ctx, cfg)
