@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const KeyT* __restrict_
     // overlaps that work. Earlier partitions took their ids before this one,
     // so they are resident and will publish; the bound only guards against a
     // bug.
-    constexpr int kWin = 8;
+    constexpr int kWin = 4;
     uint32_t exclusive = 0;
     uint32_t win[kWin];
     int64_t look = (int64_t)part - 1;
