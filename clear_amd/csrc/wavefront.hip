@@ -1120,13 +1120,19 @@ int wavefrontAutotune(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numSce
         int r = CLR_OK;
         for (uint32_t i = 0; i < images && !r; ++i)
             r = seedPixelStates(ctx, rp.width, rp.height, states + 4 * numPix * i, stream);
-        // One untimed render: pool growth and graph capture happen here.
+        // One untimed render (pool growth and graph capture happen here),
+        // then the better of two timed renders.
         if (!r) r = renderAll();
-        if (!r) r = hipCheck(ctx, hipEventRecord(e0, stream), "event record");
-        if (!r) r = renderAll();
-        if (!r) r = hipCheck(ctx, hipEventRecord(e1, stream), "event record");
-        if (!r) r = hipCheck(ctx, hipEventSynchronize(e1), "event sync");
-        if (!r) r = hipCheck(ctx, hipEventElapsedTime(ms, e0, e1), "event time");
+        *ms = 0.0f;
+        for (int rep = 0; rep < 2 && !r; ++rep) {
+            float t = 0.0f;
+            r = hipCheck(ctx, hipEventRecord(e0, stream), "event record");
+            if (!r) r = renderAll();
+            if (!r) r = hipCheck(ctx, hipEventRecord(e1, stream), "event record");
+            if (!r) r = hipCheck(ctx, hipEventSynchronize(e1), "event sync");
+            if (!r) r = hipCheck(ctx, hipEventElapsedTime(&t, e0, e1), "event time");
+            if (!r && (rep == 0 || t < *ms)) *ms = t;
+        }
         return r;
     };
     clr_wavefront_config best = w.tuned ? w.cfg : defaultConfig();
