@@ -97,6 +97,15 @@ def test_ordered_axis_aligned_and_finite_tmax(gpu_ctx):
     check_ordered(gpu_ctx, soup, rays)
 
 
+def test_ordered_several_rays_per_lane(gpu_ctx):
+    """2^20 rays: more rays than the trace grid has lanes (at most 2^18 on
+    this device), so every lane walks two rays at once and restarts each walker
+    several times; hits, distances and visit / test counts equal the oracle's."""
+    soup = ca.gen_cluster_soup(1 << 16, seed=5, num_clusters=16, sigma=0.05, e_min=0.005, e_max=0.03)
+    h = check_ordered(gpu_ctx, soup, ca.gen_random_rays(1 << 20, seed=6))
+    assert (h != 0xFFFFFFFF).mean() > 0.1
+
+
 def test_ordered_1m_tris_256k_rays_match_reference_walk(gpu_ctx):
     """Ordered and reference walks agree on every ray here (no exact ties)."""
     soup = ca.gen_cluster_soup(1 << 20, seed=1)
