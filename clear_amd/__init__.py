@@ -91,6 +91,11 @@ def lib():
         "clr_scene_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                  ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "clr_scene_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_uint64)]),
+        "clr_image_load": (i32, [ctypes.c_char_p, i32, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                 ctypes.POINTER(u32), vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+        "clr_scene_build_files": (i32, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(f32), u32,
+                                        ctypes.c_char_p, f32, vp, ctypes.POINTER(vp)]),
+        "clr_scene_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -353,6 +358,35 @@ class HostScene:
         rc = lib().clr_scene_build(kind, ctypes.byref(p), ctypes.byref(h))
         if rc:
             raise ClrError(rc, f"clr_scene_build({kind})")
+        self._take(h)
+
+    @classmethod
+    def from_files(cls, obj_paths, emissions=None, env_path=None, env_multiplier: float = 1.0,
+                   width: int = 256, height: int = 256, fov_y: float = 0.0, lens_radius: float = 0.0,
+                   focus_distance: float = 0.0, cpu_bvh: bool = False) -> "HostScene":
+        """clr_scene_build_files: OBJ models (with MTL materials and image
+        textures) loaded by the reference's loadModel rules; model k emits
+        emissions[k] (RGB) when nonzero; env_path: an OpenEXR environment."""
+        paths = [str(x).encode() for x in obj_paths]
+        arr = (ctypes.c_char_p * max(len(paths), 1))(*paths)
+        em = np.zeros(3 * max(len(paths), 1), np.float32)
+        if emissions is not None:
+            for k, e in enumerate(emissions):
+                if e is not None:
+                    em[3 * k:3 * k + 3] = e
+        p = SceneParams(width, height, 0, 0, lens_radius, focus_distance, int(cpu_bvh), 0.0, fov_y, 0)
+        h = ctypes.c_void_p()
+        rc = lib().clr_scene_build_files(arr, em.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(paths),
+                                         str(env_path).encode() if env_path else None, env_multiplier,
+                                         ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise ClrError(rc, "clr_scene_build_files: " + lib().clr_scene_last_error().decode())
+        obj = cls.__new__(cls)
+        obj._take(h)
+        return obj
+
+    def _take(self, h) -> None:
+        """Copies the host scene's arrays and frees it."""
         try:
             nf, nl, w, hh = (ctypes.c_uint32() for _ in range(4))
             lib().clr_scene_info(h, ctypes.byref(nf), ctypes.byref(nl), ctypes.byref(w), ctypes.byref(hh))
@@ -385,6 +419,34 @@ class HostScene:
         import torch
 
         return {k: torch.from_numpy(v).to(device) for k, v in self.arrays.items()}
+
+
+COLOR_RGB8, COLOR_RGBA8, COLOR_RGBA16F, COLOR_GRAY8 = 0, 1, 2, 3
+
+
+def load_image(path, gamma_correction: bool = False):
+    """clr_image_load: (pixels, width, height, color type) of a PNG or OpenEXR
+    file: uint8 [h, w, 4] for RGB8 (4th byte 0xFF) and RGBA8, float16
+    [h, w, 4] for RGBA16F, uint8 [h, w] for gray8."""
+    w, h, ct, nb = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    bpath = str(path).encode()
+    rc = lib().clr_image_load(bpath, int(gamma_correction), ctypes.byref(w), ctypes.byref(h), ctypes.byref(ct),
+                              None, 0, ctypes.byref(nb))
+    if rc:
+        raise ClrError(rc, "clr_image_load: " + lib().clr_scene_last_error().decode())
+    buf = np.zeros(max(nb.value, 1), np.uint8)
+    rc = lib().clr_image_load(bpath, int(gamma_correction), ctypes.byref(w), ctypes.byref(h), ctypes.byref(ct),
+                              buf.ctypes.data, nb.value, ctypes.byref(nb))
+    if rc:
+        raise ClrError(rc, "clr_image_load: " + lib().clr_scene_last_error().decode())
+    buf = buf[:nb.value]
+    if ct.value == COLOR_RGBA16F:
+        px = buf.view(np.float16).reshape(h.value, w.value, 4)
+    elif ct.value == COLOR_GRAY8:
+        px = buf.reshape(h.value, w.value)
+    else:
+        px = buf.reshape(h.value, w.value, 4)
+    return px, w.value, h.value, ct.value
 
 
 class DeviceScene:

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/clear_amd_loaders.hpp"
 #include "../../include/clear_amd_scene.hpp"
 #include "../../include/clear_amd_scenes.h"
 
@@ -668,6 +669,96 @@ int clr_bvh_build_midpoint(const clr_float3* vertices, const clr_face* faces, ui
     } catch (const std::exception&) {
         return CLR_ERR_OUT_OF_MEMORY;
     }
+    return CLR_OK;
+}
+
+namespace {
+thread_local std::string g_sceneError;
+}
+
+const char* clr_scene_last_error(void) { return g_sceneError.c_str(); }
+
+int clr_image_load(const char* path, int gammaCorrection, uint32_t* width, uint32_t* height, uint32_t* colorType,
+                   void* pixels, uint64_t capacity, uint64_t* bytes) {
+    g_sceneError.clear();
+    if (!path || !width || !height || !colorType || !bytes) return CLR_ERR_INVALID_ARG;
+    std::vector<uint8_t> data;
+    ColorChannel::Value ct;
+    try {
+        if (!loadImage(path, &data, width, height, &ct, gammaCorrection != 0, &g_sceneError))
+            return CLR_ERR_INVALID_ARG;
+    } catch (const std::exception& e) {
+        g_sceneError = e.what();
+        return CLR_ERR_OUT_OF_MEMORY;
+    }
+    *colorType = (uint32_t)ct;
+    *bytes = data.size();
+    if (pixels) {
+        if (capacity < data.size()) {
+            g_sceneError = "pixel buffer too small";
+            return CLR_ERR_INVALID_ARG;
+        }
+        std::memcpy(pixels, data.data(), data.size());
+    }
+    return CLR_OK;
+}
+
+int clr_scene_build_files(const char* const* objPaths, const float* emission, uint32_t numModels,
+                          const char* envPath, float envMultiplier, const clr_scene_params* params,
+                          clr_host_scene** out) {
+    g_sceneError.clear();
+    if (!out || (numModels && !objPaths)) return CLR_ERR_INVALID_ARG;
+    *out = nullptr;
+    clr_scene_params p{};
+    if (params) p = *params;
+    clr_host_scene* hs = new (std::nothrow) clr_host_scene();
+    if (!hs) return CLR_ERR_OUT_OF_MEMORY;
+    try {
+        Scene& s = hs->scene;
+        s.buildCpuBvh = p.buildCpuBvh != 0;
+        MaterialCreator& mc = MaterialCreator::sharedInstance();
+        mc.setScene(&s);
+        for (uint32_t k = 0; k < numModels; ++k) {
+            uint64_t light = UINT64_MAX;
+            if (emission && (emission[3 * k] != 0.0f || emission[3 * k + 1] != 0.0f || emission[3 * k + 2] != 0.0f)) {
+                const std::string m = "Model" + std::to_string(k) + "_M", lp = "Model" + std::to_string(k) + "_Light";
+                mc.createFloat3ConstantTexture(m.c_str(), emission[3 * k], emission[3 * k + 1], emission[3 * k + 2]);
+                mc.createDiffuseLightProperty(lp.c_str(), m.c_str());
+                light = s.idxOfLight(lp);
+            }
+            std::string err;
+            if (!loadModel(objPaths[k], &s, UINT64_MAX, light, &err)) throw std::runtime_error(err);
+        }
+        if (s.vertices.empty()) throw std::runtime_error("the models have no vertices");
+        if (envPath) {
+            createImageTexture(mc, s, "EnvImage", envPath);
+            mc.createImageBasedEnvLightPropety("EnvLight", "EnvImage", envMultiplier);
+            EnvironmentHead eh{(uint32_t)s.idxOfLight("EnvLight")};
+            s.setEnvironment(addDataAligned(&s.otherResouces, eh, 4));
+        }
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (const clr_float3& v : s.vertices) {
+            const float c[3] = {v.x, v.y, v.z};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::fmin(lo[a], c[a]);
+                hi[a] = std::fmax(hi[a], c[a]);
+            }
+        }
+        const float fov = p.fovY > 0 ? p.fovY : 45.0f;
+        const float at[3] = {0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
+        const float r = 0.5f * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                         (hi[2] - lo[2]) * (hi[2] - lo[2]));
+        const float dist = std::fmax(r, 1e-3f) / std::sin(0.5f * fov * kPiF / 180.0f);
+        const float eye[3] = {at[0], at[1], at[2] + dist};
+        setPerspective(s, hs, p.width ? p.width : 256, p.height ? p.height : 256, eye, at, fov,
+                       p.lensRadius > 0 ? p.lensRadius : 0.0f, p.focusDistance > 0 ? p.focusDistance : dist);
+        s.build();
+    } catch (const std::exception& e) {
+        g_sceneError = e.what();
+        delete hs;
+        return CLR_ERR_INVALID_ARG;
+    }
+    *out = hs;
     return CLR_OK;
 }
 

@@ -11,9 +11,8 @@
 // Additions (no reference counterpart): an FP32 RGB image texture type
 // (TextureType::ColorImageRGBF32x3) used for HDR environments built in memory,
 // and a Beckmann microfacet element (MatElem::Beckmann) for coated materials.
-// Differences: images are supplied as in-memory pixels rather than files
-// (the reference's libpng/OpenEXR loaders are out of scope), and
-// calcLightPowerDistribution does not write PMF[numItems] (the reference
+// Image files (PNG, OpenEXR) and OBJ/MTL models are read by clear_amd_loaders.hpp; images
+// can also be supplied as in-memory pixels. Differences: calcLightPowerDistribution does not write PMF[numItems] (the reference
 // writes one element past the end, Scene.h:281-284).
 #pragma once
 
@@ -268,6 +267,9 @@ public:
     std::vector<float> lightPowers;
     std::vector<uint8_t> materialsData, texturesData, otherResouces;
     std::map<std::string, uint64_t> materialsRef, lightPropsRef, texturesRef, othersRef;
+    std::map<std::string, uint64_t> texFilesDatabase;  // image file -> texture head (Scene.h:129-132)
+    std::vector<std::string> modelDatabase;           // loaded model files (Scene.h:137-139)
+    uint64_t numModelsLoaded = 0;                     // prefixes the names of each model's materials
     BVH bvh;
     bool buildCpuBvh = true;
 
@@ -315,6 +317,13 @@ public:
     bool addLightProp(uint64_t idx, const char* name) { return lightPropsRef.emplace(name, idx).second; }
     bool addTexture(uint64_t idx, const char* name) { return texturesRef.emplace(name, idx).second; }
     bool addOtherResouce(uint64_t idx, const char* name) { return othersRef.emplace(name, idx).second; }
+    bool addTexFileToDB(uint64_t idx, const char* filename) { return texFilesDatabase.emplace(filename, idx).second; }
+    void addModelToDB(const char* filename) { modelDatabase.emplace_back(filename); }
+    bool texFilehasLoaded(const std::string& filename, uint64_t* idx) const {
+        auto it = texFilesDatabase.find(filename);
+        *idx = it == texFilesDatabase.end() ? 0 : it->second;
+        return it != texFilesDatabase.end();
+    }
     bool setCamera(uint64_t idx) { return othersRef.emplace("Camera", idx).second; }
     bool setEnvironment(uint64_t idx) {
         bool ret = othersRef.emplace("Environment", idx).second;
@@ -344,6 +353,7 @@ public:
     uint64_t idxOfLight(const std::string& n) const { return lookup(lightPropsRef, n, "light property"); }
     uint64_t idxOfTex(const std::string& n) const { return lookup(texturesRef, n, "texture"); }
     uint64_t idxOfOther(const std::string& n) const { return lookup(othersRef, n, "resource"); }
+    uint8_t* ptrOfTex(const std::string& n) { return texturesData.data() + idxOfTex(n); }
 
     void calcLightPowerDistribution() {
         uint64_t head = fillZerosAligned(&otherResouces, sizeof(Discrete1D), 4);
@@ -481,15 +491,50 @@ public:
         scene->addOtherResouce(addDataAligned(&scene->otherResouces, f, 4), name);
     }
 
-    // Environment importance distribution from an FP32 RGB image: one stratum
-    // per pixel, luminance (Rec.709) times the zenith correction
-    // cos(pi (0.5 - (i + 0.5) / H)) (CreateFunctions.cpp:246-349 with
-    // averageLuminance over single-pixel regions).
+    static float halfBitsToFloat(uint16_t h) {
+        const uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 0x1F, m = h & 0x3FF;
+        float mag = e == 0 ? std::ldexp((float)m, -24) : e == 31 ? (m ? NAN : INFINITY) : std::ldexp((float)(m | 0x400), (int)e - 25);
+        return s ? -mag : mag;
+    }
+    // averageLuminance (CreateFunctions.cpp:159-244) of an RGBA16F image over the region
+    // [xl, xr) x [yt, yb) in pixels: partial corner and edge pixels weighted by their
+    // covered fraction, in the reference's order of additions.
+    float averageLuminanceHalf(const ImageTexture* im, const uint16_t* px, float xl, float xr, float yt, float yb) const {
+        const uint32_t x0 = (uint32_t)xl, x1 = (uint32_t)std::ceil(xr) - 1, y0 = (uint32_t)yt, y1 = (uint32_t)std::ceil(yb) - 1;
+        auto lum = [&](uint64_t off) {
+            const float R = halfBitsToFloat(px[off]), G = halfBitsToFloat(px[off + 1]), B = halfBitsToFloat(px[off + 2]);
+            return 0.2126f * R + 0.7152f * G + 0.0722f * B;
+        };
+        const uint64_t w = im->width;
+        float sum = 0.0f;
+        const uint64_t corners[4] = {4 * (y0 * w + x0), 4 * (y0 * w + x1), 4 * (y1 * w + x0), 4 * (y1 * w + x1)};
+        const float cc[4] = {(x0 + 1 - xl) * (y0 + 1 - yt), (xr - x1) * (y0 + 1 - yt), (x0 + 1 - xl) * (yb - y1), (xr - x1) * (yb - y1)};
+        for (int k = 0; k < 4; ++k) sum += cc[k] * lum(corners[k]);
+        const float ce[4] = {y0 + 1 - yt, x0 + 1 - xl, xr - x1, yb - y1};
+        for (uint32_t x = x0 + 1; x < x1; ++x) {
+            sum += ce[0] * lum(4 * (y0 * w + x));
+            sum += ce[3] * lum(4 * (y1 * w + x));
+        }
+        for (uint32_t y = y0 + 1; y < y1; ++y) {
+            sum += ce[1] * lum(4 * (y * w + x0));
+            sum += ce[2] * lum(4 * (y * w + x1));
+        }
+        for (uint32_t y = y0 + 1; y < y1; ++y)
+            for (uint32_t x = x0 + 1; x < x1; ++x) sum += lum(4 * (y * w + x));
+        return sum / ((xr - xl) * (yb - yt));
+    }
+
+    // Environment importance distribution from an image: one stratum per pixel. RGBA16F
+    // images (OpenEXR files) follow the reference exactly: averageLuminance of each pixel's
+    // region times the zenith correction cos(pi (0.5 - (i + 0.5) / H)) (CreateFunctions.cpp:
+    // 246-349). FP32 RGB images (built in memory) take the pixel's luminance times the same
+    // correction.
     void createContinuousConsts2D_H_FromImageTexture(const char* name, const char* image, bool zenithCorrection) {
         auto* od = &scene->otherResouces;
         const ImageTexture* im = reinterpret_cast<const ImageTexture*>(scene->texturesData.data() + scene->idxOfTex(image));
-        if (im->texType != TextureType::ColorImageRGBF32x3)
-            throw std::runtime_error("environment distribution needs an FP32 RGB image");
+        const bool half = im->texType == TextureType::ColorImageRGBA16Fx4;
+        if (im->texType != TextureType::ColorImageRGBF32x3 && !half)
+            throw std::runtime_error("environment distribution needs an FP32 RGB or RGBA16F image");
         const uint32_t sx = im->width, sy = im->height;
         const uint64_t texHead = scene->idxOfTex(image);
         const int32_t offData = im->offsetData;
@@ -513,10 +558,20 @@ public:
             c.offsetCDF = (int32_t)cCDFHead - (int32_t)childHead;
             float* cPDF = reinterpret_cast<float*>(od->data() + cPDFHead);
             float* cCDF = reinterpret_cast<float*>(od->data() + cCDFHead);
-            float corr = zenithCorrection ? std::cos((float)M_PI * (0.5f - (i + 0.5f) / sy)) : 1.0f;
-            for (uint32_t j = 0; j < sx; ++j) {
-                const float* p = px + 3 * ((size_t)i * sx + j);
-                cPDF[j] = (0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2]) * corr;
+            if (half) {
+                const uint16_t* hp = reinterpret_cast<const uint16_t*>(scene->texturesData.data() + texHead + offData);
+                const float yBase = ((float)im->height / sy) * i, yNext = ((float)im->height / sy) * (i + 1);
+                const float corr = zenithCorrection ? cosf(M_PI * (0.5f - (i + 0.5f) / sy)) : 1.0f;
+                for (uint32_t j = 0; j < sx; ++j) {
+                    const float xBase = ((float)im->width / sx) * j, xNext = ((float)im->width / sx) * (j + 1);
+                    cPDF[j] = averageLuminanceHalf(im, hp, xBase, xNext, yBase, yNext) * corr;
+                }
+            } else {
+                float corr = zenithCorrection ? std::cos((float)M_PI * (0.5f - (i + 0.5f) / sy)) : 1.0f;
+                for (uint32_t j = 0; j < sx; ++j) {
+                    const float* p = px + 3 * ((size_t)i * sx + j);
+                    cPDF[j] = (0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2]) * corr;
+                }
             }
             cCDF[0] = 0.0f;
             for (uint32_t j = 1; j <= sx; ++j) cCDF[j] = cCDF[j - 1] + cPDF[j - 1] * c.widthStratum;

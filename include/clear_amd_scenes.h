@@ -120,6 +120,36 @@ int clr_scene_diff_params(const clr_host_scene* scene, uint32_t* numMaterials, u
 /* Pointer and byte size of one host array (CLR_ARRAY_*); valid until free. */
 int clr_scene_array(const clr_host_scene* scene, int which, const void** data, uint64_t* bytes);
 
+/* ---- asset ingest --------------------------------------------------------
+ * Drop-ins for loadImage (ImageLoader.h:24, ImageLoader.cpp:208-232) and
+ * loadModel (ModelLoader.hpp:15, ModelLoader.cpp:156-508); the C++ API is
+ * include/clear_amd_loaders.hpp. */
+enum { CLR_COLOR_RGB8 = 0, CLR_COLOR_RGBA8 = 1, CLR_COLOR_RGBA16F = 2, CLR_COLOR_GRAY8 = 3 };
+
+/* Decodes a PNG or OpenEXR file: RGB8 as 4 bytes per pixel (the 4th 0xFF),
+ * RGBA8, RGBA16F (8 bytes per pixel) or gray8, rows top to bottom; PNG samples
+ * go through the reference's libpng gamma setting (gammaCorrection 0: file
+ * gamma undone; 1: samples as stored). With pixels == NULL only *bytes (the
+ * size) is returned; otherwise capacity must be at least that. The message
+ * of a failure is in clr_scene_last_error(). */
+int clr_image_load(const char* path, int gammaCorrection, uint32_t* width, uint32_t* height,
+                   uint32_t* colorType, void* pixels, uint64_t capacity, uint64_t* bytes);
+
+/* A scene from files: each OBJ model (with its MTL materials and their image
+ * textures) loaded by loadModel; model k emits emission[3k..3k+2] when that is
+ * nonzero (a diffuse light property given as the reference's lightOverride).
+ * envPath (OpenEXR, or NULL) becomes an image-based environment scaled by
+ * envMultiplier. The camera looks at the centre of the models' bounding box
+ * along -z from where the box's bounding sphere fills the field of view
+ * (params: image size, default 256x256; fovY, default 45; lens). */
+int clr_scene_build_files(const char* const* objPaths, const float* emission, uint32_t numModels,
+                          const char* envPath, float envMultiplier, const clr_scene_params* params,
+                          clr_host_scene** out);
+
+/* Message of the last failed clr_image_load / clr_scene_build_files of this
+ * thread ("" when none). */
+const char* clr_scene_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif
