@@ -8,6 +8,7 @@ FLOAT channels, hand-worked face lists for the OBJ fixture), and against the
 oracle's independent restatement (oracle/assets_oracle.py) on the same files.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -328,14 +329,86 @@ def test_obj_without_materials_and_errors(tmp_path):
         ca.HostScene.from_files([tmp_path / "e.obj"])
 
 
-def test_environment_from_exr(obj_dir):
-    """An OpenEXR environment adds one light at infinity and an image-based
-    environment record whose 2-D distribution is a proper distribution (CDFs
-    rising to 1)."""
-    h, w = 16, 32
-    env = {c: (1.0 + _smooth(h, w, k) ** 2).astype(np.float16) for k, c in enumerate("RGB")}
+def _env_distribution(host):
+    """(multiplier, marginal pdf, marginal cdf, conditional pdfs, conditional
+    cdfs) of the scene's image-based environment: others[1] -> environment
+    head -> light property -> ImageBasedEnvLElem -> ContinuousConsts2D_H."""
+    od, md = host.arrays["others"], host.arrays["materials"]
+    u32 = lambda b, o: int(b[o:o + 4].view(np.uint32)[0])
+    i32 = lambda b, o: int(b[o:o + 4].view(np.int32)[0])
+    f32 = lambda b, o, n=1: b[o:o + 4 * n].view(np.float32)
+    lp = u32(od, u32(od, 4))
+    assert md[lp] == 1  # one emission element
+    e = (lp + 1 + 3) // 4 * 4
+    assert md[e] == 0  # image based
+    mult, d = float(f32(md, e + 8)[0]), u32(md, e + 12)
+    assert od[d] == 2 and od[d + 8] == 1  # ContinuousConsts2D_H over a ContinuousConsts1D marginal
+    sy = u32(od, d + 12)
+    mpdf, mcdf = f32(od, d + i32(od, d + 28), sy), f32(od, d + i32(od, d + 32), sy + 1)
+    kids = d + i32(od, d + 4)
+    cp, cc = [], []
+    for i in range(sy):
+        c = kids + 28 * i
+        sx = u32(od, c + 4)
+        assert f32(od, c + 16)[0] == np.float32(1.0) / np.float32(sx)
+        cp.append(f32(od, c + i32(od, c + 20), sx))
+        cc.append(f32(od, c + i32(od, c + 24), sx + 1))
+    return mult, mpdf, mcdf, np.array(cp), np.array(cc)
+
+
+def test_environment_distribution_from_exr(obj_dir):
+    """An OpenEXR environment adds one light at infinity whose 2-D importance
+    distribution is, per row, the pixels' luminance (0.2126 R + 0.7152 G +
+    0.0722 B) normalised over the row, and over rows the row means times
+    sin(pi (i + 0.5) / H) (the solid angle of a latitude band) normalised;
+    every CDF rises to exactly 1."""
+    h, w = 12, 20
+    rng = _rng(8)
+    env = {c: (0.05 + rng.random((h, w)) * (1 + 3 * k)).astype(np.float16) for k, c in enumerate("RGB")}
     af.write_exr(obj_dir / "env.exr", env, af.ZIP)
     host = ca.HostScene.from_files([obj_dir / "a.obj"], env_path=obj_dir / "env.exr", env_multiplier=2.0)
     base = ca.HostScene.from_files([obj_dir / "a.obj"])
     assert host.num_lights == base.num_lights + 1
-    assert host.arrays["others"].size > base.arrays["others"].size
+    mult, mpdf, mcdf, cpdf, ccdf = _env_distribution(host)
+    assert mult == 2.0 and mpdf.size == h and cpdf.shape == (h, w)
+    R, G, B = (env[c].astype(np.float64) for c in "RGB")
+    lum = 0.2126 * R + 0.7152 * G + 0.0722 * B
+    np.testing.assert_allclose(cpdf, lum / lum.mean(axis=1, keepdims=True), rtol=1e-5)
+    band = lum.mean(axis=1) * np.sin(np.pi * (np.arange(h) + 0.5) / h)
+    np.testing.assert_allclose(mpdf, band / band.mean(), rtol=1e-5)
+    for cdf, pdf in [(mcdf, mpdf)] + list(zip(ccdf, cpdf)):
+        assert cdf[0] == 0.0 and cdf[-1] == 1.0 and (np.diff(cdf) >= 0).all()
+        np.testing.assert_allclose(np.diff(cdf), pdf / pdf.size, rtol=1e-4, atol=1e-7)
+
+
+def test_header_only_cpp_api(obj_dir, tmp_path):
+    """The C++ drop-in (include/clear_amd_loaders.hpp) compiles on its own
+    with g++ and zlib and loads the fixture as the library does."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "probe.cpp"
+    src.write_text("""
+#include <cstdio>
+#include "clear_amd_loaders.hpp"
+using namespace clr::host;
+int main(int argc, char** argv) {
+    Scene scene;
+    std::string err;
+    if (!loadModel(argv[1], &scene, UINT64_MAX, UINT64_MAX, &err)) { std::printf("error %s\\n", err.c_str()); return 1; }
+    std::vector<uint8_t> px; uint32_t w = 0, h = 0; ColorChannel::Value c;
+    if (!loadImage(argv[2], &px, &w, &h, &c, false, &err)) { std::printf("error %s\\n", err.c_str()); return 1; }
+    std::printf("%llu %llu %u %u %d %zu\\n", (unsigned long long)scene.numFaces(),
+                (unsigned long long)scene.numVertices(), w, h, (int)c, px.size());
+    return 0;
+}
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"), str(src), "-o", str(exe),
+                    "-lz"], check=True, capture_output=True)
+    out = subprocess.run([str(exe), str(obj_dir / "a.obj"), str(obj_dir / "tex.png")], check=True,
+                         capture_output=True, text=True).stdout.split()
+    assert out == [str(len(FACES_A)), "7", "6", "4", "1", str(6 * 4 * 4)]
