@@ -321,9 +321,18 @@ def cpu_lbvh(ca, soup, bits):
     c0 = time.perf_counter()
     orc.lbvh_build(verts, faces, bits)
     cdt = time.perf_counter() - c0
+    # The reference's own CPU build (Scene::build's midpoint BVH, timed at
+    # main.cpp:235-237), here the library's host restatement on the same soup.
+    log(f"cpu baseline: midpoint BVH build of {m} triangles")
+    c0 = time.perf_counter()
+    ca.build_midpoint_bvh(verts, faces)
+    mdt = time.perf_counter() - c0
     return {"value": round(m / cdt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
             "sample": f"oracle LBVH build (serial C++) of the same {m}-triangle soup, "
-                      f"{cdt:.1f} s; host threads available: {os.cpu_count()}"}
+                      f"{cdt:.1f} s; host threads available: {os.cpu_count()}",
+            "midpoint_bvh_build": {"value": round(m / mdt / 1e6, 3), "unit": "Mtri/s", "cores": 1,
+                                   "sample": f"clr_bvh_build_midpoint (the reference's CPU BVH::build "
+                                             f"split rule) on the same soup, {mdt:.1f} s"}}
 
 
 def cpu_render(ca, np, host, spp, depth):
