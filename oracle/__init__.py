@@ -245,6 +245,37 @@ def bxdf_probe(kind, p0, p1, R, wo, wi, u0, u1):
     return {"f": o[0:3], "pdf": o[3], "wi": o[4:7], "spdf": o[7], "sf": o[8:11]}
 
 
+def camera_ray(arrays, px, py, lu0, lu1):
+    """Origin, direction and direction pdf of the scene camera's ray for film
+    point (px, py) (raster units) and lens sample (lu0, lu1)."""
+    fn = _probe_fn("orc_camera_ray", [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_void_p])
+    keep, ptrs = _scene_ptrs(arrays, 1, None, None, None)
+    out = np.zeros(7, np.float32)
+    fn(ptrs, px, py, lu0, lu1, out.ctypes.data)
+    return out[0:3].astype(np.float64), out[3:6].astype(np.float64), float(out[6])
+
+
+def normal_map_probe(s_normal, u_dir, tangent, texel):
+    """Shading frame (N, S, T) after normal mapping a surface frame by a texel."""
+    fp = ctypes.POINTER(ctypes.c_float)
+    fn = _probe_fn("orc_normal_map_probe", [fp, fp, fp, fp, fp])
+    arr = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])
+    out = (ctypes.c_float * 9)()
+    fn(arr(s_normal), arr(u_dir), arr(tangent), arr(texel), out)
+    o = np.array(list(out), np.float64)
+    return o[0:3], o[3:6], o[6:9]
+
+
+def texture_probe(record: np.ndarray, u, v):
+    """evalColor of one texture record (bytes) at (u, v), as float32 [3]."""
+    fn = _probe_fn("orc_texture_probe", [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_void_p])
+    rec = np.ascontiguousarray(record, dtype=np.uint8)
+    out = np.zeros(3, np.float32)
+    fn(rec.ctypes.data, u, v, out.ctypes.data)
+    return out
+
+
 def fresnel_probe(ftype, eta, k, eta_ext, eta_int, cosi):
     fp = ctypes.POINTER(ctypes.c_float)
     fn = _probe_fn("orc_fresnel_probe", [ctypes.c_int, fp, fp, ctypes.c_float, ctypes.c_float,

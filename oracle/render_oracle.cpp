@@ -533,6 +533,29 @@ inline float cosPhi(V3 v) { float s = sinTh(v); if (s == 0.0f) return 1.0f; retu
 inline float sinPhi(V3 v) { float s = sinTh(v); if (s == 0.0f) return 0.0f; return clampf(v.y / s, -1.0f, 1.0f); }
 inline bool matchType(const BxDF& f, uint32_t mask) { return (f.type & mask) == f.type; }
 
+// Normal mapping (reflection.cl:201-223): the texel (rgb in [0, 1]) decodes
+// to a tangent-space normal nb = normalize(2 rgb - 1) in the frame (uDir,
+// vDir = uDir x sNormal, sNormal); N is that normal in world space, and the
+// tangent S is turned by the rotation that takes sNormal to N (axis
+// sNormal x N, normalised: D9), skipped when the angle's sine is <= 1e-4.
+void normalMapFrame(V3 sNormal, V3 uDir, V3 tex, V3* N, V3* S) {
+    const V3 nb = normalize(mk(2.0f * tex.x - 1.0f, 2.0f * tex.y - 1.0f, 2.0f * tex.z - 1.0f));
+    const V3 vDir = cross(uDir, sNormal);
+    *N = mk(dot(mk(uDir.x, vDir.x, sNormal.x), nb), dot(mk(uDir.y, vDir.y, sNormal.y), nb),
+            dot(mk(uDir.z, vDir.z, sNormal.z), nb));
+    V3 ax = cross(sNormal, *N);
+    const float sinTH = std::fmin(length(ax), 1.0f);
+    if (sinTH > 0.0001f) {
+        ax = normalize(ax);  // D9
+        const float cosTH = std::cos(std::asin(sinTH));
+        const float oneM = 1 - cosTH;
+        const V3 S0 = *S;
+        *S = mk(dot(mk(ax.x * ax.x * oneM + cosTH, ax.x * ax.y * oneM - ax.z * sinTH, ax.z * ax.x * oneM + ax.y * sinTH), S0),
+                dot(mk(ax.x * ax.y * oneM + ax.z * sinTH, ax.y * ax.y * oneM + cosTH, ax.y * ax.z * oneM - ax.x * sinTH), S0),
+                dot(mk(ax.z * ax.x * oneM - ax.y * sinTH, ax.y * ax.z * oneM + ax.x * sinTH, ax.z * ax.z * oneM + cosTH), S0));
+    }
+}
+
 void bsdfAlloc(const SceneC& s, uint32_t off, const SurfacePoint& sp, BSDF* B) {
     const uint8_t* m = s.materials + off;
     B->n = m[0];
@@ -540,22 +563,8 @@ void bsdfAlloc(const SceneC& s, uint32_t off, const SurfacePoint& sp, BSDF* B) {
     if (sp.hasTangent) B->S = sp.sTangent;
     else makeTangent(sp.sNormal, &B->S);
     if (m[1]) {  // normal map (reflection.cl:201-223)
-        V3 tex = evalColor(s.textures + rd<uint32_t>(m + 4), sp.uv[0], sp.uv[1]);
-        V3 nb = normalize(mk(2.0f * tex.x - 1.0f, 2.0f * tex.y - 1.0f, 2.0f * tex.z - 1.0f));
-        V3 vDir = cross(sp.uDir, sp.sNormal);
-        B->N = mk(dot(mk(sp.uDir.x, vDir.x, sp.sNormal.x), nb), dot(mk(sp.uDir.y, vDir.y, sp.sNormal.y), nb),
-                  dot(mk(sp.uDir.z, vDir.z, sp.sNormal.z), nb));
-        V3 ax = cross(sp.sNormal, B->N);
-        float sinTH = std::fmin(length(ax), 1.0f);
-        if (sinTH > 0.0001f) {
-            ax = normalize(ax);  // D9
-            float cosTH = std::cos(std::asin(sinTH));
-            float oneM = 1 - cosTH;
-            V3 S = B->S;
-            B->S = mk(dot(mk(ax.x * ax.x * oneM + cosTH, ax.x * ax.y * oneM - ax.z * sinTH, ax.z * ax.x * oneM + ax.y * sinTH), S),
-                      dot(mk(ax.x * ax.y * oneM + ax.z * sinTH, ax.y * ax.y * oneM + cosTH, ax.y * ax.z * oneM - ax.x * sinTH), S),
-                      dot(mk(ax.z * ax.x * oneM - ax.y * sinTH, ax.y * ax.z * oneM + ax.x * sinTH, ax.z * ax.z * oneM + cosTH), S));
-        }
+        const V3 tex = evalColor(s.textures + rd<uint32_t>(m + 4), sp.uv[0], sp.uv[1]);
+        normalMapFrame(sp.sNormal, sp.uDir, tex, &B->N, &B->S);
     }
     B->T = cross(B->N, B->S);
     B->NG = sp.gNormal;
@@ -1065,6 +1074,30 @@ V3 mulP4xyz(const uint8_t* m, float x, float y) {  // (x, y, 0, 1), xyz after th
     return {px, py, pz};
 }
 
+// Camera ray for film point (px, py) in raster units and lens sample
+// (lu0, lu1): sampleLensPos and sample_We (camera.cl:60-113). The lens point
+// is the concentric disk sample scaled by lensRadius in the camera's z = 0
+// plane; with a lens, the direction aims at the pinhole ray's point on the
+// plane z = -objPDistance (the focal plane). dirPDF = 1 / (cos^3 * area of the
+// image plane at unit distance).
+void cameraRay(const SceneC& s, float px, float py, float lu0, float lu1, V3* org, V3* rdir, float* dirPDF) {
+    const uint8_t* cam = s.camera;
+    const uint8_t* l2w = cam + 64;
+    const float lensRadius = rd<float>(cam + 136), objPD = rd<float>(cam + 140), vpa = rd<float>(cam + 132);
+    float ux, uy;
+    concentricSampleDisk(lu0, lu1, &ux, &uy);
+    const V3 lensLocal = mk(ux * lensRadius, uy * lensRadius, 0.0f);
+    *org = mulP3(l2w, lensLocal);
+    V3 dir = normalize(mulP4xyz(cam + 192, px, py));
+    if (lensRadius > 0.0f) {
+        const float ft = -objPD / dir.z;
+        const V3 pf = mk(0, 0, 0) + ft * dir;
+        dir = normalize(pf - mk(lensLocal.x, lensLocal.y, 0.0f));
+    }
+    *rdir = mulV3(l2w, dir);
+    *dirPDF = 1.0f / (std::pow(-dir.z, 3.0f) * vpa);
+}
+
 struct RenderArgs {
     uint32_t width, height, maxDepth;
     bool noRR;  // CLR_RENDER_NO_RR
@@ -1081,27 +1114,14 @@ struct NoHook {
 template <class Hook = NoHook>
 V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, uint32_t* rds, Hook& hook) {
     V3 pix = mk(0, 0, 0);
-    const uint8_t* cam = s.camera;
-    const uint8_t* l2w = cam + 64;
-    float lensRadius = rd<float>(cam + 136), objPD = rd<float>(cam + 140), vpa = rd<float>(cam + 132);
-    // sampleLensPos
-    float cu0 = getFloat(rds), cu1 = getFloat(rds);
-    float ux, uy;
-    concentricSampleDisk(cu0, cu1, &ux, &uy);
-    V3 idfDir = mulV3(l2w, mk(0.0f, 0.0f, 1.0f));
-    V3 lensLocal = mk(ux * lensRadius, uy * lensRadius, 0.0f);
-    V3 org = mulP3(l2w, lensLocal);
-    // sample_We
-    float px = (float)gx + getFloat(rds);
-    float py = (float)gy + getFloat(rds);
-    V3 dir = normalize(mulP4xyz(cam + 192, px, py));
-    if (lensRadius > 0.0f) {
-        float ft = -objPD / dir.z;
-        V3 pf = mk(0, 0, 0) + ft * dir;
-        dir = normalize(pf - mk(lensLocal.x, lensLocal.y, 0.0f));
-    }
-    V3 rdir = mulV3(l2w, dir);
-    float dirPDF = 1.0f / (std::pow(-dir.z, 3.0f) * vpa);
+    // sampleLensPos draws the lens sample, sample_We the film jitter.
+    const float cu0 = getFloat(rds), cu1 = getFloat(rds);
+    const float px = (float)gx + getFloat(rds);
+    const float py = (float)gy + getFloat(rds);
+    V3 org, rdir;
+    float dirPDF;
+    cameraRay(s, px, py, cu0, cu1, &org, &rdir, &dirPDF);
+    const V3 idfDir = mulV3(s.camera + 64, mk(0.0f, 0.0f, 1.0f));
     V3 alpha = mk(1, 1, 1) * (std::fabs(dot(idfDir, rdir)) / dirPDF);
     float initY = luminance(alpha);
 
@@ -1376,20 +1396,8 @@ int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t wi
 // The ray from the lens centre (lens sample 0.5, 0.5) through the centre of
 // pixel (gx, gy) (sampleLensPos, sample_We; camera.cl:60-113).
 void centreRay(const SceneC& s, uint32_t gx, uint32_t gy, V3* org, V3* rdir) {
-    const uint8_t* cam = s.camera;
-    const uint8_t* l2w = cam + 64;
-    const float lensRadius = rd<float>(cam + 136), objPD = rd<float>(cam + 140);
-    float ux, uy;
-    concentricSampleDisk(0.5f, 0.5f, &ux, &uy);
-    const V3 lensLocal = mk(ux * lensRadius, uy * lensRadius, 0.0f);
-    *org = mulP3(l2w, lensLocal);
-    V3 dir = normalize(mulP4xyz(cam + 192, gx + 0.5f, gy + 0.5f));
-    if (lensRadius > 0.0f) {
-        const float ft = -objPD / dir.z;
-        const V3 pf = mk(0, 0, 0) + ft * dir;
-        dir = normalize(pf - mk(lensLocal.x, lensLocal.y, 0.0f));
-    }
-    *rdir = mulV3(l2w, dir);
+    float dirPDF;
+    cameraRay(s, gx + 0.5f, gy + 0.5f, 0.5f, 0.5f, org, rdir, &dirPDF);
 }
 
 // Those rays for every pixel as float[7][W * H] SoA planes (tmax = inf), so
@@ -1579,6 +1587,41 @@ int orc_fresnel_probe(int type, const float* eta, const float* k, float etaExt, 
     }
     V3 r = evalFresnel(buf, cosi);
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    return 0;
+}
+
+// Camera probe: origin (3), direction (3) and direction pdf of the camera
+// ray for film point (px, py) and lens sample (lu0, lu1) of the scene's camera.
+int orc_camera_ray(const void* const* arrays, float px, float py, float lu0, float lu1, float* out) {
+    SceneC s{};
+    s.others = (const uint8_t*)arrays[8];
+    s.camera = s.others + rd<uint32_t>(s.others + 0);
+    V3 o, d;
+    float pdf;
+    cameraRay(s, px, py, lu0, lu1, &o, &d, &pdf);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z;
+    out[3] = d.x; out[4] = d.y; out[5] = d.z;
+    out[6] = pdf;
+    return 0;
+}
+
+// Normal-map probe: shading normal N, tangent S and T = N x S for a surface
+// frame (sNormal, uDir, tangent S) and a texel colour.
+int orc_normal_map_probe(const float* sNormal, const float* uDir, const float* S, const float* tex, float* out) {
+    V3 N{}, S1 = mk(S[0], S[1], S[2]);
+    normalMapFrame(mk(sNormal[0], sNormal[1], sNormal[2]), mk(uDir[0], uDir[1], uDir[2]),
+                   mk(tex[0], tex[1], tex[2]), &N, &S1);
+    const V3 T = cross(N, S1);
+    out[0] = N.x; out[1] = N.y; out[2] = N.z;
+    out[3] = S1.x; out[4] = S1.y; out[5] = S1.z;
+    out[6] = T.x; out[7] = T.y; out[8] = T.z;
+    return 0;
+}
+
+// Texture probe: evalColor of one texture record (texture.cl) at (u, v).
+int orc_texture_probe(const uint8_t* record, float u, float v, float* out) {
+    const V3 c = evalColor(record, u, v);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
     return 0;
 }
 

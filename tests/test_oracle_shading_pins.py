@@ -1,0 +1,347 @@
+"""Closed-form pins of the render oracle's Ward and Ashikhmin-Shirley values,
+thin-lens camera, normal-map frame and RGBA16F texture lookup
+(oracle/render_oracle.cpp, following reflection.cl, camera.cl and texture.cl).
+
+Each check comes from the mathematics of the model rather than from its code:
+albedo integrals that reduce to closed forms, density normalisations, the
+focusing property of a thin lens, rotation invariants, and IEEE binary16
+decoding as numpy performs it. A dropped factor, a wrong power, a swapped
+axis or a transposed rotation fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import clear_amd as ca
+import oracle as orc
+
+N_UP = [0.0, 0.0, 1.0]
+
+
+def _dir(theta, phi):
+    return [math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta)]
+
+
+def _albedo_normal_incidence(kind, p0, p1, R, n=6000, phi=0.7):
+    """integral over the upper hemisphere of f(n, wi) cos(theta_i) dwi for an
+    isotropic lobe, by the midpoint rule in theta_h = theta_i / 2 (the
+    half-vector angle when wo is the normal), which resolves narrow lobes."""
+    th_h = (np.arange(n) + 0.5) * (math.pi / 4) / n
+    tot = 0.0
+    for t in th_h:
+        ti = 2.0 * t
+        f = orc.bxdf_probe(kind, p0, p1, R, N_UP, _dir(ti, phi), 0.5, 0.5)["f"][0]
+        tot += f * math.cos(ti) * math.sin(ti)
+    return 2.0 * math.pi * tot * 2.0 * (math.pi / 4) / n
+
+
+def _pdf_integral_normal_incidence(kind, p0, p1, n=6000, phi=0.7):
+    th_h = (np.arange(n) + 0.5) * (math.pi / 4) / n
+    tot = 0.0
+    for t in th_h:
+        ti = 2.0 * t
+        tot += orc.bxdf_probe(kind, p0, p1, [1, 1, 1], N_UP, _dir(ti, phi), 0.5, 0.5)["pdf"] * math.sin(ti)
+    return 2.0 * math.pi * tot * 2.0 * (math.pi / 4) / n
+
+
+def _albedo_oblique(kind, p0, p1, theta_o, th_max, n_th=1500, n_ph=48):
+    """integral of f(wo, wi) cos(theta_i) dwi (R = 1) for wo at theta_o, over
+    half vectors h with theta_h <= th_max: wi = 2 (wo.h) h - wo and
+    dwi = 4 (wo.h) dwh (midpoint rule in theta_h and phi_h)."""
+    wo = np.array(_dir(theta_o, 0.0))
+    dth, dph = th_max / n_th, 2 * math.pi / n_ph
+    tot = 0.0
+    for i in range(n_th):
+        th = (i + 0.5) * dth
+        for j in range(n_ph):
+            h = np.array(_dir(th, (j + 0.5) * dph))
+            c = float(np.dot(wo, h))
+            wi = 2 * c * h - wo
+            if c <= 0 or wi[2] <= 0:
+                continue
+            f = orc.bxdf_probe(kind, p0, p1, [1, 1, 1], wo.tolist(), wi.tolist(), 0.5, 0.5)["f"][0]
+            tot += f * wi[2] * 4 * c * math.sin(th)
+    return tot * dth * dph
+
+
+@pytest.mark.parametrize("kind,p0,theta_o,th_max", [(3, 0.02, 1.0, 0.15), (3, 0.02, 0.5, 0.15),
+                                                    (4, 4000.0, 1.0, 0.12), (4, 4000.0, 0.5, 0.12)])
+def test_narrow_lobes_conserve_energy_at_oblique_incidence(kind, p0, theta_o, th_max):
+    """Narrow lobes at oblique incidence, R = 1. With unit-mass half-vector
+    densities p(h), Duer's Ward gives f cos_i dwi = p(h) cos_i / ((h.wo) cos_h) dh
+    and Ashikhmin-Shirley f cos_i dwi = p(h) F cos_i / max(cos_i, cos_o) dh.
+    For h tilted by delta at azimuth phi from the normal, cos_i ~ cos_o +
+    2 delta cos(phi) sin(theta_o). Ward's ratio is 1 + delta cos(phi) tan(theta_o)
+    + O(delta^2): the albedo is 1 - O(alpha^2 / cos^2) (alpha = 0.02). The A-S
+    ratio falls below 1 only where cos_i < cos_o, by 2 delta |cos phi|
+    tan(theta_o); with delta Rayleigh(1 / sqrt(n)) that averages to
+    tan(theta_o) sqrt(2 / (pi n)) at first order. A lost cos, (h.wi) or max
+    factor moves either by far more than the tolerance."""
+    got = _albedo_oblique(kind, p0, p0, theta_o, th_max)
+    want = 1.0 if kind == 3 else 1.0 - math.tan(theta_o) * math.sqrt(2.0 / (math.pi * p0))
+    assert got == pytest.approx(want, abs=2.5e-3)
+
+
+# ---------------------------------------------------------------- Ward ----
+@pytest.mark.parametrize("alpha", [0.1, 0.2, 0.35])
+def test_ward_albedo_at_normal_incidence(alpha):
+    """Ward lobe with Duer's normalisation (reflection.cl NewWard): with wo at
+    the normal, wi is the mirror of wo about h, tan^2(theta_h) / alpha^2 is
+    Exp(1)-distributed under the lobe's half-vector density, and
+    f cos(theta_i) dwi = R (1 - tan^2 theta_h) p(h) dh, so the albedo is
+    R (1 - E[u; u <= 1]) for u ~ Exp(mean alpha^2) cut at theta_i = 90 deg:
+    R (1 - alpha^2 + alpha^2 exp(-1 / alpha^2))."""
+    R = 0.8
+    got = _albedo_normal_incidence(3, alpha, alpha, [R, R, R])
+    a2 = alpha * alpha
+    want = R * (1.0 - a2 + a2 * math.exp(-1.0 / a2))
+    assert got == pytest.approx(want, rel=2e-4)
+
+
+@pytest.mark.parametrize("alpha", [0.1, 0.3])
+def test_ward_pdf_mass_at_normal_incidence(alpha):
+    """The half-vector density exp(-tan^2/alpha^2) / (pi alpha^2 cos^3) has unit
+    mass; mapped to wi with Jacobian 1 / (4 h.wi), the part with theta_i < 90
+    deg (tan^2 theta_h < 1) is 1 - exp(-1 / alpha^2)."""
+    got = _pdf_integral_normal_incidence(3, alpha, alpha)
+    assert got == pytest.approx(1.0 - math.exp(-1.0 / alpha ** 2), rel=2e-4)
+
+
+def test_ward_anisotropy_axes():
+    """anisoX scales the tangent (x) axis: the lobe is narrower along x when
+    alphaX < alphaY, by exp(-tan^2(theta_h) (1/ax^2 - 1/ay^2)) for equal tilts."""
+    ax, ay, th = 0.1, 0.4, 0.3
+    fx = orc.bxdf_probe(3, ax, ay, [1, 1, 1], N_UP, _dir(th, 0.0), 0.5, 0.5)["f"][0]
+    fy = orc.bxdf_probe(3, ax, ay, [1, 1, 1], N_UP, _dir(th, math.pi / 2), 0.5, 0.5)["f"][0]
+    t2 = math.tan(th / 2) ** 2
+    assert fx < fy
+    assert fx / fy == pytest.approx(math.exp(-t2 * (1 / ax ** 2 - 1 / ay ** 2)), rel=1e-4)
+
+
+# ---------------------------------------------------- Ashikhmin-Shirley ----
+def _as_spec_albedo(n, R):
+    """Albedo of the Ashikhmin-Shirley specular lobe at normal incidence with
+    nu = nv = n and Schlick's F(mu) = R + (1 - R)(1 - mu)^5, mu = cos(theta_h):
+    f = (n + 1) mu^(n - 1) F / (8 pi) there, and the albedo reduces to
+    (n + 1) * integral_{1/sqrt2}^{1} mu^n (2 mu^2 - 1) F(mu) dmu."""
+    P = np.polynomial.Polynomial
+    mu = P([0.0, 1.0])
+    F = R + (1.0 - R) * (1.0 - mu) ** 5
+    integrand = (mu ** n) * (2.0 * mu ** 2 - 1.0) * F
+    anti = integrand.integ()
+    return (n + 1) * (anti(1.0) - anti(1.0 / math.sqrt(2.0)))
+
+
+@pytest.mark.parametrize("n,R", [(20.0, 1.0), (100.0, 1.0), (40.0, 0.5), (10.0, 0.2)])
+def test_ashikhmin_specular_albedo_at_normal_incidence(n, R):
+    if R == 1.0:  # the closed form without Fresnel
+        want = 2 * (n + 1) / (n + 3) * (1 - 2 ** (-(n + 3) / 2)) - (1 - 2 ** (-(n + 1) / 2))
+        assert _as_spec_albedo(n, 1.0) == pytest.approx(want, rel=1e-9)
+    got = _albedo_normal_incidence(4, n, n, [R, R, R])
+    assert got == pytest.approx(_as_spec_albedo(n, R), rel=3e-4)
+
+
+@pytest.mark.parametrize("n", [10.0, 80.0])
+def test_ashikhmin_pdf_mass_at_normal_incidence(n):
+    """The half-vector density (n + 1) cos^n(theta_h) / (2 pi) has unit mass;
+    the part that maps above the horizon is 1 - 2^(-(n + 1) / 2)."""
+    got = _pdf_integral_normal_incidence(4, n, n)
+    assert got == pytest.approx(1.0 - 2.0 ** (-(n + 1) / 2), rel=2e-4)
+
+
+def test_ashikhmin_anisotropy_axes():
+    """nu is the exponent along the tangent (x): with nu > nv the lobe is
+    narrower along x."""
+    nu, nv, th = 80.0, 20.0, 0.3
+    fx = orc.bxdf_probe(4, nu, nv, [1, 1, 1], N_UP, _dir(th, 0.0), 0.5, 0.5)["f"][0]
+    fy = orc.bxdf_probe(4, nu, nv, [1, 1, 1], N_UP, _dir(th, math.pi / 2), 0.5, 0.5)["f"][0]
+    assert fx / fy == pytest.approx(math.cos(th / 2) ** (nu - nv), rel=1e-4)
+
+
+@pytest.mark.parametrize("R", [0.3, 0.7])
+@pytest.mark.parametrize("theta_o", [0.0, 0.6, 1.2])
+def test_ashikhmin_diffuse_albedo(R, theta_o):
+    """The diffuse term 28 Rd / (23 pi) (1 - Rs) A(o) A(i), A = 1 - (1 - cos/2)^5,
+    has directional albedo Rd (1 - Rs) A(o): 2 * int_0^1 A(mu) mu dmu = 23/28
+    cancels the constant. Gauss-Legendre in mu is exact for this degree-6
+    polynomial. (The probe uses R for both Rd and Rs.)"""
+    x, w = np.polynomial.legendre.leggauss(12)
+    mu = 0.5 * (x + 1.0)
+    wo = _dir(theta_o, 0.2)
+    tot = 0.0
+    for m, wt in zip(mu, w):
+        f = orc.bxdf_probe(5, 0, 0, [R, R, R], wo, _dir(math.acos(m), 1.1), 0.5, 0.5)["f"][0]
+        tot += 0.5 * wt * f * m
+    got = 2.0 * math.pi * tot
+    A = 1.0 - (1.0 - math.cos(theta_o) / 2.0) ** 5
+    assert got == pytest.approx(R * (1.0 - R) * A, rel=2e-5)
+
+
+# -------------------------------------------------------------- camera ----
+W, H = 160, 96
+EYE = np.array([0.0, 0.0, 2.8])
+TARGET = np.array([0.0, -0.2, 0.0])
+FOV_Y = math.radians(50.0)  # the showcase scene's camera (eye, target, 50 deg vertical)
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def test_pinhole_camera_geometry():
+    host = ca.HostScene(ca.SCENE_SHOWCASE, W, H)
+    axis = _unit(TARGET - EYE)
+    up = np.array([0.0, 1.0, 0.0])
+    right = _unit(np.cross(axis, up))
+    true_up = np.cross(right, axis)
+    o, d, _ = orc.camera_ray(host.arrays, W / 2, H / 2, 0.1, 0.9)
+    np.testing.assert_allclose(o, EYE, atol=1e-6)
+    np.testing.assert_allclose(_unit(d), axis, atol=2e-6)
+    for lu in [(0.0, 0.0), (0.3, 0.8), (1.0, 1.0)]:  # no lens: every ray leaves the eye
+        o2, _, _ = orc.camera_ray(host.arrays, 17.25, 80.5, *lu)
+        np.testing.assert_allclose(o2, EYE, atol=1e-6)
+    # film row 0 is the top edge: fovY / 2 above the axis
+    _, dt, _ = orc.camera_ray(host.arrays, W / 2, 0.0, 0.5, 0.5)
+    assert math.acos(np.clip(np.dot(_unit(dt), axis), -1, 1)) == pytest.approx(FOV_Y / 2, abs=2e-6)
+    assert np.dot(dt, true_up) > 0
+    # film column W is the right edge: atan(aspect tan(fovY / 2)) right of the axis
+    _, dr, _ = orc.camera_ray(host.arrays, float(W), H / 2, 0.5, 0.5)
+    half_x = math.atan(W / H * math.tan(FOV_Y / 2))
+    assert math.acos(np.clip(np.dot(_unit(dr), axis), -1, 1)) == pytest.approx(half_x, abs=2e-6)
+    assert np.dot(dr, right) > 0
+
+
+@pytest.mark.parametrize("radius,focus", [(0.05, 2.0), (0.2, 3.5)])
+def test_thin_lens_focuses_on_a_plane(radius, focus):
+    """Every lens point sends the ray of one film point through the same point
+    of the plane at depth `focus` along the axis: the pinhole ray's point there.
+    Lens points lie in the plane through the eye normal to the axis, within
+    `radius`; the concentric map's rim samples land on the rim in every
+    direction."""
+    pin = ca.HostScene(ca.SCENE_SHOWCASE, W, H)
+    lens = ca.HostScene(ca.SCENE_SHOWCASE, W, H, lens_radius=radius, focus_distance=focus)
+    axis = _unit(TARGET - EYE)
+    rng = np.random.default_rng(5)
+    for _ in range(12):
+        px, py = rng.uniform(0, W), rng.uniform(0, H)
+        o0, d0, _ = orc.camera_ray(pin.arrays, px, py, 0.5, 0.5)
+        p_focus = o0 + d0 * (focus / np.dot(d0, axis))
+        for _ in range(6):
+            lu = rng.uniform(0, 1, 2)
+            o, d, _ = orc.camera_ray(lens.arrays, px, py, *lu)
+            off = o - EYE
+            assert abs(np.dot(off, axis)) < 1e-6
+            assert np.linalg.norm(off) <= radius * (1 + 1e-5)
+            p = o + d * ((focus - np.dot(off, axis)) / np.dot(d, axis))
+            np.testing.assert_allclose(p, p_focus, atol=2e-5 * focus)
+    for lu in [(1.0, 0.5), (0.0, 0.5), (0.5, 1.0), (0.5, 0.0), (1.0, 1.0)]:  # the disk's rim
+        o_rim, _, _ = orc.camera_ray(lens.arrays, W / 2, H / 2, *lu)
+        assert np.linalg.norm(o_rim - EYE) == pytest.approx(radius, rel=1e-5)
+
+
+# ---------------------------------------------------------- normal map ----
+def _frame(seed):
+    rng = np.random.default_rng(seed)
+    n = _unit(rng.normal(size=3))
+    u = _unit(np.cross(n, rng.normal(size=3)))
+    return n, u
+
+
+def test_normal_map_decode_axes():
+    """Texel (r, g, b) decodes to 2 rgb - 1 in the frame (uDir, uDir x sNormal,
+    sNormal) (reflection.cl:201-210)."""
+    n, u = _frame(1)
+    v = np.cross(u, n)
+    for tex, want in [((0.5, 0.5, 1.0), n), ((1.0, 0.5, 0.5), u), ((0.5, 1.0, 0.5), v),
+                      ((0.5, 0.0, 0.5), -v)]:
+        N, _, _ = orc.normal_map_probe(n, u, u, tex)
+        np.testing.assert_allclose(N, want, atol=2e-6)
+
+
+def test_normal_map_flat_texel_keeps_the_frame():
+    n, u = _frame(2)
+    N, S, T = orc.normal_map_probe(n, u, u, (0.5, 0.5, 1.0))
+    np.testing.assert_allclose(N, n, atol=1e-6)
+    np.testing.assert_allclose(S, u, atol=1e-6)
+    np.testing.assert_allclose(T, np.cross(n, u), atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_normal_map_rotation_takes_the_normal_to_the_mapped_normal(seed):
+    """The tangent turns by the rotation R about sNormal x N through the angle
+    between them (Rodrigues): R sNormal = N, R leaves the axis alone, and R
+    keeps the tangent orthogonal to the new normal, so (N, S, T) stays an
+    orthonormal right-handed frame."""
+    n, u = _frame(100 + seed)
+    rng = np.random.default_rng(seed)
+    tex = np.concatenate([rng.uniform(0.1, 0.9, 2), rng.uniform(0.75, 1.0, 1)])
+    N, S, T = orc.normal_map_probe(n, u, u, tex)
+    Nn, Sn, _ = orc.normal_map_probe(n, u, n, tex)  # rotate sNormal itself
+    np.testing.assert_allclose(Sn, N, atol=3e-6)
+    axis = _unit(np.cross(n, N))
+    _, Sa, _ = orc.normal_map_probe(n, u, axis, tex)
+    np.testing.assert_allclose(Sa, axis, atol=3e-6)
+    for a, b in [(N, S), (N, T), (S, T)]:
+        assert abs(np.dot(a, b)) < 3e-6
+    for a in (N, S, T):
+        assert np.linalg.norm(a) == pytest.approx(1.0, abs=3e-6)
+    assert np.dot(np.cross(N, S), T) == pytest.approx(1.0, abs=1e-5)
+    # a rotation preserves angles: S makes with N the angle u made with n (90 deg)
+    # and with the axis the angle u made with the axis
+    assert np.dot(S, axis) == pytest.approx(np.dot(u, axis), abs=3e-6)
+
+
+# ------------------------------------------------------------- texture ----
+def _image_record(tex_type, w, h, texel_bytes):
+    """ImageTexture record (texType, 3 pad bytes, width, height, offsetData)
+    followed by its texels (texture.cl's image layout, offsetData = 16)."""
+    head = np.zeros(16, np.uint8)
+    head[0] = tex_type
+    head[4:16] = np.frombuffer(np.array([w, h, 16], np.uint32).tobytes(), np.uint8)
+    return np.concatenate([head, np.frombuffer(texel_bytes, np.uint8)])
+
+
+def test_rgba16f_texture_decodes_every_half():
+    """Type 3 (RGBA16F): every one of the 65536 binary16 patterns, placed in
+    the RGB channels of a 128 x 171 image, reads back as numpy's IEEE
+    half -> float conversion at the texel's centre (bit-exact; NaN stays NaN)."""
+    w, h = 128, 171
+    halves = np.zeros((h * w, 4), np.uint16)
+    flat = np.arange(65536, dtype=np.uint32)
+    rgb = np.zeros(h * w * 3, np.uint16)
+    rgb[:65536] = flat
+    halves[:, :3] = rgb.reshape(-1, 3)
+    halves[:, 3] = 0x3C00  # alpha 1, ignored by the colour lookup
+    rec = _image_record(3, w, h, halves.tobytes())
+    want = halves[:, :3].view(np.float16).astype(np.float32)
+    got = np.zeros_like(want)
+    for y in range(h):
+        for x in range(w):
+            got[y * w + x] = orc.texture_probe(rec, (x + 0.5) / w, (y + 0.5) / h)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_image_texture_addressing():
+    """u picks the column and v the row (row-major from the first row); the
+    coordinate wraps with period 1 (fmod), and a negative coordinate reads
+    texel 0 (texture.cl's texel index)."""
+    w, h = 5, 3
+    texels = np.zeros((h, w, 4), np.uint16)
+    vals = np.arange(w * h, dtype=np.float16).reshape(h, w)
+    texels[:, :, 0] = vals.view(np.uint16)
+    rec = _image_record(3, w, h, texels.tobytes())
+    for y in range(h):
+        for x in range(w):
+            u, v = (x + 0.5) / w, (y + 0.5) / h
+            assert orc.texture_probe(rec, u, v)[0] == float(vals[y, x])
+            assert orc.texture_probe(rec, u + 1.0, v + 2.0)[0] == float(vals[y, x])
+    assert orc.texture_probe(rec, -0.3, 0.5)[0] == float(vals[1, 0])
+    # RGB8 (type 1): bytes / 255
+    rgb8 = np.arange(w * h * 4, dtype=np.uint8).reshape(h, w, 4)
+    rec8 = _image_record(1, w, h, rgb8.tobytes())
+    got = orc.texture_probe(rec8, 2.5 / w, 1.5 / h)
+    np.testing.assert_array_equal(got, rgb8[1, 2, :3].astype(np.float32) / np.float32(255.0))
