@@ -152,67 +152,23 @@ __global__ __launch_bounds__(kBlock) void k_init(uint32_t* words, size_t count,
 }
 
 // -------------------------------------------------------------- bounds ----
-// SOUP: reads tris; else gathers faces + vertices and writes the box array.
+// One pass over the triangles (soup, or faces gathering vertices): per
+// triangle its 32-B box record and its box centre (SoA, three floats) for the
+// Morton pass, and the block's min / max towards the scene box. The centre is
+// (lo + hi) * 0.5 per axis, the value calcMortonCodes quantises
+// (lbvh_construction.cl:169-181), so the Morton pass reads 12 B per triangle
+// instead of the triangle again.
 template <bool SOUP>
 __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tris,
                                                    const clr_float3* __restrict__ verts,
                                                    const clr_face* __restrict__ faces, uint32_t n,
                                                    float4* __restrict__ boxesOut,
+                                                   float* __restrict__ cx, float* __restrict__ cy,
+                                                   float* __restrict__ cz,
                                                    uint32_t* __restrict__ sceneOrd) {
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    // A 16-B aligned soup is read as a flat float4 array, lane k of a wave
-    // reading float4 k (whole lines per load). Component c of float4 f holds
-    // axis (f + c) % 3; with a grid stride that is a multiple of 3, f % 3 is
-    // the same for every float4 a thread reads, so four accumulators per bound
-    // are folded into the three axes once at the end. The scene box is the
-    // min / max over all vertices, which equals the union of the triangle
-    // boxes.
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    const bool vec = SOUP && (reinterpret_cast<uintptr_t>(tris) & 15u) == 0 && stride % 3 == 0;
-    if (vec) {
-        const float4* t4 = reinterpret_cast<const float4*>(tris);
-        const size_t floats = 9 * (size_t)n;
-        const size_t nf4 = floats / 4;
-        const size_t f0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
-        float l4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-        float h4[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        auto acc = [&](const float4 v) {
-            l4[0] = omin(l4[0], v.x); l4[1] = omin(l4[1], v.y); l4[2] = omin(l4[2], v.z); l4[3] = omin(l4[3], v.w);
-            h4[0] = omax(h4[0], v.x); h4[1] = omax(h4[1], v.y); h4[2] = omax(h4[2], v.z); h4[3] = omax(h4[3], v.w);
-        };
-        size_t f = f0;
-        for (; f + 3 * stride < nf4; f += 4 * stride) {
-            const float4 a = t4[f], b = t4[f + stride], c = t4[f + 2 * stride], d = t4[f + 3 * stride];
-            acc(a);
-            acc(b);
-            acc(c);
-            acc(d);
-        }
-        for (; f < nf4; f += stride) acc(t4[f]);
-        const uint32_t r = (uint32_t)(f0 % 3);
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if ((r + (uint32_t)c) % 3u == (uint32_t)ax) {
-                    lo[ax] = omin(lo[ax], l4[c]);
-                    hi[ax] = omax(hi[ax], h4[c]);
-                }
-        // The last floats % 4 floats; constant indices keep lo / hi in registers.
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            for (size_t q = nf4 * 4; q < floats; ++q) {
-                const float v = tris[q];
-                const uint32_t ax = (uint32_t)(q % 3);
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    if (ax == (uint32_t)k) {
-                        lo[k] = omin(lo[k], v);
-                        hi[k] = omax(hi[k], v);
-                    }
-            }
-    }
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; !vec && i < n; i += gridDim.x * kBlock) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         Box b;
         if constexpr (SOUP) {
             b = soupBox(tris, i);
@@ -224,8 +180,11 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
             float p0[3] = {v0.x, v0.y, v0.z}, p1[3] = {v1.x, v1.y, v1.z},
                   p2[3] = {v2.x, v2.y, v2.z};
             b = boxOf3(p0, p1, p2);
-            storeBoxRec(boxesOut, i, b, i);
         }
+        storeBoxRec(boxesOut, i, b, i);
+        cx[i] = (b.lo[0] + b.hi[0]) * 0.5f;
+        cy[i] = (b.lo[1] + b.hi[1]) * 0.5f;
+        cz[i] = (b.lo[2] + b.hi[2]) * 0.5f;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             lo[k] = omin(lo[k], b.lo[k]);
@@ -308,10 +267,11 @@ __device__ __forceinline__ uint32_t digitExclusiveScan(uint32_t v, uint32_t* s_t
     return pre + x - v;
 }
 
-// Keys, digit histograms of every pass and (soup input) the box records, in
-// input order.
-template <bool SOUP, typename KeyT>
-__global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint32_t bits,
+// Keys from the box centres, the hierarchy's arrival words, and the digit
+// histograms of every radix pass, in input order.
+template <typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ cx, const float* __restrict__ cy,
+                                                   const float* __restrict__ cz, uint32_t n, uint32_t bits,
                                                    const uint32_t* __restrict__ sceneOrd,
                                                    KeyT* __restrict__ keys,
                                                    uint32_t* __restrict__ otherEnd,
@@ -328,14 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint
     }
     const uint32_t numDivs = 1u << bits;
     const float fDivs = (float)numDivs;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        Box b = loadBox<SOUP>(src, i);
-        if constexpr (SOUP) storeBoxRec(src.boxes, i, b, i);
+    auto emit = [&](uint32_t i, const float* c) {
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            float c = (b.lo[k] + b.hi[k]) * 0.5f;
-            float np = (c - smin[k]) / ssize[k];
+            float np = (c[k] - smin[k]) / ssize[k];
             q[k] = quantize(np * fDivs, numDivs);
         }
         KeyT key = spread3<KeyT>(q[0]) | (spread3<KeyT>(q[1]) << 1) | (spread3<KeyT>(q[2]) << 2);
@@ -343,6 +300,25 @@ __global__ __launch_bounds__(kBlock) void k_morton(PrimSrc src, uint32_t n, uint
         if (i + 1 < n) otherEnd[i] = kEmpty;
         for (int p = 0; p < numPasses; ++p)
             atomicAdd(&s_hist[p * 256 + (uint32_t)((key >> (8 * p)) & 0xFFu)], 1u);
+    };
+    // Four triangles per thread per round, their centres loaded up front so
+    // twelve loads are in flight before the first key is formed.
+    const uint32_t stride = gridDim.x * kBlock;
+    uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
+    for (; i0 + 3 * stride < n; i0 += 4 * stride) {
+        float cc[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cc[u][0] = cx[i0 + u * stride];
+            cc[u][1] = cy[i0 + u * stride];
+            cc[u][2] = cz[i0 + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) emit(i0 + u * stride, cc[u]);
+    }
+    for (uint32_t i = i0; i < n; i += stride) {
+        const float c[3] = {cx[i], cy[i], cz[i]};
+        emit(i, c);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < numPasses * 256; j += kBlock)
@@ -1088,13 +1064,18 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
              stream>>>(L.hist, L.clearWords, L.sceneOrd);
     // One set of same-address atomics per block: a few blocks per CU.
     const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
-    // The soup path reads float4s with a grid stride that is a multiple of 3.
-    const unsigned boundsGrid = SOUP ? std::max(3u, gridFor(n, reduceGrid) / 3u * 3u) : gridFor(n, reduceGrid);
-    k_bounds<SOUP><<<boundsGrid, kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n, L.boxes, L.sceneOrd);
+    // Box centres live in buffers the first radix pass has not written yet:
+    // the second key buffer and both payload buffers (pass 0 generates its
+    // payload instead of reading it).
+    float* cx = reinterpret_cast<float*>(L.keysB);
+    float* cy = reinterpret_cast<float*>(L.valsA);
+    float* cz = reinterpret_cast<float*>(L.valsB);
+    k_bounds<SOUP><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n, L.boxes, cx,
+                                                                  cy, cz, L.sceneOrd);
     ev(1);
     PrimSrc src{in.tris, L.boxes};
-    k_morton<SOUP, KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
-        src, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.otherEnd, L.hist, P);
+    k_morton<KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
+        cx, cy, cz, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.otherEnd, L.hist, P);
     ev(2);
     KeyT* kin = static_cast<KeyT*>(L.keysA);
     KeyT* kout = static_cast<KeyT*>(L.keysB);
