@@ -110,6 +110,29 @@ def numpy_morton_keys(soup, bits):
     return key, lo, hi
 
 
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_soup_at_unaligned_address(gpu_ctx, shift):
+    """A soup that starts 4, 8 or 12 bytes past a 16-B boundary builds the same
+    nodes as the oracle (the bounds pass reads triangles with 4-B loads)."""
+    import torch
+
+    n = 70001
+    soup = random_soup(n, 11 + shift, dup=False)
+    print(f"[lbvh parity] unaligned soup n={n} shift={shift * 4} B", flush=True)
+    flat = torch.zeros(n * 9 + shift, dtype=torch.float32, device="cuda")
+    flat[shift:] = torch.from_numpy(soup.reshape(-1)).cuda()
+    view = flat[shift:]
+    assert view.data_ptr() % 16 == 4 * shift
+    inodes, leaves = ca.node_buffers(n)
+    ca.LBVHBuilder(gpu_ctx, 0).perform_soup(view, n, 10, inodes, leaves)
+    torch.cuda.synchronize()
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, 10)
+    np.testing.assert_array_equal(leaves.cpu().numpy().reshape(-1, 48), o["leaves"])
+    np.testing.assert_array_equal(inodes.cpu().numpy().reshape(-1, 48)[: n - 1], o["inodes"])
+    assert gpu_ctx.device_status() == 0
+
+
 def test_full_size_16m_properties(gpu_ctx):
     """2^24 triangles: permutation, sorted keys with index tie-break, exact leaf
     boxes, parent box == fmin/fmax of children, every leaf reached once."""
