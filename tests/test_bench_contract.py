@@ -1,0 +1,28 @@
+"""bench.py's command-line contract on the CPU: one GPU only, and the LBVH
+byte model it reports against."""
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_build_byte_model():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    # the build's contract model: bounds 36 + Morton 40 + sort 64 + Karras 20 + refit 164
+    assert bench.BUILD_BYTES_PER_TRI == 36 + 40 + 64 + 20 + 164
+    stages = {name: (kernel, bpt, launches) for name, kernel, bpt, launches in bench.STAGES}
+    assert stages["hier_local"] == ("k_hier_local", 184.0, 1)
+    assert stages["sort"][1:] == (64.0, 4)
+    # k_bounds: soup 36 in, box record 32 + centre 12 out; k_morton: centre 12 in, key 4 + arrival word 4 out
+    assert stages["bounds"][1] == 36 + 32 + 12
+    assert stages["morton"][1] == 12 + 4 + 4
