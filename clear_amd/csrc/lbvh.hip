@@ -108,9 +108,24 @@ __device__ __forceinline__ Box soupBox(const float* tris, uint32_t i) {
     return boxOf3(p0, p1, p2);
 }
 
+// Streaming stores for records the next kernel does not read (box records,
+// gathered only by the hierarchy pass; leaf and node records, the build's
+// output): nontemporal, so they do not push the next kernel's inputs out of
+// L2 and the Infinity Cache (the Morton pass's centres, the sort's keys).
+typedef float StreamF4 __attribute__((ext_vector_type(4)));
+typedef unsigned int StreamU4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void streamStore(float4* p, float4 v) {
+    const StreamF4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<StreamF4*>(p));
+}
+__device__ __forceinline__ void streamStore(uint4* p, uint4 v) {
+    const StreamU4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<StreamU4*>(p));
+}
+
 __device__ __forceinline__ void storeBoxRec(float4* boxes, uint32_t pos, const Box& b, uint32_t obj) {
-    boxes[2 * (size_t)pos] = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
-    boxes[2 * (size_t)pos + 1] = make_float4(b.hi[1], b.hi[2], __uint_as_float(obj), 0.0f);
+    streamStore(boxes + 2 * (size_t)pos, make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]));
+    streamStore(boxes + 2 * (size_t)pos + 1, make_float4(b.hi[1], b.hi[2], __uint_as_float(obj), 0.0f));
 }
 
 __device__ __forceinline__ Box loadBoxRec(const float4* boxes, uint32_t pos, uint32_t* obj = nullptr) {
@@ -779,7 +794,7 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
         else
             v = make_uint4(__float_as_uint(s_box[3 * part][j]), __float_as_uint(s_box[3 * part + 1][j]),
                            __float_as_uint(s_box[3 * part + 2][j]), 0u);
-        reinterpret_cast<uint4*>(leaves + bs)[q] = v;
+        streamStore(reinterpret_cast<uint4*>(leaves + bs) + q, v);
     }
     __syncthreads();
 
@@ -912,7 +927,7 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
             v = make_uint4(__float_as_uint(s_box[3 * part][j]), __float_as_uint(s_box[3 * part + 1][j]),
                            __float_as_uint(s_box[3 * part + 2][j]), 0u);
         }
-        reinterpret_cast<uint4*>(inodes + (bs + j - seg))[part] = v;
+        streamStore(reinterpret_cast<uint4*>(inodes + (bs + j - seg)) + part, v);
     }
 }
 
