@@ -168,7 +168,9 @@ def render_leg(ca, np, torch, ctx, args):
                        f"max depth {depth}, thin lens 0.02", "triangles": host.num_faces,
            "spp": spp}
     tune_t = time.perf_counter()
-    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 1, depth)
+    # Tuned on 4-spp renders (about 30 ms each at 1080p): 1-spp timings are
+    # noisy enough to settle on a different configuration from run to run.
+    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 4, depth)
     out["wavefront_config"] = cfg.as_dict()
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     legs = [("megakernel_plain", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL),
