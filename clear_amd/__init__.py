@@ -61,7 +61,8 @@ def lib():
         "clr_lbvh_build": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
         "clr_lbvh_build_soup": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "clr_lbvh_reserve": (i32, [vp, u32, u32]),
-        "clr_lbvh_build_segmented": (i32, [vp, vp, ctypes.POINTER(u32), u32, u32, vp, vp, vp]),
+        "clr_lbvh_build_segmented": (i32, [vp, vp, vp, u32, u32, u32, vp, vp, vp]),
+        "clr_lbvh_reserve_segmented": (i32, [vp, u32, u32, u32]),
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_trace_closest_ordered": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_trace_closest_bvh": (i32, [vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, i32, vp]),
@@ -212,13 +213,13 @@ class LBVHBuilder:
             self.ctx.handle, _ptr(tris), num_tris, num_bits_per_dim,
             _ptr(internal_nodes), _ptr(leaf_nodes), _stream(stream)))
 
-    def perform_segmented(self, tris, seg_offsets: np.ndarray, num_bits_per_dim: int,
+    def perform_segmented(self, tris, seg_offsets, num_tris: int, num_bits_per_dim: int,
                           internal_nodes, leaf_nodes, stream=None) -> None:
-        offs = np.ascontiguousarray(seg_offsets, dtype=np.uint32)
+        """clr_lbvh_build_segmented: seg_offsets is a device uint32/int32
+        tensor of numSegments + 1 prefix offsets ending at num_tris."""
         self.ctx.check(lib().clr_lbvh_build_segmented(
-            self.ctx.handle, _ptr(tris), offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-            len(offs) - 1, num_bits_per_dim, _ptr(internal_nodes), _ptr(leaf_nodes),
-            _stream(stream)))
+            self.ctx.handle, _ptr(tris), _ptr(seg_offsets), seg_offsets.numel() - 1, num_tris,
+            num_bits_per_dim, _ptr(internal_nodes), _ptr(leaf_nodes), _stream(stream)))
 
 
 def trace_closest_bvh(ctx: Context, nodes, num_faces: int, vertices, faces, rays, num_rays: int, hit_face,
@@ -318,6 +319,7 @@ def decode_leaves(buf: np.ndarray, n: int):
 # ---- scenes and rendering -------------------------------------------------
 
 SCENE_CORNELL, SCENE_SHOWCASE, SCENE_FURNACE, SCENE_TERRAIN, SCENE_ADJOINT, SCENE_CLUSTERS = 0, 1, 2, 3, 4, 5
+SCENE_AREA_LIGHTS = 6
 ARRAY_NAMES = ["vertices", "normals", "tangents", "uvs", "faces", "lights", "materials",
                "textures", "others", "bvh_nodes"]
 RENDER_MEGAKERNEL, RENDER_WAVEFRONT = 0, 1
@@ -518,6 +520,7 @@ class SceneBatch:
         total = int(self.offsets[-1])
         self.tris_host = soup_of(hosts, self.offsets)
         self.tris = torch.from_numpy(self.tris_host.reshape(-1)).to(device)
+        self.offsets_dev = torch.from_numpy(self.offsets.view(np.int32)).to(device)
         self.inodes = torch.empty(max(total - len(hosts), 1) * INODE_BYTES, dtype=torch.uint8, device=device)
         self.leaves = torch.empty(total * LEAF_BYTES, dtype=torch.uint8, device=device)
         self.build(bits, stream)
@@ -535,7 +538,8 @@ class SceneBatch:
     def build(self, bits: int = 10, stream=None) -> None:
         """(Re)builds every scene's LBVH in one segmented build."""
         b = LBVHBuilder(self.ctx, 0, bits)
-        b.perform_segmented(self.tris, self.offsets, bits, self.inodes, self.leaves, stream)
+        b.perform_segmented(self.tris, self.offsets_dev, int(self.offsets[-1]), bits, self.inodes, self.leaves,
+                            stream)
 
     def __len__(self) -> int:
         return len(self.hosts)
@@ -567,6 +571,8 @@ def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None
 
 RENDER_REFERENCE_TRAVERSAL = 1
 RENDER_NO_RR = 2
+RENDER_LIGHT_SAMPLING_ONLY = 4
+RENDER_BSDF_SAMPLING_ONLY = 8
 
 
 class DiffParams(ctypes.Structure):

@@ -47,31 +47,24 @@ int ensurePool(clr_ctx* ctx, size_t bytes) {
     return CLR_OK;
 }
 
-int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut) {
+int ensureExchange(clr_ctx* ctx, size_t nodes) {
     if (nodes == 0) nodes = 1;
-    if (ctx->xchgNodes < nodes || ctx->epoch == 0xFFFFFFFFu) {
-        int rc = hipCheck(ctx, hipDeviceSynchronize(), "exchange regrow sync");
-        if (rc) return rc;
-        size_t want = std::max(nodes + nodes / 8, ctx->xchgNodes);
-        if (ctx->xchgNodes < nodes) {
-            if (ctx->xchg) (void)hipFree(ctx->xchg);
-            ctx->xchg = nullptr;
-            ctx->xchgNodes = 0;
-            hipError_t e = hipMalloc(&ctx->xchg, want * 6 * sizeof(uint64_t));
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                ctx->xchg = nullptr;
-                return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "hierarchy exchange buffer");
-            }
-            ctx->xchgNodes = want;
-        }
-        fillZero(reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12, nullptr);
-        rc = hipCheck(ctx, hipDeviceSynchronize(), "exchange clear");
-        if (rc) return rc;
-        ctx->epoch = 0;
+    if (ctx->xchgNodes >= nodes) return CLR_OK;
+    int rc = hipCheck(ctx, hipDeviceSynchronize(), "exchange regrow sync");
+    if (rc) return rc;
+    const size_t want = nodes + nodes / 8;
+    if (ctx->xchg) (void)hipFree(ctx->xchg);
+    ctx->xchg = nullptr;
+    ctx->xchgNodes = 0;
+    hipError_t e = hipMalloc(&ctx->xchg, want * 6 * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->xchg = nullptr;
+        return fail(ctx, CLR_ERR_OUT_OF_MEMORY, "hierarchy exchange buffer");
     }
-    *epochOut = ++ctx->epoch;
-    return CLR_OK;
+    ctx->xchgNodes = want;
+    fillZero(reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12, nullptr);
+    return hipCheck(ctx, hipDeviceSynchronize(), "exchange clear");
 }
 
 int ensureRenderScratch(clr_ctx* ctx, uint32_t numFaces) {
@@ -173,6 +166,12 @@ int clr_create(clr_ctx** out, int device) {
     if (hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
         (void)hipGetLastError();
         ctx->numCUs = 0;
+    }
+    // The status and build-epoch words exist from the start, so no entry
+    // point allocates them while a caller's stream is being captured.
+    if (ensureStatus(ctx) != CLR_OK) {
+        clr_destroy(ctx);
+        return CLR_ERR_OUT_OF_MEMORY;
     }
     *out = ctx;
     return CLR_OK;
@@ -308,11 +307,23 @@ int clr_lbvh_reserve(clr_ctx* ctx, uint32_t maxFaces, uint32_t numBitsPerDim) {
         return fail(ctx, CLR_ERR_UNSUPPORTED, "reserve size out of range");
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    return ensurePool(ctx, lbvhScratchBytes(maxFaces, numBitsPerDim));
+    const int rc = ensurePool(ctx, lbvhScratchBytes(maxFaces, numBitsPerDim));
+    return rc ? rc : ensureExchange(ctx, maxFaces);
+}
+
+int clr_lbvh_reserve_segmented(clr_ctx* ctx, uint32_t maxTris, uint32_t numSegments, uint32_t numBitsPerDim) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (maxTris == 0 || maxTris >= (1u << 30) || numSegments == 0 || numSegments > maxTris || numBitsPerDim < 1 ||
+        numBitsPerDim > 10)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "reserve size out of range");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    const int rc = ensurePool(ctx, lbvhSegScratchBytes(maxTris, numSegments, numBitsPerDim));
+    return rc ? rc : ensureExchange(ctx, maxTris);
 }
 
 int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
-                             uint32_t numSegments, uint32_t numBitsPerDim,
+                             uint32_t numSegments, uint32_t numTris, uint32_t numBitsPerDim,
                              clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes,
                              void* stream) {
     if (!ctx) return CLR_ERR_INVALID_ARG;
@@ -320,17 +331,13 @@ int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* se
         return fail(ctx, CLR_ERR_INVALID_ARG, "null argument or zero segments");
     if (numBitsPerDim < 1 || numBitsPerDim > 10)
         return fail(ctx, CLR_ERR_UNSUPPORTED, "segmented build supports numBitsPerDim 1..10");
-    if (segOffsets[0] != 0) return fail(ctx, CLR_ERR_INVALID_ARG, "segOffsets[0] must be 0");
-    for (uint32_t s = 0; s < numSegments; ++s)
-        if (segOffsets[s + 1] <= segOffsets[s])
-            return fail(ctx, CLR_ERR_INVALID_ARG, "every segment needs >= 1 triangle");
-    if (segOffsets[numSegments] >= (1u << 30))
-        return fail(ctx, CLR_ERR_UNSUPPORTED, "total triangles must be < 2^30");
+    if (numTris < numSegments) return fail(ctx, CLR_ERR_INVALID_ARG, "every segment needs >= 1 triangle");
+    if (numTris >= (1u << 30)) return fail(ctx, CLR_ERR_UNSUPPORTED, "total triangles must be < 2^30");
     if (!aligned16(internalNodes) || !aligned16(leafNodes))
         return fail(ctx, CLR_ERR_INVALID_ARG, "node buffers must be 16-byte aligned");
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
-    return lbvhBuildSegmented(ctx, tris, segOffsets, numSegments, numBitsPerDim, internalNodes,
+    return lbvhBuildSegmented(ctx, tris, segOffsets, numSegments, numTris, numBitsPerDim, internalNodes,
                               leafNodes, static_cast<hipStream_t>(stream));
 }
 
@@ -544,6 +551,8 @@ int checkRenderArgs(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_p
         return fail(ctx, CLR_ERR_INVALID_ARG, "empty image or zero bounce cap");
     if ((uint64_t)params->width * params->height >= (1ull << 31))
         return fail(ctx, CLR_ERR_UNSUPPORTED, "image too large");
+    if ((params->flags & CLR_RENDER_LIGHT_SAMPLING_ONLY) && (params->flags & CLR_RENDER_BSDF_SAMPLING_ONLY))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "LIGHT_SAMPLING_ONLY and BSDF_SAMPLING_ONLY exclude each other");
     if (!aligned16(s.vertices) || (s.leafNodes && !aligned16(s.leafNodes)) ||
         (s.internalNodes && !aligned16(s.internalNodes)) ||
         !aligned16(s.otherResources) || !aligned16(s.texturesData) || !aligned16(s.materialsData))

@@ -29,13 +29,14 @@ struct clr_ctx {
     int profiledBuilds = 0;
     // Box hand-off buffer of the bottom-up hierarchy pass: six 64-bit words
     // (float bits | build epoch) per internal node. Only ever written with
-    // epoch-tagged words, zeroed when (re)allocated, so a stale word can never
-    // carry the current epoch.
+    // epoch-tagged words, zeroed when (re)allocated and when the epoch wraps,
+    // so a stale word can never carry the current epoch.
     uint64_t* xchg = nullptr;
     size_t xchgNodes = 0;
-    uint32_t epoch = 0;
-    // Device-side failure flags (bounded spin exceeded, stack overflow):
-    // one 32-bit word, read back by clr_device_status.
+    // 64 device words, allocated by clr_create: word kWordStatus holds the
+    // device-side failure flags (bounded spin exceeded, stack overflow), read
+    // back by clr_device_status; word kWordEpoch the LBVH build epoch, which
+    // the build's kernels advance and read on the build's stream.
     uint32_t* devStatus = nullptr;
     // Render scratch: face -> first light index, and 64 small device words
     // (word 0: first light at infinity).
@@ -76,14 +77,18 @@ constexpr uint32_t kStatusLookbackSpin = 1u << 0;
 constexpr uint32_t kStatusExchangeSpin = 1u << 1;
 constexpr uint32_t kStatusStackOverflow = 1u << 2;
 constexpr uint32_t kStatusHierarchy = 1u << 3;
+constexpr uint32_t kStatusSegments = 1u << 4;
 // Upper bound on any device-side wait loop (iterations of one atomic load).
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 // Grows the box hand-off buffer to `nodes` entries (synchronous when it has
-// to reallocate) and returns the epoch for the next build.
-int ensureExchange(clr_ctx* ctx, size_t nodes, uint32_t* epochOut);
-// Lazily allocates the device status word.
+// to reallocate; the new buffer is zeroed).
+int ensureExchange(clr_ctx* ctx, size_t nodes);
+// Allocates the device status words if clr_create has not (synchronous).
 int ensureStatus(clr_ctx* ctx);
+// Indices into clr_ctx::devStatus.
+constexpr uint32_t kWordStatus = 0;
+constexpr uint32_t kWordEpoch = 1;
 
 // Bump allocator over the pool (256-B aligned sub-buffers).
 // Makes the context's device current for the duration of a C-ABI call and
@@ -128,8 +133,9 @@ int lbvhBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bitsPerDim
 // Fills `count` 32-bit words with zero on `stream` (kernel, not hipMemset).
 void fillZero(uint32_t* words, size_t count, hipStream_t stream);
 size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bitsPerDim);
-int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegOffsets,
-                       uint32_t numSegments, uint32_t bitsPerDim, clr_lbvh_inode* inodes,
+// segOffsets: device array of numSegments + 1 offsets, ending at numTris.
+int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
+                       uint32_t numSegments, uint32_t numTris, uint32_t bitsPerDim, clr_lbvh_inode* inodes,
                        clr_lbvh_leaf* leaves, hipStream_t stream);
 
 // ---- rendering (render.hip, wavefront.hip) ----

@@ -157,13 +157,30 @@ __device__ __forceinline__ float waveMax(float v) {
 }
 
 // ---------------------------------------------------------------- init ----
-__global__ __launch_bounds__(kBlock) void k_init(uint32_t* words, size_t count,
-                                                 uint32_t* sceneOrd) {
+// Clears `count` words (histograms, look-back words) and sets the scene box
+// to empty. With `epochWord`: when the build epoch is about to wrap to 0
+// (every 2^32 - 1 builds) it also clears the hierarchy's exchange words, so
+// that no word left by an earlier build carries the next epoch. No kernel
+// writes the epoch word while this one runs, so every block reads one value.
+__global__ __launch_bounds__(kBlock) void k_init(uint32_t* words, size_t count, uint32_t* sceneOrd,
+                                                 const uint32_t* epochWord = nullptr,
+                                                 uint32_t* xchgWords = nullptr, size_t xchgCount = 0) {
     size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     size_t stride = (size_t)gridDim.x * kBlock;
-    for (; i < count; i += stride) words[i] = 0u;
+    for (size_t j = i; j < count; j += stride) words[j] = 0u;
+    if (epochWord && *epochWord == 0xFFFFFFFFu)
+        for (size_t j = i; j < xchgCount; j += stride) xchgWords[j] = 0u;
     if (sceneOrd && blockIdx.x == 0 && threadIdx.x < 6)
         sceneOrd[threadIdx.x] = threadIdx.x < 3 ? ordOf(INFINITY) : ordOf(-INFINITY);
+}
+
+// The build epoch tags the hierarchy's box hand-off words. It lives in device
+// memory and advances once per build on the build's own stream (in the
+// kernel after k_init), never 0, so a build replayed from a captured graph
+// still gets a fresh epoch.
+__device__ __forceinline__ void advanceEpoch(uint32_t* epochWord) {
+    const uint32_t e = *epochWord;
+    *epochWord = e == 0xFFFFFFFFu ? 1u : e + 1u;
 }
 
 // -------------------------------------------------------------- bounds ----
@@ -180,7 +197,9 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
                                                    float4* __restrict__ boxesOut,
                                                    float* __restrict__ cx, float* __restrict__ cy,
                                                    float* __restrict__ cz,
-                                                   uint32_t* __restrict__ sceneOrd) {
+                                                   uint32_t* __restrict__ sceneOrd,
+                                                   uint32_t* __restrict__ epochWord) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) advanceEpoch(epochWord);
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -550,6 +569,7 @@ __device__ __forceinline__ int deltaAdj(const KeyT* __restrict__ k, uint32_t x, 
 struct SegView {
     const uint32_t* segOf;
     const uint32_t* segOff;
+    const uint32_t* valid;  // 0: the scene offsets were rejected, nothing is built
 };
 
 // delta(x, x+1), -1 outside [0, n) and, in a segmented build, across scenes.
@@ -714,9 +734,12 @@ __device__ __forceinline__ uint32_t xcdContiguous(uint32_t g, uint32_t G) {
 template <bool SOUP, typename KeyT, bool SEG = false>
 __global__ __launch_bounds__(kHierThreads) void k_hier_local(
     PrimSrc src, const KeyT* __restrict__ skeys, const uint32_t* __restrict__ sidx, uint32_t n,
-    int keyBits, uint32_t* __restrict__ otherEnd, uint64_t* __restrict__ xchg, uint32_t epoch,
+    int keyBits, uint32_t* __restrict__ otherEnd, uint64_t* __restrict__ xchg, const uint32_t* __restrict__ epochWord,
     clr_lbvh_inode* __restrict__ inodes, clr_lbvh_leaf* __restrict__ leaves,
-    Pending* __restrict__ pend, uint32_t* __restrict__ devStatus, SegView sv = SegView{nullptr, nullptr}) {
+    Pending* __restrict__ pend, uint32_t* __restrict__ devStatus, SegView sv = SegView{nullptr, nullptr, nullptr}) {
+    if constexpr (SEG)
+        if (*sv.valid == 0u) return;
+    const uint32_t epoch = *epochWord;
     constexpr int B = kLocalLeaves;
     __shared__ uint32_t s_other[B];
     __shared__ float s_box[6][B];
@@ -934,11 +957,14 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
 template <typename KeyT, bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_hier_global(
     const KeyT* __restrict__ skeys, uint32_t n, int keyBits, uint32_t* __restrict__ otherEnd,
-    uint64_t* __restrict__ xchg, uint32_t epoch, uint32_t* __restrict__ devStatus,
+    uint64_t* __restrict__ xchg, const uint32_t* __restrict__ epochWord, uint32_t* __restrict__ devStatus,
     clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
-    const Pending* __restrict__ pend, uint32_t numPending, SegView sv = SegView{nullptr, nullptr}) {
+    const Pending* __restrict__ pend, uint32_t numPending, SegView sv = SegView{nullptr, nullptr, nullptr}) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= numPending) return;
+    if constexpr (SEG)
+        if (*sv.valid == 0u) return;
+    const uint32_t epoch = *epochWord;
     const Pending P = pend[p];
     if (!P.valid) return;
     uint32_t l = P.l, r = P.r, split = P.split;
@@ -1060,11 +1086,11 @@ inline unsigned gridFor(uint32_t n, unsigned cap) {
 template <typename KeyT, bool SOUP>
 int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
              clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
-    uint32_t epoch = 0;
     int rc = ensureStatus(ctx);
     if (rc) return rc;
-    rc = ensureExchange(ctx, n, &epoch);
+    rc = ensureExchange(ctx, n);
     if (rc) return rc;
+    uint32_t* epochWord = ctx->devStatus + kWordEpoch;
     Layout L = layoutFor(ctx->pool, n, bits);
     const int P = numPassesFor(bits);
     const uint32_t tile = tileFor(bits);
@@ -1075,8 +1101,8 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
         if (prof) (void)hipEventRecord(ctx->events[ctx->profiledBuilds][k], stream);
     };
     ev(0);
-    k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0,
-             stream>>>(L.hist, L.clearWords, L.sceneOrd);
+    k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
+        L.hist, L.clearWords, L.sceneOrd, epochWord, reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12);
     // One set of same-address atomics per block: a few blocks per CU.
     const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
     // Box centres live in buffers the first radix pass has not written yet:
@@ -1086,7 +1112,7 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     float* cy = reinterpret_cast<float*>(L.valsA);
     float* cz = reinterpret_cast<float*>(L.valsB);
     k_bounds<SOUP><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(in.tris, in.vertices, in.faces, n, L.boxes, cx,
-                                                                  cy, cz, L.sceneOrd);
+                                                                  cy, cz, L.sceneOrd, epochWord);
     ev(1);
     PrimSrc src{in.tris, L.boxes};
     k_morton<KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
@@ -1112,10 +1138,10 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     ev(3);
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
     k_hier_local<SOUP, KeyT><<<localBlocks, kHierThreads, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, ctx->devStatus);
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, inodes, leaves, L.pend, ctx->devStatus);
     ev(4);
     k_hier_global<KeyT><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend,
+        kin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, ctx->devStatus, inodes, leaves, L.pend,
         2 * localBlocks);
     ev(5);
     if (prof) ++ctx->profiledBuilds;
@@ -1124,15 +1150,84 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
 
 // ------------------------------------------------------ segmented build ----
 // Many scenes (contiguous ranges of one triangle soup) in one set of
-// launches. The host splits every scene into chunks of at most kSegChunk
-// triangles; a block handles one chunk, so it knows its scene without a
-// search. Keys are (scene << 3 * bits) | Morton code within the scene's own
-// box, so one LSD sort orders scenes by index and each scene by its codes with
-// the index tie-break, exactly as a build of that scene alone would.
+// launches. Every scene is split into chunks of at most kSegChunk triangles
+// by k_seg_chunks, on the device, from the caller's device offsets; a block
+// handles one chunk, so it knows its scene without a search. Keys are
+// (scene << 3 * bits) | Morton code within the scene's own box, so one LSD
+// sort orders scenes by index and each scene by its codes with the index
+// tie-break, exactly as a build of that scene alone would.
 constexpr uint32_t kSegChunk = 4096;
+constexpr int kSegScanThreads = 1024;
+// Control words of a segmented build: the number of chunks, and whether the
+// scene offsets were accepted.
+constexpr uint32_t kSegWordChunks = 0, kSegWordValid = 1;
 
-__global__ __launch_bounds__(kBlock) void k_seg_init(uint32_t* segBox, uint32_t numSegments) {
+// At most this many chunks: sum over scenes of ceil(len / kSegChunk).
+inline uint32_t segChunkBound(uint32_t n, uint32_t numSegments) { return n / kSegChunk + numSegments; }
+
+// Chunk table (one block): checks that the offsets start at 0, grow strictly
+// and end at n; if not, it raises kStatusSegments and publishes no chunk and
+// valid = 0, so the later kernels of the build write nothing. Otherwise chunk
+// k is (scene, first, end) in scene order: a block-wide scan of the scenes'
+// chunk counts, 1024 scenes per round.
+__global__ __launch_bounds__(kSegScanThreads) void k_seg_chunks(const uint32_t* __restrict__ segOff,
+                                                                uint32_t numSegments, uint32_t n,
+                                                                uint4* __restrict__ chunks,
+                                                                uint32_t* __restrict__ words,
+                                                                uint32_t* __restrict__ devStatus) {
+    __shared__ uint32_t s_wave[kSegScanThreads / 64];
+    __shared__ uint32_t s_bad;
+    if (threadIdx.x == 0) s_bad = (segOff[0] != 0u || segOff[numSegments] != n) ? 1u : 0u;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < numSegments; s += kSegScanThreads)
+        if (segOff[s + 1] <= segOff[s]) s_bad = 1u;  // every writer stores the same value
+    __syncthreads();
+    if (s_bad) {
+        if (threadIdx.x == 0) {
+            words[kSegWordChunks] = 0u;
+            words[kSegWordValid] = 0u;
+            atomicOr(devStatus, kStatusSegments);
+        }
+        return;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t base = 0;
+    for (uint32_t s0 = 0; s0 < numSegments; s0 += kSegScanThreads) {
+        const uint32_t sc = s0 + threadIdx.x;
+        uint32_t lo = 0, hi = 0, c = 0;
+        if (sc < numSegments) {
+            lo = segOff[sc];
+            hi = segOff[sc + 1];
+            c = (hi - lo + kSegChunk - 1) / kSegChunk;
+        }
+        uint32_t x = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int w = 0; w < kSegScanThreads / 64; ++w) {
+            if (w < wave) pre += s_wave[w];
+            tot += s_wave[w];
+        }
+        __syncthreads();  // s_wave is rewritten in the next round
+        uint32_t k = base + pre + x - c;
+        for (uint32_t b = lo; b < hi; b += kSegChunk, ++k) chunks[k] = make_uint4(sc, b, min(b + kSegChunk, hi), 0u);
+        base += tot;
+    }
+    if (threadIdx.x == 0) {
+        words[kSegWordChunks] = base;
+        words[kSegWordValid] = 1u;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_init(uint32_t* segBox, uint32_t numSegments,
+                                                     uint32_t* epochWord) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) advanceEpoch(epochWord);
     if (i < 6 * numSegments) segBox[i] = (i % 6) < 3 ? ordOf(INFINITY) : ordOf(-INFINITY);
 }
 
@@ -1140,7 +1235,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_init(uint32_t* segBox, uint32_t 
 // ordered-int atomics on the scene's six words).
 __global__ __launch_bounds__(kBlock) void k_seg_bounds(const float* __restrict__ tris,
                                                        const uint4* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ words,
                                                        uint32_t* __restrict__ segBox) {
+    if (blockIdx.x >= words[kSegWordChunks]) return;  // the grid covers the bound on chunks
     const uint4 ch = chunks[blockIdx.x];
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1181,10 +1278,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds(const float* __restrict__
 // and the digit histograms of all passes.
 __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__ tris, float4* __restrict__ boxes,
                                                        uint32_t n, uint32_t bits, const uint4* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ words,
                                                        const uint32_t* __restrict__ segBox,
                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ segOf,
                                                        uint32_t* __restrict__ otherEnd, uint32_t* __restrict__ hist,
                                                        int numPasses) {
+    if (blockIdx.x >= words[kSegWordChunks]) return;  // the grid covers the bound on chunks
     __shared__ uint32_t s_hist[kMaxPasses * 256];
     for (int j = threadIdx.x; j < numPasses * 256; j += kBlock) s_hist[j] = 0u;
     __syncthreads();
@@ -1225,7 +1324,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__
 struct SegLayout {
     Layout L;
     uint32_t* segOf;
-    uint32_t* segOff;
+    uint32_t* words;
     uint32_t* segBox;
     uint4* chunks;
     size_t bytes;
@@ -1237,35 +1336,28 @@ int segKeyBits(uint32_t bits, uint32_t numSegments) {
     return (int)(3 * bits) + sb;
 }
 
-SegLayout segLayoutFor(void* base, uint32_t n, uint32_t numSegments, uint32_t numChunks, int P) {
+SegLayout segLayoutFor(void* base, uint32_t n, uint32_t numSegments, int P) {
     Carve c(base);
     SegLayout S;
     S.L = layoutFor(c, n, 8, P, (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems));
     S.segOf = c.take<uint32_t>(n);
-    S.segOff = c.take<uint32_t>((size_t)numSegments + 1);
+    S.words = c.take<uint32_t>(64);
     S.segBox = c.take<uint32_t>(6 * (size_t)numSegments);
-    S.chunks = c.take<uint4>(numChunks);
+    S.chunks = c.take<uint4>(segChunkBound(n, numSegments));
     S.bytes = c.off + 256;
     return S;
-}
-
-uint32_t countChunks(const uint32_t* off, uint32_t numSegments) {
-    uint64_t c = 0;
-    for (uint32_t s = 0; s < numSegments; ++s) c += (off[s + 1] - off[s] + kSegChunk - 1) / kSegChunk;
-    return (uint32_t)c;
 }
 
 }  // namespace
 
 size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bitsPerDim) {
     const int P = (segKeyBits(bitsPerDim, numSegments) + 7) / 8;
-    const uint32_t chunks = totalTris / kSegChunk + numSegments;  // upper bound
-    return segLayoutFor(nullptr, totalTris, numSegments, chunks, P).bytes;
+    return segLayoutFor(nullptr, totalTris, numSegments, P).bytes;
 }
 
-int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegOffsets, uint32_t numSegments,
-                       uint32_t bits, clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
-    const uint32_t n = hostSegOffsets[numSegments];
+int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets, uint32_t numSegments,
+                       uint32_t n, uint32_t bits, clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves,
+                       hipStream_t stream) {
     const int keyBits = segKeyBits(bits, numSegments);
     const int P = (keyBits + 7) / 8;
     if (P > kMaxPasses) return fail(ctx, CLR_ERR_UNSUPPORTED, "too many segments for 64-bit keys");
@@ -1273,34 +1365,20 @@ int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegO
     if (rc) return rc;
     rc = ensureStatus(ctx);
     if (rc) return rc;
-    uint32_t epoch = 0;
-    rc = ensureExchange(ctx, n, &epoch);
+    rc = ensureExchange(ctx, n);
     if (rc) return rc;
-    const uint32_t numChunks = countChunks(hostSegOffsets, numSegments);
-    SegLayout S = segLayoutFor(ctx->pool, n, numSegments, numChunks, P);
+    uint32_t* epochWord = ctx->devStatus + kWordEpoch;
+    SegLayout S = segLayoutFor(ctx->pool, n, numSegments, P);
     Layout& L = S.L;
-    // Chunk table and scene offsets: built on the host (the caller passes
-    // host offsets), copied once.
-    std::vector<uint4> chunks;
-    chunks.reserve(numChunks);
-    for (uint32_t s = 0; s < numSegments; ++s)
-        for (uint32_t b = hostSegOffsets[s]; b < hostSegOffsets[s + 1]; b += kSegChunk)
-            chunks.push_back(make_uint4(s, b, std::min(b + kSegChunk, hostSegOffsets[s + 1]), 0u));
-    rc = hipCheck(ctx, hipMemcpyAsync(S.chunks, chunks.data(), sizeof(uint4) * numChunks, hipMemcpyHostToDevice,
-                                      stream), "chunk table copy");
-    if (!rc)
-        rc = hipCheck(ctx, hipMemcpyAsync(S.segOff, hostSegOffsets, sizeof(uint32_t) * (numSegments + 1),
-                                          hipMemcpyHostToDevice, stream), "segment offsets copy");
-    // The copies read pageable host memory: finish them before the vector goes.
-    if (!rc) rc = hipCheck(ctx, hipStreamSynchronize(stream), "segment table sync");
-    if (rc) return rc;
+    const uint32_t maxChunks = segChunkBound(n, numSegments);
     const uint32_t tile = (uint32_t)(kSortBlock * SortCfg<uint64_t>::kItems);
     const uint32_t parts = (n + tile - 1) / tile;
     k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
-        L.hist, L.clearWords, nullptr);
-    k_seg_init<<<(6 * numSegments + kBlock - 1) / kBlock, kBlock, 0, stream>>>(S.segBox, numSegments);
-    k_seg_bounds<<<numChunks, kBlock, 0, stream>>>(tris, S.chunks, S.segBox);
-    k_seg_morton<<<numChunks, kBlock, 0, stream>>>(tris, L.boxes, n, bits, S.chunks, S.segBox,
+        L.hist, L.clearWords, nullptr, epochWord, reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12);
+    k_seg_init<<<(6 * numSegments + kBlock - 1) / kBlock, kBlock, 0, stream>>>(S.segBox, numSegments, epochWord);
+    k_seg_chunks<<<1, kSegScanThreads, 0, stream>>>(segOffsets, numSegments, n, S.chunks, S.words, ctx->devStatus);
+    k_seg_bounds<<<maxChunks, kBlock, 0, stream>>>(tris, S.chunks, S.words, S.segBox);
+    k_seg_morton<<<maxChunks, kBlock, 0, stream>>>(tris, L.boxes, n, bits, S.chunks, S.words, S.segBox,
                                                    static_cast<uint64_t*>(L.keysA), S.segOf, L.otherEnd, L.hist, P);
     uint64_t* kin = static_cast<uint64_t*>(L.keysA);
     uint64_t* kout = static_cast<uint64_t*>(L.keysB);
@@ -1319,13 +1397,14 @@ int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* hostSegO
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    const SegView sv{S.segOf, S.segOff};
+    const SegView sv{S.segOf, segOffsets, S.words + kSegWordValid};
     PrimSrc src{tris, L.boxes};
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
     k_hier_local<true, uint64_t, true><<<localBlocks, kHierThreads, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epoch, inodes, leaves, L.pend, ctx->devStatus, sv);
+        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, inodes, leaves, L.pend, ctx->devStatus, sv);
     k_hier_global<uint64_t, true><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        kin, n, keyBits, L.otherEnd, ctx->xchg, epoch, ctx->devStatus, inodes, leaves, L.pend, 2 * localBlocks, sv);
+        kin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, ctx->devStatus, inodes, leaves, L.pend, 2 * localBlocks,
+        sv);
     return hipCheck(ctx, hipGetLastError(), "segmented LBVH build launch");
 }
 

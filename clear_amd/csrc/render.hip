@@ -24,11 +24,13 @@ struct NoHook {
     __device__ __forceinline__ void bounce(uint32_t, const BSDF&, V3, V3, const SurfPt&, V3) {}
 };
 
-// One camera path for pixel (gx, gy) (pathtracer.cl:48-196). noRR disables
-// Russian roulette (the bounce cap still applies).
+// One camera path for pixel (gx, gy) (pathtracer.cl:48-196). flags: the
+// CLR_RENDER_* word (NO_RR disables Russian roulette, the bounce cap still
+// applies; the SAMPLING_ONLY bits select one strategy).
 template <int WALK, class Hook = NoHook>
-__device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, bool noRR, Rng& rng,
+__device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxDepth, uint32_t flags, Rng& rng,
                         uint32_t* stack, int stride, Hook& hook) {
+    const bool noRR = (flags & CLR_RENDER_NO_RR) != 0;
     V3 pix = v3z();
     V3 org, dir;
     float w, dirPDF;
@@ -56,32 +58,35 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
             bsdfAlloc(S, F.matPtr, sp, B);
             if (hasNonSpecular(B)) {
                 const float lu = rng.uniform(), l0 = rng.uniform(), l1 = rng.uniform();
-                SurfPt lp;
-                bool atInf = false;
-                float lightPDF;
-                sampleLight(S, lu, l0, l1, lp, atInf, E, lightPDF);
-                V3 vinL = lp.p - sp.p;
-                float dist2 = dot(vinL, vinL);
-                vinL = vinL * (1.0f / sqrtf(dist2));
-                if (atInf) {
-                    vinL = lp.p;
-                    dist2 = 1.0f;
-                }
-                const V3 sorg = sp.p + vinL * kRayEps;
-                SurfPt lsp;
-                const bool hit = intersect<WALK>(S, sorg, vinL, stack, stride, lsp);
-                const bool visible = atInf ? !hit : (hit && lsp.face == lp.face);
-                // A zero or infinite light density (an environment sample at a
-                // pole, where sin(theta) = 0) contributes nothing.
-                if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {
-                    const V3 vinLrev = -vinL;
-                    const float fsPDF = evalFsPdf(B, vout, vinL) * absCosEDF(E, vinL) / dist2;
-                    const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
-                    const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(B.N, vinL)) / dist2 / lightPDF;
-                    const V3 term = wgt * ((alpha * evalLe(E, vinLrev)) * evalFs(B, vout, vinL));
-                    pix += term;
-                    hook.lightSample(term, atInf ? 0xFFFFFFFFu : (uint32_t)S.g.faces[lp.face].lightPtr, F.matPtr, B,
-                                     vout, vinL, sp);
+                // With BSDF samples alone the light sample is skipped (its numbers are drawn).
+                if (!(flags & CLR_RENDER_BSDF_SAMPLING_ONLY)) {
+                    SurfPt lp;
+                    bool atInf = false;
+                    float lightPDF;
+                    sampleLight(S, lu, l0, l1, lp, atInf, E, lightPDF);
+                    V3 vinL = lp.p - sp.p;
+                    float dist2 = dot(vinL, vinL);
+                    vinL = vinL * (1.0f / sqrtf(dist2));
+                    if (atInf) {
+                        vinL = lp.p;
+                        dist2 = 1.0f;
+                    }
+                    const V3 sorg = sp.p + vinL * kRayEps;
+                    SurfPt lsp;
+                    const bool hit = intersect<WALK>(S, sorg, vinL, stack, stride, lsp);
+                    const bool visible = atInf ? !hit : (hit && lsp.face == lp.face);
+                    // A zero or infinite light density (an environment sample at a
+                    // pole, where sin(theta) = 0) contributes nothing.
+                    if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {
+                        const V3 vinLrev = -vinL;
+                        const float fsPDF = evalFsPdf(B, vout, vinL) * absCosEDF(E, vinL) / dist2;
+                        const float mis = lightSampleMis(flags, lightPDF, fsPDF);
+                        const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(B.N, vinL)) / dist2 / lightPDF;
+                        const V3 term = wgt * ((alpha * evalLe(E, vinLrev)) * evalFs(B, vout, vinL));
+                        pix += term;
+                        hook.lightSample(term, atInf ? 0xFFFFFFFFu : (uint32_t)S.g.faces[lp.face].lightPtr, F.matPtr, B,
+                                         vout, vinL, sp);
+                    }
                 }
             }
             traceContinue = true;
@@ -108,7 +113,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
                     const V3 dd = sp.p - org;
                     const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
                     const float lightPDF = areaPdfOfFace(S, face) * d2 / absCosEDF(E, vout);
-                    mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+                    mis = emissionHitMis(flags, dirPDF, lightPDF);
                 }
                 const V3 term = (mis * alpha) * evalLe(E, vout);
                 pix += term;
@@ -138,7 +143,7 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
         float mis = 1.0f;
         if ((sampledType & kNonSingular) != 0 && depth > 0) {
             const float lightPDF = areaPdfEnv(S, u, v) * 1.0f / absCosEDF(E, vout);
-            mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+            mis = emissionHitMis(flags, dirPDF, lightPDF);
         }
         pix += (mis * alpha) * evalLe(E, vout);
     }
@@ -264,7 +269,7 @@ constexpr int kTile = 8;  // 8x8 pixels per wave
 
 template <int WALK>
 __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t width, uint32_t height,
-                                                    uint32_t spp, uint32_t maxDepth, bool noRR,
+                                                    uint32_t spp, uint32_t maxDepth, uint32_t flags,
                                                     uint32_t* __restrict__ states,
                                                     clr_float3* __restrict__ pixels) {
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(64) void k_render_mega(dev::RScene scene, uint32_t 
     dev::V3 acc = dev::v3z();
     dev::NoHook hook;
     for (uint32_t k = 0; k < spp; ++k)
-        acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64, hook);
+        acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, s_stack + threadIdx.x, 64, hook);
     reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     float4 p = reinterpret_cast<float4*>(pixels)[pi];
     p.x += acc.x;
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(64) void k_bvh_visualizer(dev::RScene scene, uint32
 // global memory once per block.
 template <int WALK>
 __global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32_t width, uint32_t height,
-                                                       uint32_t spp, uint32_t maxDepth, bool noRR,
+                                                       uint32_t spp, uint32_t maxDepth, uint32_t flags,
                                                        const uint32_t* __restrict__ states,
                                                        const float4* __restrict__ dLdI,
                                                        const uint16_t* __restrict__ table, uint32_t numMat,
@@ -375,7 +380,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32
         const float inv = 1.0f / (float)spp;
         dev::AdjHook hook{S, table, lightProp, M, {a.x * inv, a.y * inv, a.z * inv}, &s_acc, 0};
         for (uint32_t k = 0; k < spp; ++k) {
-            const dev::V3 L = dev::tracePath<WALK>(S, gx, gy, maxDepth, noRR, rng, s_stack + threadIdx.x, 64,
+            const dev::V3 L = dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, s_stack + threadIdx.x, 64,
                                                       hook);
             hook.finish(L);
         }
@@ -495,16 +500,16 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     int rc = prepareRenderScene(ctx, sd, S, stream);
     if (rc) return rc;
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
-    const bool noRR = (rp.flags & CLR_RENDER_NO_RR) != 0;
+    const uint32_t flags = rp.flags;
     if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
         k_render_mega<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                   rp.maxDepth, noRR, states, pixels);
+                                                                   rp.maxDepth, flags, states, pixels);
     else if (sd.bvhNodes)
         k_render_mega<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                    rp.maxDepth, noRR, states, pixels);
+                                                                    rp.maxDepth, flags, states, pixels);
     else
         k_render_mega<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                rp.maxDepth, noRR, states, pixels);
+                                                                rp.maxDepth, flags, states, pixels);
     return hipCheck(ctx, hipGetLastError(), "render launch");
 }
 
@@ -547,19 +552,19 @@ int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params
     rc = prepareRenderScene(ctx, sd, S, stream);
     if (rc) return rc;
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
-    const bool noRR = (rp.flags & CLR_RENDER_NO_RR) != 0;
+    const uint32_t flags = rp.flags;
     const float4* adj = reinterpret_cast<const float4*>(dLdI);
     if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
         k_render_adjoint<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(
-            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
             dp.lightProp, gAlb, gSig, gEm);
     else if (sd.bvhNodes)
         k_render_adjoint<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(
-            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
             dp.lightProp, gAlb, gSig, gEm);
     else
         k_render_adjoint<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(
-            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, noRR, states, adj, table, dp.numMaterials,
+            S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
             dp.lightProp, gAlb, gSig, gEm);
     return hipCheck(ctx, hipGetLastError(), "adjoint launch");
 }

@@ -377,6 +377,21 @@ __device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint3
     return true;
 }
 
+// ---- MIS weights (pathtracer.cl:106-128,154-163) ----------------------------
+// Power heuristic of a light sample, or 1 when light samples are used alone
+// (CLR_RENDER_LIGHT_SAMPLING_ONLY).
+__device__ __forceinline__ float lightSampleMis(uint32_t flags, float lightPDF, float fsPDF) {
+    if (flags & CLR_RENDER_LIGHT_SAMPLING_ONLY) return 1.0f;
+    return (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
+}
+// Weight of emission reached by a diffuse or glossy BSDF sample: the power
+// heuristic, 0 with light samples alone, 1 with BSDF samples alone.
+__device__ __forceinline__ float emissionHitMis(uint32_t flags, float dirPDF, float lightPDF) {
+    if (flags & CLR_RENDER_LIGHT_SAMPLING_ONLY) return 0.0f;
+    if (flags & CLR_RENDER_BSDF_SAMPLING_ONLY) return 1.0f;
+    return (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+}
+
 // ---- BSDF (reflection.cl) --------------------------------------------------
 enum : uint32_t { kRefl = 1, kTrans = 2, kDiff = 4, kGlossy = 8, kSpec = 16, kNonSingular = 12, kAll = 31 };
 

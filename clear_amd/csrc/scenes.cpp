@@ -618,6 +618,64 @@ void buildClusters(clr_host_scene* hs, const clr_scene_params& p) {
                    p.fovY > 0 ? p.fovY : 45.0f, 0.0f, 1.0f);
 }
 
+// Direct-lighting scene: a Lambert floor under diffuse emitters (see
+// CLR_SCENE_AREA_LIGHTS): the quad, the quad plus a coplanar triangle, or a
+// tilted triangle.
+void buildAreaLights(clr_host_scene* hs, const clr_scene_params& p) {
+    Scene& s = hs->scene;
+    MaterialCreator& mc = MaterialCreator::sharedInstance();
+    mc.setScene(&s);
+    const float rho = p.albedo > 0.0f ? p.albedo : 0.5f;
+    mc.createFloatConstantTexture("sigma0", 0.0f);
+    mc.createFloat3ConstantTexture("R_floor", rho, rho, rho);
+    mc.createFloat3ConstantTexture("R_white", 0.75f, 0.75f, 0.75f);
+    mc.createMatteMaterial("mat_floor", nullptr, "R_floor", "sigma0");
+    mc.createMatteMaterial("mat_white", nullptr, "R_white", "sigma0");
+    mc.createFloat3ConstantTexture("M_light", 4.0f, 2.0f, 1.0f);
+    mc.createDiffuseLightProperty("light", "M_light");
+    const uint16_t floorMat = (uint16_t)s.idxOfMat("mat_floor"), white = (uint16_t)s.idxOfMat("mat_white");
+    const uint16_t light = (uint16_t)s.idxOfLight("light");
+    s.beginObject();
+    s.addVertex(-2.0f, 0.0f, -2.0f);
+    s.addVertex(2.0f, 0.0f, -2.0f);
+    s.addVertex(2.0f, 0.0f, 2.0f);
+    s.addVertex(-2.0f, 0.0f, 2.0f);
+    s.addFace(Face_P(0, 2, 1, floorMat));
+    s.addFace(Face_P(0, 3, 2, floorMat));
+    s.endObject();
+    if (p.detail == 0 || p.detail == 1) {
+        // [-0.1, 0.7] x [-0.45, 0.05] at y = 0.6: the unit quad scaled and moved.
+        s.localToWorld.push();
+        s.localToWorld *= Mat4::translate(0.3f, 0.6f, -0.2f);
+        s.localToWorld *= Mat4::scale(0.4f, 1.0f, 0.25f);
+        addCeilingLight(s, 1.0f, 0.0f, white, light);
+        s.localToWorld.pop();
+    }
+    if (p.detail == 1) {
+        s.beginObject();
+        s.addVertex(-0.9f, 0.6f, 0.1f);
+        s.addVertex(-0.3f, 0.6f, 0.5f);
+        s.addVertex(-0.8f, 0.6f, 0.7f);
+        s.addFace(Face_P(0, 1, 2, white, light));
+        s.endObject();
+    } else if (p.detail == 2) {
+        // Circumradius 0.5 about (0, 0.6, 0) in the plane of normal n = (1, -1, 0) / sqrt 2,
+        // spanned by e1 = (0, 0, 1) and e2 = n x e1 = (-1, -1, 0) / sqrt 2.
+        const float r2 = 0.70710678f;
+        s.beginObject();
+        for (int k = 0; k < 3; ++k) {
+            const float a = (90.0f + 120.0f * k) * kPiF / 180.0f, c = 0.5f * std::cos(a), d = 0.5f * std::sin(a);
+            s.addVertex(-d * r2, 0.6f - d * r2, c);
+        }
+        // (p1 - p0) x (p2 - p0) points along +n for this order.
+        s.addFace(Face_P(0, 1, 2, white, light));
+        s.endObject();
+    }
+    const float eye[3] = {0.0f, 2.5f, 2.5f}, at[3] = {0.0f, 0.0f, 0.0f};
+    setPerspective(s, hs, p.width ? p.width : 32, p.height ? p.height : 32, eye, at,
+                   p.fovY > 0 ? p.fovY : 45.0f, 0.0f, 1.0f);
+}
+
 }  // namespace
 
 extern "C" {
@@ -638,6 +696,7 @@ int clr_scene_build(int kind, const clr_scene_params* params, clr_host_scene** o
             case CLR_SCENE_TERRAIN: buildTerrain(hs, p); break;
             case CLR_SCENE_ADJOINT: buildAdjoint(hs, p); break;
             case CLR_SCENE_CLUSTERS: buildClusters(hs, p); break;
+            case CLR_SCENE_AREA_LIGHTS: buildAreaLights(hs, p); break;
             default: delete hs; return CLR_ERR_UNSUPPORTED;
         }
         hs->scene.build();

@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
                     const V3 dd = sp.p - org;
                     const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
                     const float lightPDF = areaPdfOfFace(S, h.x) * d2 / absCosEDF(E, vout);
-                    mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+                    mis = emissionHitMis(A.flags, dirPDF, lightPDF);
                 }
                 if (depth == 0) pix += alpha * evalLe(E, vout);
                 else pix += (mis * alpha) * evalLe(E, vout);
@@ -490,9 +490,10 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
                         vinL = lp.p;
                         dist2 = 1.0f;
                     }
-                    if (lightPDF > 0.0f && lightPDF < INFINITY) {
+                    // With BSDF samples alone no shadow ray is cast (the numbers are drawn).
+                    if (lightPDF > 0.0f && lightPDF < INFINITY && !(A.flags & CLR_RENDER_BSDF_SAMPLING_ONLY)) {
                         const float fsPDF = evalFsPdf(Bs, vout, vinL) * absCosEDF(E, vinL) / dist2;
-                        const float mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
+                        const float mis = lightSampleMis(A.flags, lightPDF, fsPDF);
                         const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(Bs.N, vinL)) / dist2 / lightPDF;
                         contrib = wgt * ((alpha * evalLe(E, -vinL)) * evalFs(Bs, vout, vinL));
                         sorg = sp.p + vinL * kRayEps;
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256) void k_wf_miss(WfArgs A, uint32_t in) {
             float mis = 1.0f;
             if ((sampledType & kNonSingular) != 0 && depth > 0) {
                 const float lightPDF = areaPdfEnv(S, u, v) * 1.0f / absCosEDF(E, vout);
-                mis = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+                mis = emissionHitMis(A.flags, dirPDF, lightPDF);
             }
             pix += (mis * alpha) * evalLe(E, vout);
             wantRay = endSample(A, S, slot, vLocal, pixBase, rng, pix, org, dir);
@@ -955,7 +956,8 @@ int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_re
     A.width = rp.width;
     A.height = rp.height;
     A.maxDepth = rp.maxDepth;
-    A.flags = rp.flags & (CLR_RENDER_REFERENCE_TRAVERSAL | CLR_RENDER_NO_RR);
+    A.flags = rp.flags & (CLR_RENDER_REFERENCE_TRAVERSAL | CLR_RENDER_NO_RR | CLR_RENDER_LIGHT_SAMPLING_ONLY |
+                          CLR_RENDER_BSDF_SAMPLING_ONLY);
     A.refillMin = w.cfg.refillMin;
     // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).
     const uint64_t bound = (uint64_t)rp.samplesPerPixel * (rp.maxDepth + 1ull) + 1ull;

@@ -6,7 +6,11 @@
  * and must live on the context's device. Work is enqueued on `stream`
  * (a hipStream_t passed as void*; NULL = the default stream) and is
  * stream-ordered: no entry point synchronises the host unless its comment says
- * so. A clr_ctx is externally synchronised: one host thread at a time.
+ * so. A clr_ctx is externally synchronised: one host thread at a time, and
+ * its scratch (LBVH pool, exchange words, render and wavefront state) is shared
+ * by all its calls, so work enqueued through one context must stay on one
+ * stream, or calls on different streams must be ordered by the caller (use
+ * one context per concurrent stream).
  *
  * Reference interfaces replaced (paths relative to the reference repository):
  *   OpenCL_PathTracer/LBVHBuilder.h:32-75, LBVHBuilder.cpp:160-375  (LBVHBuilder)
@@ -106,7 +110,8 @@ int clr_scratch_bytes(const clr_ctx* ctx, uint64_t* bytes);
 /* Device-side failure flags raised by kernels since the last clear: bit 0 a
  * radix-sort look-back wait, bit 1 a hierarchy hand-off wait exceeded its
  * bound (the result is then wrong), bit 2 a traversal stack overflowed, bit 3
- * the hierarchy met a node with more than two children (unsorted keys).
+ * the hierarchy met a node with more than two children (unsorted keys), bit 4
+ * a segmented build rejected its scene offsets.
  * Synchronises the device. No reference counterpart: the reference has no
  * failure detection (its atomic refit has no bound, lbvh_construction.cl:465). */
 int clr_device_status(clr_ctx* ctx, uint32_t* flags, int clear);
@@ -123,7 +128,9 @@ int clr_device_status(clr_ctx* ctx, uint32_t* flags, int clear);
  * Unlike the reference there is no host read-back of the scene bounds inside
  * the build (LBVHBuilder.cpp:228-229): the call is fully stream-ordered and
  * may be captured into a hipGraph once the scratch pool is large enough
- * (see clr_lbvh_reserve). numFaces == 1 writes leafNodes[0] only. */
+ * (see clr_lbvh_reserve); every replay is a build of the inputs' contents at
+ * that time (the hierarchy's hand-off tags advance on the device).
+ * numFaces == 1 writes leafNodes[0] only. */
 int clr_lbvh_build(clr_ctx* ctx, const clr_float3* vertices, const clr_face* faces,
                    uint32_t numFaces, uint32_t numBitsPerDim,
                    clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes, void* stream);
@@ -135,21 +142,35 @@ int clr_lbvh_build_soup(clr_ctx* ctx, const float* tris, uint32_t numTris,
                         uint32_t numBitsPerDim, clr_lbvh_inode* internalNodes,
                         clr_lbvh_leaf* leafNodes, void* stream);
 
-/* Pre-grows the scratch pool for builds of up to maxFaces triangles (mirrors
- * the capacity argument of the LBVHBuilder ctor). Synchronous. */
+/* Pre-grows the scratch pool and the hierarchy's exchange buffer for builds
+ * of up to maxFaces triangles (mirrors the capacity argument of the
+ * LBVHBuilder ctor). Synchronous. After it, builds of up to maxFaces
+ * triangles allocate nothing and may be captured into a hipGraph. */
 int clr_lbvh_reserve(clr_ctx* ctx, uint32_t maxFaces, uint32_t numBitsPerDim);
 
 /* Segmented build: numSegments independent scenes in one set of launches.
- * segOffsets[numSegments+1] (host) are prefix offsets into the soup (in
- * triangles); scene s owns tris[segOffsets[s] .. segOffsets[s+1]).
+ * segOffsets[numSegments+1] (DEVICE) are prefix offsets into the soup (in
+ * triangles): scene s owns tris[segOffsets[s] .. segOffsets[s+1]), every
+ * scene holds >= 1 triangle, segOffsets[0] = 0 and segOffsets[numSegments] =
+ * numTris (the total, passed so that the host never reads the offsets back).
  * Outputs are concatenated: scene s writes leafNodes[segOffsets[s] ...] and
  * internalNodes[segOffsets[s] - s ...] (numTris_s - 1 nodes), with child
  * indices local to the scene, exactly as clr_lbvh_build_soup would write them
- * for that scene alone. Every segment must hold >= 1 triangle. */
+ * for that scene alone. The offsets are checked on the device: offsets that
+ * break these rules raise bit 4 of clr_device_status and nothing is written.
+ * Stream-ordered with no host synchronisation once the scratch pool holds
+ * the build (a first call, or clr_lbvh_reserve_segmented, grows it), so the
+ * call may be captured into a hipGraph and replayed. numBitsPerDim 1..10,
+ * numTris < 2^30. No counterpart in the reference, which builds one scene per
+ * LBVHBuilder::perform (LBVHBuilder.cpp:160-375). */
 int clr_lbvh_build_segmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
-                             uint32_t numSegments, uint32_t numBitsPerDim,
+                             uint32_t numSegments, uint32_t numTris, uint32_t numBitsPerDim,
                              clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes,
                              void* stream);
+
+/* Pre-grows the scratch pool for segmented builds of up to maxTris triangles
+ * in numSegments scenes. Synchronous. */
+int clr_lbvh_reserve_segmented(clr_ctx* ctx, uint32_t maxTris, uint32_t numSegments, uint32_t numBitsPerDim);
 
 /* Stage profiling of LBVH builds. clr_set_profiling(ctx, 1) starts recording
  * HIP events at the stage boundaries of every following build (up to 64
@@ -261,6 +282,19 @@ enum { CLR_RENDER_MEGAKERNEL = 0, CLR_RENDER_WAVEFRONT = 1 };
  * with common random numbers need. Not in the reference (pathtracer.cl:167-172
  * always applies it). */
 #define CLR_RENDER_NO_RR 2u
+/* Sampling strategy. By default every diffuse or glossy vertex combines a
+ * light sample and the BSDF sample's emission hit with the power heuristic
+ * (pathtracer.cl:106-128,154-163). These flags keep one strategy alone, for
+ * unbiasedness checks (all three converge to the same image):
+ *   LIGHT_SAMPLING_ONLY: light samples weigh 1; emission (area or
+ *     environment) reached by a diffuse or glossy BSDF sample weighs 0.
+ *   BSDF_SAMPLING_ONLY: no light sample is added (its three random numbers
+ *     are still drawn, so the paths are the same); emission reached by a BSDF
+ *     sample weighs 1.
+ * Emission seen by the camera or through a specular bounce weighs 1 in every
+ * mode. Setting both flags is an invalid argument. Not in the reference. */
+#define CLR_RENDER_LIGHT_SAMPLING_ONLY 4u
+#define CLR_RENDER_BSDF_SAMPLING_ONLY 8u
 
 /* Adds samplesPerPixel camera paths per pixel into `pixels` (clr_float3 per
  * pixel, row-major, not divided by the sample count: the caller scales, as

@@ -83,7 +83,20 @@ enum {
      * light (emission 10) facing down above the cube (faces numTriangles and
      * numTriangles+1) and a white Lambert floor quad under it (faces
      * numTriangles+2, +3); pinhole camera, 128x128 by default. */
-    CLR_SCENE_CLUSTERS = 5
+    CLR_SCENE_CLUSTERS = 5,
+    /* Direct-lighting check: a Lambert floor of albedo `albedo` (0 = 0.5)
+     * over [-2,2]^2 at y = 0 facing up (faces 0, 1), no environment, and
+     * diffuse emitters of emission (4, 2, 1) that are also white Lambert
+     * surfaces: detail 0 (default) a quad [-0.1,0.7] x [-0.45,0.05] at
+     * y = 0.6 facing down (faces 2, 3); detail 1 that quad plus the triangle
+     * (-0.9,0.6,0.1), (-0.3,0.6,0.5), (-0.8,0.6,0.7) in its plane, facing
+     * down (face 4); detail 2 an equilateral triangle of circumradius 0.5
+     * centred at (0, 0.6, 0) facing along (1, -1, 0) / sqrt 2 (face 2), so
+     * the floor at x < -0.6 lies behind it. Pinhole camera at (0, 2.5, 2.5)
+     * looking at the origin, 45 degrees, 32x32 by default. At maxDepth 1 the
+     * image is the floor's direct lighting, which Lambert's polygon formula
+     * gives in closed form. */
+    CLR_SCENE_AREA_LIGHTS = 6
 };
 
 enum {

@@ -286,6 +286,69 @@ def fresnel_probe(ftype, eta, k, eta_ext, eta_int, cosi):
     return list(out)
 
 
+# Planted mistakes (orc_set_plant): each makes the restatement commit one
+# error, so a test can show that its pin detects it.
+PLANT_LIGHT_NO_INV_AREA = 1
+PLANT_AREA_PDF_NO_INV_AREA = 2
+PLANT_NEE_NO_COS_L = 3
+PLANT_MIS_NO_COS_L = 4
+PLANT_NEE_MIS_SWAPPED = 5
+PLANT_IMPLICIT_NO_DIST2 = 6
+PLANT_MIX_NO_AVERAGE = 7
+PLANT_MIX_OWN_PDF_ONLY = 8
+PLANT_NO_LIGHT_CHOICE_PROB = 9
+PLANT_LE_NO_INV_PI = 10
+PLANT_LE_TWO_SIDED = 11
+PLANT_BECKMANN_NO_4 = 12
+PLANT_MIX_PICK_FIRST = 13
+
+
+class planted:
+    """Context manager: the oracle commits mistake `k` inside the block."""
+
+    def __init__(self, k: int):
+        self.k = k
+
+    def __enter__(self):
+        _probe_fn("orc_set_plant", [ctypes.c_int])(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        _probe_fn("orc_set_plant", [ctypes.c_int])(0)
+        return False
+
+
+def camera_rays(arrays, film):
+    """Camera rays for film points film[i] = (px, py, lu0, lu1): (origins
+    [n, 3], directions [n, 3], direction pdfs [n]) in float64, and the camera's
+    +z axis in world space."""
+    fn = _probe_fn("orc_camera_rays", [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p])
+    keep, ptrs = _scene_ptrs(arrays, 1, None, None, None)
+    film = np.ascontiguousarray(film, dtype=np.float32).reshape(-1, 4)
+    n = film.shape[0]
+    out = np.zeros(7 * n + 3, np.float32)
+    fn(ptrs, n, film.ctypes.data, out.ctypes.data)
+    r = out[: 7 * n].reshape(n, 7).astype(np.float64)
+    return r[:, 0:3], r[:, 3:6], r[:, 6], out[7 * n:].astype(np.float64)
+
+
+def bsdf_probe(arrays, mat_offset, wo, wi, u):
+    """BSDF mixture of the material at byte offset `mat_offset`, at a point
+    with normal +z: dict of fs [n, 3], fs_pdf [n] at (wo, wi), and sample_fs
+    at (wo, u = (uc, u0, u1)): direction [n, 3], pdf [n], value [n, 3], type [n]."""
+    fn = _probe_fn("orc_bsdf_probe", [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p])
+    keep, ptrs = _scene_ptrs(arrays, 1, None, None, None)
+    wo = np.ascontiguousarray(wo, dtype=np.float32).reshape(-1, 3)
+    n = wo.shape[0]
+    wi = np.ascontiguousarray(np.broadcast_to(np.asarray(wi, np.float32).reshape(-1, 3), (n, 3)))
+    u = np.ascontiguousarray(np.broadcast_to(np.asarray(u, np.float32).reshape(-1, 3), (n, 3)))
+    out = np.zeros((n, 12), np.float32)
+    fn(ptrs, mat_offset, n, wo.ctypes.data, wi.ctypes.data, u.ctypes.data, out.ctypes.data)
+    return {"f": out[:, 0:3], "pdf": out[:, 3], "wi": out[:, 4:7], "spdf": out[:, 7], "sf": out[:, 8:11],
+            "type": out[:, 11].astype(np.uint32)}
+
+
 def disk_probe(u1, u2):
     fn = _probe_fn("orc_disk_probe", [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)])
     out = (ctypes.c_float * 2)()

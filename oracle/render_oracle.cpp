@@ -52,6 +52,27 @@ constexpr float kPi = 3.14159274101257324219f;  // M_PI_F
 constexpr float kEps = 0.00001f;                // EPSILON (global.cl:28)
 constexpr uint32_t kNoFace = 0xFFFFFFFFu;
 
+// Planted mistakes, for tests only: orc_set_plant(k) makes the restatement
+// commit mistake k, so that a test can show its pin detects that mistake.
+// 0 (the default) plants nothing.
+enum Plant : int {
+    kPlantLightNoInvArea = 1,     // sampleLightPos: pdf not divided by the triangle area
+    kPlantAreaPdfNoInvArea = 2,   // sampleLightPos and getAreaPDF both drop 1 / area
+    kPlantNeeNoCosL = 3,          // light-sample weight without |cos| at the light
+    kPlantMisNoCosL = 4,          // BSDF pdf taken to area measure without |cos| at the light
+    kPlantNeeMisSwapped = 5,      // light-sample MIS weight with the BSDF pdf on top
+    kPlantImplicitNoDist2 = 6,    // emission-hit light pdf taken to solid angle without dist^2
+    kPlantMixNoAverage = 7,       // mixture pdf not divided by the element count
+    kPlantMixOwnPdfOnly = 8,      // sample_fs returns the sampled element's pdf alone
+    kPlantNoLightChoiceProb = 9,  // the probability of the chosen light dropped
+    kPlantLeNoInvPi = 10,         // diffuse emission M instead of M / pi
+    kPlantLeTwoSided = 11,        // diffuse emission on both sides
+    kPlantBeckmannNo4 = 12,       // Beckmann f without the 4 of its denominator
+    kPlantMixPickFirst = 13,      // sample_fs always samples element 0
+};
+int g_plant = 0;
+inline bool planted(int k) { return g_plant == k; }
+
 struct V3 {
     float x, y, z;
 };
@@ -816,7 +837,7 @@ V3 fx(const BxDF& f, V3 wo, V3 wi) {
             float D = beckmannD(h, f.a);
             float G = beckmannG1(wo, f.a) * beckmannG1(wi, f.a);
             V3 F = evalFresnel(f.fresnel, dot(wo, h));
-            float k = D * G / (4.0f * absCosT(wo) * absCosT(wi));
+            float k = D * G / ((planted(kPlantBeckmannNo4) ? 1.0f : 4.0f) * absCosT(wo) * absCosT(wi));
             return (f.R * F) * k;
         }
         default: break;
@@ -863,17 +884,17 @@ V3 sampleFs(const BSDF& B, V3 vout, float uc, float u0, float u1, V3* vin, float
     uint32_t flags = BxDF_All;
     uint32_t nm = (uint32_t)B.n;
     V3 wo = worldToLocal(B.S, B.T, B.N, vout);
-    uint32_t which = randUInt(uc, (uint32_t)B.n);
+    uint32_t which = planted(kPlantMixPickFirst) ? 0u : randUInt(uc, (uint32_t)B.n);
     const BxDF& f = B.c[which];
     V3 wi{};
     V3 ret = sampleFx(f, wo, u0, u1, &wi, pdf);
     if (*pdf == 0.0f) { *sampledType = 0; return mk(0, 0, 0); }
     *sampledType = f.type;
     *vin = localToWorld(B.S, B.T, B.N, wi);
-    if (!(f.type & BxDF_Specular) && nm > 1)
+    if (!(f.type & BxDF_Specular) && nm > 1 && !planted(kPlantMixOwnPdfOnly))
         for (int i = 0; i < B.n; ++i)
             if ((uint32_t)i != which && matchType(B.c[i], flags)) *pdf += fxPdf(B.c[i], wo, wi);
-    *pdf /= (float)nm;
+    if (!planted(kPlantMixNoAverage)) *pdf /= (float)nm;
     if (!(f.type & BxDF_Specular)) {
         ret = mk(0, 0, 0);
         if (dot(*vin, B.NG) * dot(vout, B.NG) > 0) flags &= ~BxDF_Transmission;
@@ -898,7 +919,7 @@ float fsPdf(const BSDF& B, V3 vout, V3 vin) {
     float pdf = 0.0f;
     for (int i = 0; i < B.n; ++i)
         if (matchType(B.c[i], BxDF_All)) pdf += fxPdf(B.c[i], wo, wi);
-    return pdf / (float)B.n;
+    return planted(kPlantMixNoAverage) ? pdf : pdf / (float)B.n;
 }
 inline float absCosNsBSDF(const BSDF& B, V3 v) { return std::fabs(dot(B.N, v)); }
 
@@ -947,7 +968,9 @@ V3 Le(const EDF& E, V3 vout) {
     V3 r = mk(0, 0, 0);
     if (!E.env) {
         V3 wo = worldToLocal(E.S, E.T, E.N, vout);
-        for (int i = 0; i < E.n; ++i) r += wo.z > 0.0f ? E.L[i] / kPi : mk(0, 0, 0);
+        for (int i = 0; i < E.n; ++i)
+            r += (wo.z > 0.0f || planted(kPlantLeTwoSided)) ? (planted(kPlantLeNoInvPi) ? E.L[i] : E.L[i] / kPi)
+                                                            : mk(0, 0, 0);
     } else {
         for (int i = 0; i < E.n; ++i) r += E.L[i];
     }
@@ -957,6 +980,7 @@ inline float absCosNsEDF(const EDF& E, V3 v) { return !E.env ? std::fabs(dot(E.N
 
 void sampleLightPos(const SceneC& s, float ul, float u0, float u1, LightPoint* lp, EDF* E, float* areaPDF) {
     clr_light_info li = s.lights[sampleDiscrete1D(s.lightDist, ul, areaPDF)];
+    if (planted(kPlantNoLightChoiceProb)) *areaPDF = 1.0f;
     uint32_t prop = 0xFFFFu;
     if (li.atInfinity) {
         lp->atInfinity = true;
@@ -989,7 +1013,7 @@ void sampleLightPos(const SceneC& s, float ul, float u0, float u1, LightPoint* l
         float b0 = 1.0f - su, b1 = u1 * su, b2 = 1.0f - b0 - b1;
         V3 ng = cross(p1 - p0, p2 - p0);
         float area = 0.5f * length(ng);
-        *areaPDF *= 1.0f / area;
+        if (!planted(kPlantLightNoInvArea) && !planted(kPlantAreaPdfNoInvArea)) *areaPDF *= 1.0f / area;
         lp->p = b0 * p0 + b1 * p1 + b2 * p2;
         lp->gNormal = normalize(ng);
         bool hasVN = f.vn0 != 0xFFFFFFFFu && f.vn1 != 0xFFFFFFFFu && f.vn2 != 0xFFFFFFFFu;
@@ -1039,9 +1063,10 @@ float getAreaPDF(const SceneC& s, const LightPoint& lp) {
         const clr_face& f = s.faces[lp.faceID];
         V3 p0 = vtx(s.vertices, f.p0), p1 = vtx(s.vertices, f.p1), p2 = vtx(s.vertices, f.p2);
         V3 ng = cross(p1 - p0, p2 - p0);
-        float areaPDF = 1.0f / (0.5f * length(ng));
+        float areaPDF = planted(kPlantAreaPdfNoInvArea) ? 1.0f : 1.0f / (0.5f * length(ng));
         for (uint32_t i = 0; i < nItems; ++i)
-            if (s.lights[i].reference == lp.faceID) return areaPDF * probDiscrete1D(s.lightDist, i);
+            if (s.lights[i].reference == lp.faceID)
+                return areaPDF * (planted(kPlantNoLightChoiceProb) ? 1.0f : probDiscrete1D(s.lightDist, i));
     }
     return 0.0f;
 }
@@ -1100,8 +1125,22 @@ void cameraRay(const SceneC& s, float px, float py, float lu0, float lu1, V3* or
 
 struct RenderArgs {
     uint32_t width, height, maxDepth;
-    bool noRR;  // CLR_RENDER_NO_RR
+    bool noRR;       // CLR_RENDER_NO_RR
+    bool lightOnly;  // CLR_RENDER_LIGHT_SAMPLING_ONLY
+    bool bsdfOnly;   // CLR_RENDER_BSDF_SAMPLING_ONLY
 };
+RenderArgs renderArgs(uint32_t width, uint32_t height, uint32_t maxDepth, uint32_t flags) {
+    return {width, height, maxDepth, (flags & CLR_RENDER_NO_RR) != 0, (flags & CLR_RENDER_LIGHT_SAMPLING_ONLY) != 0,
+            (flags & CLR_RENDER_BSDF_SAMPLING_ONLY) != 0};
+}
+// Weight of an emission term reached by a diffuse or glossy BSDF sample
+// (pathtracer.cl:154-163; per strategy: 0 with light samples alone, 1 with
+// BSDF samples alone).
+inline float emissionHitMIS(const RenderArgs& ra, float dirPDF, float lightPDF) {
+    if (ra.lightOnly) return 0.0f;
+    if (ra.bsdfOnly) return 1.0f;
+    return (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+}
 
 // Observer of a path's radiance terms (the adjoint's differentiates them).
 struct NoHook {
@@ -1149,25 +1188,30 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
             float dist2;
             if (hasNonSpecular(B)) {
                 float lu = getFloat(rds), l0 = getFloat(rds), l1 = getFloat(rds);
-                lp = LightPoint();
-                sampleLightPos(s, lu, l0, l1, &lp, &E, &lightPDF);
-                V3 vinL = lp.p - sp.p;
-                dist2 = dot(vinL, vinL);
-                vinL = vinL * (1.0f / std::sqrt(dist2));
-                if (lp.atInfinity) { vinL = lp.p; dist2 = 1.0f; }
-                V3 sorg = sp.p + vinL * kEps;
-                SurfacePoint lsp;
-                bool hit = rayIntersection(s, sorg, vinL, &lsp);
-                bool visible = lp.atInfinity ? !hit : (hit && lsp.faceID == lp.faceID);  // D4
-                if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {  // D11
-                    V3 vinLrev = -vinL;
-                    float fsPDF = fsPdf(B, vout, vinL) * absCosNsEDF(E, vinL) / dist2;
-                    MIS = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
-                    float w = MIS * absCosNsEDF(E, vinL) * absCosNsBSDF(B, vinL) / dist2 / lightPDF;
-                    const V3 term = w * ((alpha * Le(E, vinLrev)) * fs(B, vout, vinL));
-                    pix += term;
-                    hook.lightSample(term, lp.atInfinity ? 0xFFFFFFFFu : (uint32_t)s.faces[lp.faceID].lightPtr,
-                                     face->matPtr, B, vout, vinL, sp);
+                if (!ra.bsdfOnly) {  // light sample (skipped, its numbers drawn, with BSDF samples alone)
+                    lp = LightPoint();
+                    sampleLightPos(s, lu, l0, l1, &lp, &E, &lightPDF);
+                    V3 vinL = lp.p - sp.p;
+                    dist2 = dot(vinL, vinL);
+                    vinL = vinL * (1.0f / std::sqrt(dist2));
+                    if (lp.atInfinity) { vinL = lp.p; dist2 = 1.0f; }
+                    V3 sorg = sp.p + vinL * kEps;
+                    SurfacePoint lsp;
+                    bool hit = rayIntersection(s, sorg, vinL, &lsp);
+                    bool visible = lp.atInfinity ? !hit : (hit && lsp.faceID == lp.faceID);  // D4
+                    if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {  // D11
+                        V3 vinLrev = -vinL;
+                        float fsPDF = fsPdf(B, vout, vinL) * (planted(kPlantMisNoCosL) ? 1.0f : absCosNsEDF(E, vinL)) / dist2;
+                        if (ra.lightOnly) MIS = 1.0f;
+                        else if (planted(kPlantNeeMisSwapped)) MIS = (fsPDF * fsPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
+                        else MIS = (lightPDF * lightPDF) / (lightPDF * lightPDF + fsPDF * fsPDF);
+                        const float cosL = planted(kPlantNeeNoCosL) ? 1.0f : absCosNsEDF(E, vinL);
+                        float w = MIS * cosL * absCosNsBSDF(B, vinL) / dist2 / lightPDF;
+                        const V3 term = w * ((alpha * Le(E, vinLrev)) * fs(B, vout, vinL));
+                        pix += term;
+                        hook.lightSample(term, lp.atInfinity ? 0xFFFFFFFFu : (uint32_t)s.faces[lp.faceID].lightPtr,
+                                         face->matPtr, B, vout, vinL, sp);
+                    }
                 }
             }
             traceContinue = true;
@@ -1190,8 +1234,9 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
                 if (sampledType & BxDF_Non_Singular) {
                     V3 dd = sp.p - org;
                     float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
+                    if (planted(kPlantImplicitNoDist2)) d2 = 1.0f;
                     lightPDF = getAreaPDF(s, lp) * d2 / absCosNsEDF(E, vout);
-                    MIS = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+                    MIS = emissionHitMIS(ra, dirPDF, lightPDF);
                 }
                 const V3 term = (MIS * alpha) * Le(E, vout);
                 pix += term;
@@ -1221,7 +1266,7 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
         MIS = 1.0f;
         if ((sampledType & BxDF_Non_Singular) != 0 && depth > 0) {
             lightPDF = getAreaPDF(s, lp) * 1.0f / absCosNsEDF(E, vout);
-            MIS = (dirPDF * dirPDF) / (lightPDF * lightPDF + dirPDF * dirPDF);
+            MIS = emissionHitMIS(ra, dirPDF, lightPDF);
         }
         pix += (MIS * alpha) * Le(E, vout);
     }
@@ -1351,7 +1396,7 @@ int orc_render_adjoint(const void* const* arrays, uint32_t numFaces, uint32_t wi
     s.environment = s.others + rd<uint32_t>(s.others + 4);
     s.lightDist = s.others + rd<uint32_t>(s.others + 8);
     s.ordered = (flags & CLR_RENDER_REFERENCE_TRAVERSAL) == 0;
-    RenderArgs ra{width, height, maxDepth, (flags & CLR_RENDER_NO_RR) != 0};
+    const RenderArgs ra = renderArgs(width, height, maxDepth, flags);
     std::vector<int> table(65536, -1);
     for (uint32_t k = 0; k < numMat; ++k) table[matOffsets[k] & 0xFFFFu] = (int)k;
     V3 M = mk(0, 0, 0);
@@ -1496,7 +1541,7 @@ int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uin
     s.camera = s.others + rd<uint32_t>(s.others + 0);
     s.environment = s.others + rd<uint32_t>(s.others + 4);
     s.lightDist = s.others + rd<uint32_t>(s.others + 8);
-    RenderArgs ra{width, height, maxDepth, (flags & CLR_RENDER_NO_RR) != 0};
+    const RenderArgs ra = renderArgs(width, height, maxDepth, flags);
     if (pixEnd > width * height) pixEnd = width * height;
     int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<uint32_t> next{pixBegin};
@@ -1627,6 +1672,71 @@ int orc_texture_probe(const uint8_t* record, float u, float v, float* out) {
 
 int orc_disk_probe(float u1, float u2, float* out) {
     concentricSampleDisk(u1, u2, out, out + 1);
+    return 0;
+}
+
+// Plants mistake k (the Plant list above) in the restatement until the next
+// call; 0 removes it. Tests use it to show that a pin detects a mistake.
+int orc_set_plant(int k) {
+    g_plant = k;
+    return 0;
+}
+
+// Camera probe over many film points: film[4i..4i+3] = (px, py, lu0, lu1);
+// out[7i..7i+6] = origin, direction, direction pdf (orc_camera_ray), and
+// out[7n..7n+2] = the camera's +z axis in world space (idfDir of tracePath).
+int orc_camera_rays(const void* const* arrays, uint32_t n, const float* film, float* out) {
+    SceneC s{};
+    s.others = (const uint8_t*)arrays[8];
+    s.camera = s.others + rd<uint32_t>(s.others + 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 o, d;
+        float pdf;
+        cameraRay(s, film[4 * i], film[4 * i + 1], film[4 * i + 2], film[4 * i + 3], &o, &d, &pdf);
+        float* r = out + 7 * (size_t)i;
+        r[0] = o.x; r[1] = o.y; r[2] = o.z;
+        r[3] = d.x; r[4] = d.y; r[5] = d.z;
+        r[6] = pdf;
+    }
+    const V3 z = mulV3(s.camera + 64, mk(0.0f, 0.0f, 1.0f));
+    out[7 * (size_t)n] = z.x; out[7 * (size_t)n + 1] = z.y; out[7 * (size_t)n + 2] = z.z;
+    return 0;
+}
+
+// BSDF mixture probe: the material at byte offset matOffset of the scene's
+// materialsData, allocated (bsdfAlloc) at a surface point with geometric and
+// shading normal +z, uv (0, 0) and no tangent. Per i (directions in world
+// space): out[12i..] = fs(wo_i, wi_i) (3), fs_pdf(wo_i, wi_i), then
+// sample_fs(wo_i, uc, u0, u1 = u[3i..3i+2]): direction (3), pdf, value (3)
+// and the sampled element's type bits.
+int orc_bsdf_probe(const void* const* arrays, uint32_t matOffset, uint32_t n, const float* wo, const float* wi,
+                   const float* u, float* out) {
+    SceneC s{};
+    s.materials = (const uint8_t*)arrays[6];
+    s.textures = (const uint8_t*)arrays[7];
+    s.others = (const uint8_t*)arrays[8];
+    SurfacePoint sp{};
+    sp.p = mk(0, 0, 0);
+    sp.gNormal = sp.sNormal = mk(0, 0, 1);
+    sp.uDir = mk(1, 0, 0);
+    sp.hasTangent = false;
+    BSDF B;
+    bsdfAlloc(s, matOffset, sp, &B);
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 vo = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), vi = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        float* r = out + 12 * (size_t)i;
+        const V3 f = fs(B, vo, vi);
+        r[0] = f.x; r[1] = f.y; r[2] = f.z;
+        r[3] = fsPdf(B, vo, vi);
+        V3 sd{};
+        float spdf = 0.0f;
+        uint32_t type = 0;
+        const V3 sf = sampleFs(B, vo, u[3 * i], u[3 * i + 1], u[3 * i + 2], &sd, &spdf, &type);
+        r[4] = sd.x; r[5] = sd.y; r[6] = sd.z;
+        r[7] = spdf;
+        r[8] = sf.x; r[9] = sf.y; r[10] = sf.z;
+        r[11] = (float)type;
+    }
     return 0;
 }
 

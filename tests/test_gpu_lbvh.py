@@ -180,7 +180,8 @@ def run_segmented(ctx, soup, offs, bits=10):
     k = len(offs) - 1
     inodes = torch.zeros(max(n - k, 1) * 48, dtype=torch.uint8, device="cuda")
     leaves = torch.zeros(n * 48, dtype=torch.uint8, device="cuda")
-    ca.LBVHBuilder(ctx, n, bits).perform_segmented(torch.from_numpy(soup).cuda(), offs, bits,
+    offs_dev = torch.from_numpy(np.asarray(offs, np.uint32).view(np.int32)).cuda()
+    ca.LBVHBuilder(ctx, n, bits).perform_segmented(torch.from_numpy(soup).cuda(), offs_dev, n, bits,
                                                    inodes, leaves)
     torch.cuda.synchronize()
     assert ctx.device_status() == 0
@@ -225,12 +226,40 @@ def test_segmented_256_scenes_match_single_builds(gpu_ctx):
 
 
 def test_segmented_rejects_bad_offsets(gpu_ctx):
+    """Offsets live on the device and are checked there: an empty scene, a
+    first offset other than 0 or a last one other than numTris raise status
+    bit 4 and leave the outputs untouched; more scenes than triangles is
+    rejected on the host."""
     import torch
 
-    soup = ca.gen_cluster_soup(100, seed=1)
+    soup = torch.from_numpy(ca.gen_cluster_soup(100, seed=1)).cuda()
     inodes = torch.zeros(100 * 48, dtype=torch.uint8, device="cuda")
     leaves = torch.zeros(100 * 48, dtype=torch.uint8, device="cuda")
     b = ca.LBVHBuilder(gpu_ctx, 100, 10)
-    for offs in ([0, 50, 50, 100], [1, 100], [0, 60, 40]):
-        with pytest.raises(ca.ClrError):
-            b.perform_segmented(torch.from_numpy(soup).cuda(), np.array(offs, np.uint32), 10, inodes, leaves)
+    assert gpu_ctx.device_status() == 0
+    for offs in ([0, 50, 50, 100], [1, 100], [0, 60, 40], [0, 40, 30, 100]):
+        print(f"[lbvh segmented] rejected offsets {offs}", flush=True)
+        dev = torch.tensor(offs, dtype=torch.int32, device="cuda")
+        b.perform_segmented(soup, dev, 100, 10, inodes, leaves)
+        torch.cuda.synchronize()
+        assert gpu_ctx.device_status() == 16
+        assert int(inodes.count_nonzero()) == 0 and int(leaves.count_nonzero()) == 0
+    with pytest.raises(ca.ClrError):
+        b.perform_segmented(soup, torch.tensor([0, 1, 2, 3], dtype=torch.int32, device="cuda"), 2, 10, inodes,
+                            leaves)
+
+
+def test_segmented_large_scenes_match_oracle(gpu_ctx):
+    """Scenes of the batched workload's upper range (300k-500k triangles)
+    beside small ones, each byte-identical to the oracle's build of it."""
+    sizes = [499388, 1007, 312500, 20000, 431000]
+    soup, offs, soups = segmented_soup(sizes, 120)
+    print(f"[lbvh segmented] large scenes {sizes}", flush=True)
+    gi, gl = run_segmented(gpu_ctx, soup, offs)
+    for s, sp in enumerate(soups):
+        n = sp.shape[0]
+        v, f = ca.soup_to_mesh(sp)
+        o = orc.lbvh_build(v, f, 10)
+        b = int(offs[s])
+        np.testing.assert_array_equal(gl[b:b + n], o["leaves"], err_msg=f"scene {s} leaves")
+        np.testing.assert_array_equal(gi[b - s:b - s + n - 1], o["inodes"], err_msg=f"scene {s} inodes")

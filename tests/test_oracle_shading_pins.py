@@ -119,6 +119,51 @@ def test_ward_anisotropy_axes():
     assert fx / fy == pytest.approx(math.exp(-t2 * (1 / ax ** 2 - 1 / ay ** 2)), rel=1e-4)
 
 
+# ------------------------------------------------------------ Beckmann ----
+def _fresnel_dielectric(cos_i, eta):
+    """Unpolarised Fresnel reflectance from vacuum into a dielectric of
+    index eta (Fresnel's equations)."""
+    sin_t = math.sqrt(max(0.0, 1.0 - cos_i * cos_i)) / eta
+    cos_t = math.sqrt(1.0 - sin_t * sin_t)
+    rs = (cos_i - eta * cos_t) / (cos_i + eta * cos_t)
+    rp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.1])
+def test_beckmann_albedo_at_normal_incidence(alpha):
+    """Beckmann coat values (addition D8): with wo at the normal,
+    f cos(theta_i) dwi = F(cos theta_h) D(h) G cos(theta_h) dh, and the
+    half-vector density D cos has unit mass with tan^2(theta_h) exponential
+    of mean alpha^2. For alpha <= 0.1 the shadowing G is 1 wherever D has
+    mass, so without Fresnel the albedo is the mass with theta_i < 90 deg,
+    1 - exp(-1 / alpha^2), and with a dielectric coat it is the Fresnel
+    reflectance at normal incidence, ((eta - 1) / (eta + 1))^2, up to
+    O(alpha^2) from the spread of theta_h."""
+    R = 0.9
+    got = _albedo_normal_incidence(6, alpha, 0.0, [R, R, R])
+    assert got == pytest.approx(R * (1.0 - math.exp(-1.0 / alpha ** 2)), rel=2e-4)
+    got = _albedo_normal_incidence(6, alpha, 1.5, [R, R, R])
+    assert got == pytest.approx(R * _fresnel_dielectric(1.0, 1.5), rel=2e-3)
+    assert _fresnel_dielectric(1.0, 1.5) == pytest.approx(0.04, rel=1e-12)
+
+
+@pytest.mark.parametrize("theta_o", [0.5, 1.0])
+def test_beckmann_narrow_lobe_at_oblique_incidence(theta_o):
+    """A narrow coat lobe (alpha 0.02) at oblique incidence reflects the
+    Fresnel fraction F(cos theta_o): f cos_i dwi = F D G (wo.h) / cos_o dh,
+    where (wo.h) / cos_o and F(wo.h) deviate from 1 and F(cos_o) at first
+    order in the tilt, which averages out over the symmetric lobe."""
+    got = _albedo_oblique(6, 0.02, 1.5, theta_o, 0.15)
+    assert got == pytest.approx(_fresnel_dielectric(math.cos(theta_o), 1.5), rel=5e-3)
+
+
+def test_beckmann_pin_detects_planted_mistake():
+    with orc.planted(orc.PLANT_BECKMANN_NO_4):
+        got = _albedo_normal_incidence(6, 0.05, 0.0, [1.0, 1.0, 1.0])
+    assert abs(got - (1.0 - math.exp(-400.0))) > 0.5
+
+
 # ---------------------------------------------------- Ashikhmin-Shirley ----
 def _as_spec_albedo(n, R):
     """Albedo of the Ashikhmin-Shirley specular lobe at normal incidence with
