@@ -84,9 +84,26 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+CPP_DRIVER_SRC = ROOT / "tests" / "cpp" / "wrapper_driver.cpp"
+CPP_DRIVER = ROOT / "tests" / "cpp" / "wrapper_driver"
+
+
+def build_cpp_driver() -> Path:
+    """Host-only C++ driver of the include/clear_amd.hpp wrappers (run by a
+    GPU test), linked against libclear_amd.so by a relative rpath."""
+    rocm = Path(HIPCC).resolve().parent.parent
+    tmp = CPP_DRIVER.with_suffix(".tmp")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{rocm / 'include'}",
+          f"-I{ROOT / 'include'}", str(CPP_DRIVER_SRC), "-o", str(tmp), f"-L{LIB.parent}", "-lclear_amd",
+          f"-L{rocm / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../clear_amd", f"-Wl,-rpath,{rocm / 'lib'}"])
+    os.replace(tmp, CPP_DRIVER)
+    return CPP_DRIVER
+
+
 def build_all(jobs: int = 8) -> None:
     build_hip(jobs)
     build_oracle()
+    build_cpp_driver()
 
 
 if __name__ == "__main__":

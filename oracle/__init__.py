@@ -63,6 +63,24 @@ def lbvh_build(verts: np.ndarray, faces: np.ndarray, bits: int):
             "scene": scene}
 
 
+def lbvh_build_parallel(verts: np.ndarray, faces: np.ndarray, bits: int, threads: int = 0):
+    """The same build on `threads` host threads (0: all): dict of inodes
+    ([n-1, 48] bytes) and leaves ([n, 48]), equal to lbvh_build's."""
+    fn = lib().orc_lbvh_build_parallel
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_int]
+    n = faces.shape[0]
+    verts = np.ascontiguousarray(verts, dtype=np.float32)
+    faces = np.ascontiguousarray(faces)
+    inodes = np.zeros((max(n - 1, 1), 48), dtype=np.uint8)
+    leaves = np.zeros((n, 48), dtype=np.uint8)
+    rc = fn(_p(verts), _p(faces), n, bits, _p(inodes), _p(leaves), threads)
+    if rc:
+        raise RuntimeError(f"orc_lbvh_build_parallel failed: {rc}")
+    return {"inodes": inodes[: n - 1], "leaves": leaves}
+
+
 def trace_closest(inodes, leaves, n, verts, faces, rays, stats=False):
     R = rays.shape[1]
     hit = np.zeros(R, dtype=np.uint32)
@@ -76,6 +94,15 @@ def trace_closest(inodes, leaves, n, verts, faces, rays, stats=False):
     if rc:
         raise RuntimeError(f"orc_trace_closest failed: {rc}")
     return (hit, t, st) if stats else (hit, t)
+
+
+def trace_overflows(reset=True) -> int:
+    """Rays whose reference walk (trace_closest) dropped a push because its
+    64-entry stack was full, since the last reset."""
+    fn = lib().orc_trace_overflows
+    fn.restype = ctypes.c_uint64
+    fn.argtypes = [ctypes.c_int]
+    return int(fn(int(reset)))
 
 
 def trace_ordered(inodes, leaves, n, verts, faces, rays, any_hit=False):
@@ -232,6 +259,20 @@ def _probe_fn(name, argtypes):
     fn.restype = ctypes.c_int
     fn.argtypes = argtypes
     return fn
+
+
+def work_counters(reset=True, count_any_hit=True):
+    """Traversal work of oracle renders (ordered walk) since the last reset:
+    dict of rays, visits, tests (closest-hit walks) and, for environment
+    shadow rays, their count, closest-hit visits / tests and the visits /
+    tests of an any-hit walk of the same rays. count_any_hit makes later
+    renders count the any-hit walks."""
+    fn = _probe_fn("orc_work_counters", [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+    out = np.zeros(8, np.uint64)
+    fn(out.ctypes.data, int(reset), int(count_any_hit))
+    keys = ["rays", "visits", "tests", "env_shadow_rays", "env_shadow_visits", "env_shadow_tests",
+            "env_shadow_anyhit_visits", "env_shadow_anyhit_tests"]
+    return {k: int(v) for k, v in zip(keys, out)}
 
 
 def bxdf_probe(kind, p0, p1, R, wo, wi, u0, u1):

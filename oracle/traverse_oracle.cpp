@@ -13,6 +13,7 @@
 //
 // Pinned by orc_trace_bruteforce (an O(N) loop of the same triangle test over
 // every face, no boxes) in tests/test_oracle_traverse.py.
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -21,6 +22,8 @@
 #include "ordered_traversal.h"
 
 namespace {
+
+std::atomic<uint64_t> g_overflowedRays{0};
 
 struct V3 {
     float x, y, z;
@@ -103,6 +106,7 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
         float t = rays[6 * (size_t)numRays + r];
         uint32_t best = 0xFFFFFFFFu;
         uint32_t visits = 0, tests = 0;
+        bool overflowed = false;
         if (n == 1) {
             if (boxTest(leaves[0].min, leaves[0].max, org, dir, t)) {
                 ++tests;
@@ -119,7 +123,12 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
                 ++visits;
                 for (int i = 0; i < 2; ++i) {
                     if (!N.isChild[i]) {
-                        if (depth >= 64) return 2;  // stack bound exceeded
+                        // A push beyond the 64 entries is dropped, as the GPU walk
+                        // drops it (raising its overflow status); counted per ray.
+                        if (depth >= 64) {
+                            overflowed = true;
+                            continue;
+                        }
                         stack[depth++] = N.c[i];
                         continue;
                     }
@@ -130,6 +139,7 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
                 }
             }
         }
+        if (overflowed) ++g_overflowedRays;
         bool hit = best != 0xFFFFFFFFu && !std::isinf(t);
         hitFace[r] = hit ? best : 0xFFFFFFFFu;
         hitT[r] = t;
@@ -139,6 +149,14 @@ int orc_trace_closest(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
         }
     }
     return 0;
+}
+
+// Rays whose reference walk (orc_trace_closest) dropped a push since the last
+// reset.
+uint64_t orc_trace_overflows(int reset) {
+    const uint64_t v = g_overflowedRays;
+    if (reset) g_overflowedRays = 0;
+    return v;
 }
 
 // rayIntersection over the CPU midpoint BVH (bvh_traversal.cl:256-285):

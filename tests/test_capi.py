@@ -107,3 +107,20 @@ def test_cluster_scene_layout_and_batch_soup():
         a, b = orc.lbvh_build(h.vertices, h.faces, 10), orc.lbvh_build(v, f, 10)
         np.testing.assert_array_equal(a["leaves"], b["leaves"])
         np.testing.assert_array_equal(a["inodes"], b["inodes"])
+
+
+def test_cpp_wrapper_header_compiles_standalone(tmp_path):
+    """include/clear_amd.hpp (the C++ LBVHBuilder / render wrappers) needs
+    only clear_amd.h and the C++ standard library."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "clear_amd.hpp"\n'
+                   'void use(clr::Context& c, clr_float3* v, clr_face* f, clr_lbvh_inode* i, clr_lbvh_leaf* l) {\n'
+                   '    clr::LBVHBuilder b(c, 16);\n    b.perform(nullptr, v, f, 16, 10, i, l);\n}\n')
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{inc}", str(src)], check=True)

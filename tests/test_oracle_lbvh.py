@@ -172,3 +172,27 @@ def test_pins_catch_planted_mistakes():
     bad_sorted[[10, 11]] = bad_sorted[[11, 10]]
     keys = orc.interleave(o["morton"], 10)
     assert not np.array_equal(bad_sorted, orc.stable_order(keys))
+
+
+@pytest.mark.parametrize("case", ["clusters", "duplicates", "identical", "nan", "flat"])
+@pytest.mark.parametrize("bits", [10, 21])
+def test_parallel_build_equals_serial_build(case, bits):
+    """The multi-threaded oracle build (bench.py's all-cores CPU baseline)
+    gives the serial restatement's nodes byte for byte."""
+    if case == "clusters":
+        soup = ca.gen_cluster_soup(100003, seed=5)
+    elif case == "duplicates":
+        soup = soup_for(21, 5000, dup=True)
+    elif case == "identical":
+        soup = np.tile(np.float32([0.1, 0.2, 0.3, 0.2, 0.3, 0.4, 0.3, 0.1, 0.2]), (777, 1))
+    elif case == "nan":
+        soup = soup_for(22, 3000)
+        soup[17, 4] = np.nan
+    else:
+        soup = soup_for(23, 4000)
+        soup[:, 1::3] = np.float32(0.5)
+    o, v, f = build(soup, bits)
+    for threads in (1, 3, 8):
+        p = orc.lbvh_build_parallel(v, f, bits, threads)
+        np.testing.assert_array_equal(p["leaves"], o["leaves"])
+        np.testing.assert_array_equal(p["inodes"], o["inodes"])
