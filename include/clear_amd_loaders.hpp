@@ -334,7 +334,11 @@ inline bool loadEXR(const char* fileName, std::vector<uint8_t>* storage, uint32_
         pos += size;
     }
     if (box[2] < box[0] || box[3] < box[1]) return fail("empty data window");
-    const uint32_t w = (uint32_t)(box[2] - box[0] + 1), h = (uint32_t)(box[3] - box[1] + 1);
+    // Window extents in 64-bit arithmetic (int32 corners can be 2^32 apart),
+    // capped like the PNG reader's limit.
+    const int64_t w64 = (int64_t)box[2] - box[0] + 1, h64 = (int64_t)box[3] - box[1] + 1;
+    if (w64 > 16384 || h64 > 16384) return fail("image size outside 1..16384");
+    const uint32_t w = (uint32_t)w64, h = (uint32_t)h64;
     if (compression < 0 || compression > 3) return fail("only NONE, RLE, ZIPS and ZIP compression are supported");
     const uint32_t linesPerChunk = compression == 3 ? 16u : 1u;
     size_t lineBytes = 0;
@@ -356,12 +360,12 @@ inline bool loadEXR(const char* fileName, std::vector<uint8_t>* storage, uint32_
     std::vector<uint8_t> tmp, unpacked;
     for (uint32_t ci = 0; ci < chunks; ++ci) {
         const uint64_t off = le64(&f[tableAt + 8ull * ci]);
-        if (off + 8 > f.size()) return fail("bad chunk offset");
+        if (f.size() < 8 || off > f.size() - 8) return fail("bad chunk offset");  // no wrap near 2^64
         const int32_t y0 = (int32_t)le32(&f[off]);
         const uint32_t dsize = le32(&f[off + 4]);
         if (dsize > f.size() - off - 8) return fail("truncated chunk");
         if (y0 < box[1] || y0 > box[3]) return fail("chunk outside the data window");
-        const uint32_t lines = std::min<uint32_t>(linesPerChunk, (uint32_t)(box[3] - y0 + 1));
+        const uint32_t lines = (uint32_t)std::min<int64_t>(linesPerChunk, (int64_t)box[3] - y0 + 1);
         const size_t expect = lineBytes * lines;
         const uint8_t* d = &f[off + 8];
         const uint8_t* data = d;

@@ -8,6 +8,7 @@ FLOAT channels, hand-worked face lists for the OBJ fixture), and against the
 oracle's independent restatement (oracle/assets_oracle.py) on the same files.
 """
 import math
+import struct
 import os
 
 import numpy as np
@@ -212,6 +213,28 @@ def test_exr_rejects(tmp_path):
     p.write_bytes(bytes(b))
     with pytest.raises(ca.ClrError, match="compression"):
         ca.load_image(p)
+
+
+def test_exr_rejects_malformed_offsets_and_windows(tmp_path):
+    """A chunk offset near 2^64 (where off + 8 wraps) and a data window whose
+    int32 corners are 2^32 - 1 apart are rejected, not read."""
+    p = tmp_path / "ok.exr"
+    af.write_exr(p, {"R": np.zeros((2, 2), np.float16)}, af.NONE)
+    good = p.read_bytes()
+    table = good.index(b"\0", good.rindex(b"screenWindowWidth")) + 1 + len("float") + 1 + 4 + 4 + 1
+    q = tmp_path / "offset.exr"
+    b = bytearray(good)
+    b[table:table + 8] = struct.pack("<Q", 2 ** 64 - 4)
+    q.write_bytes(bytes(b))
+    with pytest.raises(ca.ClrError, match="chunk offset"):
+        ca.load_image(q)
+    q = tmp_path / "window.exr"
+    b = bytearray(good)
+    at = b.index(b"dataWindow\0box2i\0") + len(b"dataWindow\0box2i\0") + 4
+    b[at:at + 16] = struct.pack("<iiii", -(2 ** 31), 0, 2 ** 31 - 1, 1)
+    q.write_bytes(bytes(b))
+    with pytest.raises(ca.ClrError, match="image size"):
+        ca.load_image(q)
 
 
 # ---- OBJ / MTL ------------------------------------------------------------------------
