@@ -9,7 +9,15 @@ synchronisation; value = Mtri/s. Per-stage times come from HIP events the
 library records on the build's stream; the roofline is that of the dominant
 kernel (k_hier_local) with the algorithmic bytes per triangle stated below.
 
+The roofline also reports the whole build's fraction of HBM peak on the
+build's algorithmic 324 B per triangle; its measured traffic comes from a
+committed PMC profile only when that profile was taken on the current kernel
+source (its sha256 is recorded with it).
+
 Secondary legs of the same metric family, each on its stated workload:
+  dropin   clr_lbvh_build (the LBVHBuilder::perform drop-in) from 16-B
+           vertices + 56-B Face records of the same 2^24 triangles; its nodes
+           must equal the soup build's;
   trace    2^22 random rays (origins in the unit cube, directions uniform,
            tmax = inf) against the 2^24 BVH: Mrays/s, reference-order walk and
            ordered walk;
@@ -17,14 +25,20 @@ Secondary legs of the same metric family, each on its stated workload:
            spheres, coat BSDF, normal map, 1024x512 HDR sky, thin lens 0.02) at
            1920x1080, 64 spp, max depth 8: Msamples/s of the plain megakernel
            (reference traversal), the megakernel (ordered traversal) and the
-           wavefront renderer (autotuned);
+           wavefront renderer (autotuned); the timed wavefront image must equal
+           the megakernel's bit for bit; the roofline uses the oracle's
+           traversal counts per camera sample (every 16th row, 2 spp);
   adjoint  the 200,000-triangle, 64-material differentiation scene, 512x512,
            32 spp, depth 5, Russian roulette off: forward and adjoint time;
-  cpu      the CPU oracle's serial LBVH build on the same soup (cpu_baseline)
-           and its renderer on a band of the outdoor scene (all host threads).
+  batch    256 cluster scenes: one segmented build vs 256 builds, batched vs
+           scene-by-scene rendering;
+  cpu      the CPU oracle on the host: LBVH build of the same soup on 1 thread
+           (cpu_baseline) and on all of the GPU's host threads, the reference
+           closest-hit walk on 1 thread and on all threads, and the renderer
+           with the reference walk on 1 thread and on all threads.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-trace]
-                    [--no-render] [--no-adjoint]
+                    [--no-render] [--no-adjoint] [--no-batch] [--no-dropin]
 """
 from __future__ import annotations
 
@@ -62,17 +76,29 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def kernel_source_sha() -> str:
+    """sha256 of the LBVH kernels' source (the traffic profile records it)."""
+    import hashlib
+
+    return hashlib.sha256((ROOT / "clear_amd" / "csrc" / "lbvh.hip").read_bytes()).hexdigest()
+
+
 def load_traffic():
     """HBM bytes per launch of k_hier_local from rocprofv3 PMC runs
     (FETCH_SIZE, WRITE_SIZE) scaled by a copy-kernel calibration, committed
-    under profiles/ (None if absent)."""
+    under profiles/ together with the sha256 of the kernel source it was
+    measured on. None if absent or measured on other kernel source."""
     p = ROOT / "profiles" / "lbvh_16m_pmc.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text())
+        tr = json.loads(p.read_text())
     except Exception:
         return None
+    if tr.get("lbvh_hip_sha256") != kernel_source_sha():
+        log("traffic profile was measured on other kernel source: traffic not reported")
+        return None
+    return tr
 
 
 def lbvh_leg(ca, np, torch, args, dev):
@@ -115,9 +141,45 @@ def lbvh_leg(ca, np, torch, args, dev):
                 "traffic": traffic, "algorithmic_bytes_per_launch": 184.0 * n,
                 "ms_per_launch": loc["ms"]}
     whole = BUILD_BYTES_PER_TRI * n / (dt / K) / 1e9
+    roofline["whole_build"] = {"achieved": round(whole, 1), "frac": round(whole / HBM_PEAK_GBS, 4),
+                               "algorithmic_bytes_per_tri": BUILD_BYTES_PER_TRI}
     return ctx, soup, tsoup, inodes, leaves, {
         "mtri_s": n * K / dt / 1e6, "ms_per_step": dt / K * 1e3, "stages": stage_info,
         "roofline": roofline, "build_effective_GB_s": round(whole, 1)}
+
+
+def dropin_leg(ca, np, torch, ctx, soup, ref_inodes, ref_leaves, args, dev):
+    """clr_lbvh_build, the drop-in for LBVHBuilder::perform, from 16-B
+    vertices + 56-B Face records of the same soup (vertex 3i + k = corner k
+    of triangle i): its bounds pass gathers each face's three vertex indices
+    and vertices (56 + 48 B) where the soup build reads 36 B. Its nodes must
+    equal the soup build's."""
+    n, bits, K, W = args.tris, args.bits, args.steps, args.warmup
+    verts, faces = ca.soup_to_mesh(soup)
+    tv = torch.from_numpy(verts).to(dev)
+    tf = torch.from_numpy(faces.view(np.uint8)).to(dev)
+    del verts, faces
+    inodes, leaves = ca.node_buffers(n, dev)
+    builder = ca.LBVHBuilder(ctx, n, bits)
+    for _ in range(W):
+        builder.perform(tv, tf, n, bits, inodes, leaves)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(K):
+        builder.perform(tv, tf, n, bits, inodes, leaves)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    same = torch.equal(inodes, ref_inodes) and torch.equal(leaves, ref_leaves)
+    assert same, "the drop-in build's nodes differ from the soup build's"
+    assert ctx.device_status() == 0
+    bpt = BUILD_BYTES_PER_TRI - 36.0 + 104.0
+    out = {"workload": f"clr_lbvh_build of the same {n} triangles as clr_float3 vertices + Face records",
+           "ms": round(ms, 4), "Mtri_s": round(n / (ms * 1e-3) / 1e6, 1), "algorithmic_bytes_per_tri": bpt,
+           "frac_hbm": round(bpt * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "nodes_equal_soup_build": same}
+    log(f"drop-in build: {out}")
+    return out
 
 
 def trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev):
@@ -175,9 +237,15 @@ def render_leg(ca, np, torch, ctx, args):
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     legs = [("megakernel_plain", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL),
             ("megakernel", ca.RENDER_MEGAKERNEL, 0), ("wavefront", ca.RENDER_WAVEFRONT, 0)]
+    images = {}
     for name, mode, flags in legs:
+        # Untimed warm-up render into scratch copies; the timed render starts
+        # from freshly seeded states and a zero image, so the legs' images can
+        # be compared.
         ca.seed_pixel_states(ctx, W, H, states)
         ca.render(ctx, ds, 1, depth, states, pixels, mode=mode, flags=flags)
+        ca.seed_pixel_states(ctx, W, H, states)
+        pixels.zero_()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -185,13 +253,44 @@ def render_leg(ca, np, torch, ctx, args):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1)
+        images[name] = (pixels.clone(), states.clone())
         out[name] = {"ms": round(ms, 2), "Msamples_s": round(W * H * spp / (ms * 1e-3) / 1e6, 1)}
         log(f"render {name}: {out[name]}")
     assert ctx.device_status() == 0
+    # Both renderers consume every pixel's random stream in the same order and
+    # add the same terms in the same order: the timed images are identical.
+    same = all(torch.equal(a, b) for a, b in zip(images["megakernel"], images["wavefront"]))
+    out["wavefront_equals_megakernel"] = bool(same)
+    assert same, "the timed wavefront image differs from the megakernel image"
+    del images
     out["wavefront_vs_megakernel"] = round(out["wavefront"]["Msamples_s"] / out["megakernel"]["Msamples_s"], 3)
     out["wavefront_vs_plain_megakernel"] = round(
         out["wavefront"]["Msamples_s"] / out["megakernel_plain"]["Msamples_s"], 3)
     return out, host
+
+
+def render_roofline(render, work, W, H, spp):
+    """Roofline of the render legs from the oracle's traversal counts per
+    camera sample (ordered walk, all rays cast: camera, bounce and shadow
+    rays; the wavefront renderer walks environment shadow rays any-hit), with
+    the traversal work model per node visit and triangle test. Shading flops
+    are not counted, so the fractions are lower bounds."""
+    samples = W * H * spp
+    per = work["samples"]
+    legs = {"megakernel": (work["visits"], work["tests"]),
+            "wavefront": (work["visits"] - work["env_shadow_visits"] + work["env_shadow_anyhit_visits"],
+                          work["tests"] - work["env_shadow_tests"] + work["env_shadow_anyhit_tests"])}
+    out = {"work_sample": work["sample"], "rays_per_sample": round(work["rays"] / per, 3)}
+    for name, (v, t) in legs.items():
+        V, T = v / per * samples, t / per * samples
+        flops = TRACE_FLOPS_V * V + TRACE_FLOPS_T * T
+        byts = TRACE_BYTES_V * V + TRACE_BYTES_T * T
+        roof = min(FP32_PEAK_TFLOPS, flops / byts * HBM_PEAK_GBS / 1e3)
+        tf_s = flops / (render[name]["ms"] * 1e-3) / 1e12
+        out[name] = {"bound": "latency (traversal)", "visits_per_sample": round(v / per, 2),
+                     "tests_per_sample": round(t / per, 2), "achieved": round(tf_s, 3), "peak": round(roof, 2),
+                     "unit": "TFLOP/s", "frac": round(tf_s / roof, 4)}
+    return out
 
 
 def adjoint_leg(ca, np, torch, ctx):
@@ -319,6 +418,11 @@ def batch_leg(ca, np, torch, ctx, args):
     return out
 
 
+def host_threads() -> int:
+    """The host threads one GPU's share of the box may use."""
+    return min(16, os.cpu_count() or 1)
+
+
 def cpu_lbvh(ca, soup, bits):
     import oracle as orc
 
@@ -326,7 +430,7 @@ def cpu_lbvh(ca, soup, bits):
     verts, faces = ca.soup_to_mesh(soup)
     log(f"cpu baseline: oracle LBVH build of {m} triangles")
     c0 = time.perf_counter()
-    orc.lbvh_build(verts, faces, bits)
+    built = orc.lbvh_build(verts, faces, bits)
     cdt = time.perf_counter() - c0
     # The reference's own CPU build (Scene::build's midpoint BVH, timed at
     # main.cpp:235-237), here the library's host restatement on the same soup.
@@ -334,32 +438,83 @@ def cpu_lbvh(ca, soup, bits):
     c0 = time.perf_counter()
     ca.build_midpoint_bvh(verts, faces)
     mdt = time.perf_counter() - c0
-    return {"value": round(m / cdt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
-            "sample": f"oracle LBVH build (serial C++) of the same {m}-triangle soup, "
-                      f"{cdt:.1f} s; host threads available: {os.cpu_count()}",
-            "midpoint_bvh_build": {"value": round(m / mdt / 1e6, 3), "unit": "Mtri/s", "cores": 1,
-                                   "sample": f"clr_bvh_build_midpoint (the reference's CPU BVH::build "
-                                             f"split rule) on the same soup, {mdt:.1f} s"}}
+    out = {"value": round(m / cdt / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+           "sample": f"oracle LBVH build (serial C++) of the same {m}-triangle soup, "
+                     f"{cdt:.1f} s; host threads available: {os.cpu_count()}",
+           "midpoint_bvh_build": {"value": round(m / mdt / 1e6, 3), "unit": "Mtri/s", "cores": 1,
+                                  "sample": f"clr_bvh_build_midpoint (the reference's CPU BVH::build "
+                                            f"split rule) on the same soup, {mdt:.1f} s"}}
+    par = getattr(orc, "lbvh_build_parallel", None)
+    if par is not None:
+        th = host_threads()
+        c0 = time.perf_counter()
+        par(verts, faces, bits, th)
+        pdt = time.perf_counter() - c0
+        out["all_cores"] = {"value": round(m / pdt / 1e6, 3), "unit": "Mtri/s", "cores": th,
+                            "sample": f"oracle LBVH build on {th} threads, same soup, {pdt:.1f} s"}
+    return out, built, verts, faces
 
 
-def cpu_render(ca, np, host, spp, depth):
+def cpu_trace(np, built, verts, faces, rays_np):
+    """The reference's closest-hit walk (bvh_traversal.cl:182-215, restated
+    by the oracle) over the oracle's build of the same soup: a bounded sample
+    of the benchmark's rays on one thread and on all the GPU's host threads
+    (chunks of rays on worker threads)."""
+    import concurrent.futures as cf
+
+    import oracle as orc
+
+    n = faces.shape[0]
+    out = {}
+    for label, threads, count in (("one_core", 1, 1 << 16), ("all_cores", host_threads(), 1 << 19)):
+        sample = np.ascontiguousarray(rays_np[:, :count])
+        chunks = np.array_split(np.arange(count), max(1, threads * 8))
+        c0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda idx: orc.trace_closest(built["inodes"], built["leaves"], n, verts, faces,
+                                                      np.ascontiguousarray(sample[:, idx])), chunks))
+        dt = time.perf_counter() - c0
+        out[label] = {"value": round(count / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                      "sample": f"first {count} of the benchmark rays, reference walk: {dt:.1f} s"}
+        log(f"cpu trace {label}: {out[label]}")
+    return out
+
+
+def cpu_render(ca, np, host, depth):
+    """The render restatement with the reference's walk (the CPU Equivalent
+    path's semantics; the reference loops over pixels on one thread,
+    main.cpp:416-443) on one thread and on all the GPU's host threads, each on
+    a band of the outdoor scene's middle rows; and the traversal work of the
+    ordered walk per camera sample on every 16th row (2 samples per pixel),
+    for the render roofline."""
     import oracle as orc
 
     W, H = host.width, host.height
-    rows = 96
-    threads = min(16, os.cpu_count() or 1)
     o = orc.lbvh_build(host.vertices, host.faces, 10)
+    out = {}
+    for label, threads, rows, spp in (("one_core", 1, 2, 4), ("all_cores", host_threads(), 24, 8)):
+        st = orc.seed_states(W, H)
+        px = np.zeros(W * H * 4, np.float32)
+        b = (H // 2) * W
+        c0 = time.perf_counter()
+        orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, spp, depth, st, px,
+                   pix_begin=b, pix_end=b + rows * W, threads=threads, flags=ca.RENDER_REFERENCE_TRAVERSAL)
+        dt = time.perf_counter() - c0
+        out[label] = {"value": round(rows * W * spp / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
+                      "kind": "port", "sample": f"render restatement, reference walk, {rows} middle rows of the "
+                                                f"{W}x{H} outdoor scene, {spp} spp, depth {depth}: {dt:.1f} s"}
+        log(f"cpu render {label}: {out[label]}")
     st = orc.seed_states(W, H)
     px = np.zeros(W * H * 4, np.float32)
-    b = (H // 2) * W
-    e = b + rows * W
-    c0 = time.perf_counter()
-    orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, spp, depth, st, px,
-               pix_begin=b, pix_end=e, threads=threads)
-    cdt = time.perf_counter() - c0
-    return {"value": round(rows * W * spp / cdt / 1e6, 4), "unit": "Msamples/s", "cores": threads,
-            "kind": "port", "sample": f"oracle renderer (ordered traversal) on {rows} middle rows of the "
-                                      f"1920x1080 outdoor scene, {spp} spp, depth {depth}: {cdt:.1f} s"}
+    orc.work_counters(reset=True, count_any_hit=True)
+    rows = list(range(8, H, 16))
+    for y in rows:
+        orc.render(host.arrays, host.num_faces, o["inodes"], o["leaves"], W, H, 2, depth, st, px,
+                   pix_begin=y * W, pix_end=(y + 1) * W, threads=host_threads())
+    work = orc.work_counters(reset=True, count_any_hit=False)
+    work["samples"] = len(rows) * W * 2
+    work["sample"] = f"oracle renders (ordered walk) of every 16th row ({len(rows)} rows x {W}), 2 spp, depth {depth}"
+    return out, work
 
 
 def main() -> int:
@@ -376,6 +531,7 @@ def main() -> int:
     ap.add_argument("--no-render", action="store_true", help="skip the render leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (Face + vertices) build leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true",
                     help="render legs use the default wavefront configuration (for kernel profiles)")
@@ -404,17 +560,22 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    dropin = None if args.no_dropin else dropin_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev)
     trace = None if args.no_trace else trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev)
     del tsoup
     render, host = (None, None) if args.no_render else render_leg(ca, np, torch, ctx, args)
     adjoint = None if args.no_adjoint else adjoint_leg(ca, np, torch, ctx)
     batch = None if args.no_batch else batch_leg(ca, np, torch, ctx, args)
-    cpu = cpu_r = None
+    cpu = cpu_r = cpu_t = None
     if not args.no_cpu:
-        cpu = cpu_lbvh(ca, soup, args.bits)
+        cpu, built, verts, faces = cpu_lbvh(ca, soup, args.bits)
+        if trace is not None:
+            cpu_t = cpu_trace(np, built, verts, faces, ca.gen_random_rays(args.rays, seed=2))
+        del built, verts, faces
         if host is not None:
-            cpu_r = cpu_render(ca, np, host, args.render_spp, 8)
-            log(f"cpu render: {cpu_r}")
+            cpu_r, work = cpu_render(ca, np, host, 8)
+            render["roofline"] = render_roofline(render, work, host.width, host.height, args.render_spp)
+            log(f"render roofline: {render['roofline']}")
 
     line = {
         "metric": "LBVH Mtri/s; path-trace Msamples/s wavefront vs megakernel; adjoint/fwd time",
@@ -437,7 +598,9 @@ def main() -> int:
         "cpu_baseline": cpu,
         "stages": lb["stages"],
         "build_effective_GB_s": lb["build_effective_GB_s"],
+        "dropin_build": dropin,
         "trace": trace,
+        "cpu_trace_baseline": cpu_t,
         "render": render,
         "adjoint": adjoint,
         "batch": batch,

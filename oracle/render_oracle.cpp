@@ -73,6 +73,23 @@ enum Plant : int {
 int g_plant = 0;
 inline bool planted(int k) { return g_plant == k; }
 
+// Traversal work of renders with the ordered walk, for the render roofline
+// (orc_work_counters): [0] rays traced, [1] internal-node visits, [2]
+// triangle tests, all as closest-hit walks; of those, environment shadow rays:
+// [3] count, [4] visits, [5] tests, and [6] visits, [7] tests of the any-hit
+// walk the wavefront renderer uses for them instead (counted on the side;
+// the restatement's results do not change).
+constexpr int kWorkWords = 8;
+thread_local uint64_t t_work[kWorkWords];
+std::atomic<uint64_t> g_work[kWorkWords];
+std::atomic<bool> g_countWork{false};
+void flushWork() {
+    for (int k = 0; k < kWorkWords; ++k) {
+        g_work[k] += t_work[k];
+        t_work[k] = 0;
+    }
+}
+
 struct V3 {
     float x, y, z;
 };
@@ -471,6 +488,9 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp, uint32_t
         orc_ordered::Counts cnt;
         const uint32_t f = orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr);
         hits = cnt.visits + cnt.tests;
+        ++t_work[0];
+        t_work[1] += cnt.visits;
+        t_work[2] += cnt.tests;
         const bool hit = f != kNoFace && !std::isinf(t);
         if (hit) {
             // The accepted triangle's barycentrics: the same test again.
@@ -1149,6 +1169,27 @@ struct NoHook {
     void bounce(uint32_t, const BSDF&, V3, V3, const SurfacePoint&, V3) {}
 };
 
+// Work of an environment shadow ray: its closest-hit counts since (v0, c0)
+// and the counts of an any-hit walk of the same ray.
+void countAnyHit(const SceneC& s, V3 org, V3 dir, uint64_t v0, uint64_t c0) {
+    ++t_work[3];
+    t_work[4] += t_work[1] - v0;
+    t_work[5] += t_work[2] - c0;
+    const float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z};
+    auto tri = [&](uint32_t face, float bound, float* tt) {
+        float x = bound;
+        Isect tmp{};
+        if (!triTest(s, org, dir, face, &x, &tmp)) return false;
+        *tt = x;
+        return true;
+    };
+    float t = INFINITY;
+    orc_ordered::Counts cnt;
+    orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, true, nullptr);
+    t_work[6] += cnt.visits;
+    t_work[7] += cnt.tests;
+}
+
 // One camera path of pathtracing (pathtracer.cl:25-197) for pixel (x, y).
 template <class Hook = NoHook>
 V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, uint32_t* rds, Hook& hook) {
@@ -1195,9 +1236,11 @@ V3 tracePath(const SceneC& s, const RenderArgs& ra, uint32_t gx, uint32_t gy, ui
                     dist2 = dot(vinL, vinL);
                     vinL = vinL * (1.0f / std::sqrt(dist2));
                     if (lp.atInfinity) { vinL = lp.p; dist2 = 1.0f; }
-                    V3 sorg = sp.p + vinL * kEps;
+                        V3 sorg = sp.p + vinL * kEps;
                     SurfacePoint lsp;
+                    const uint64_t v0 = t_work[1], c0 = t_work[2];
                     bool hit = rayIntersection(s, sorg, vinL, &lsp);
+                    if (lp.atInfinity && s.ordered && g_countWork) countAnyHit(s, sorg, vinL, v0, c0);
                     bool visible = lp.atInfinity ? !hit : (hit && lsp.faceID == lp.faceID);  // D4
                     if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {  // D11
                         V3 vinLrev = -vinL;
@@ -1564,6 +1607,7 @@ int orc_render(const void* const* arrays, uint32_t numFaces, uint32_t width, uin
                 pixels[i].z += acc.z;
             }
         }
+        flushWork();
     };
     if (nt == 1) work();
     else {
@@ -1672,6 +1716,18 @@ int orc_texture_probe(const uint8_t* record, float u, float v, float* out) {
 
 int orc_disk_probe(float u1, float u2, float* out) {
     concentricSampleDisk(u1, u2, out, out + 1);
+    return 0;
+}
+
+// Traversal work of the renders since the last reset (the kWorkWords
+// counters above), and whether later renders count the any-hit walks of
+// environment shadow rays (count != 0).
+int orc_work_counters(uint64_t* out, int reset, int count) {
+    for (int k = 0; k < kWorkWords; ++k) {
+        if (out) out[k] = g_work[k];
+        if (reset) g_work[k] = 0;
+    }
+    g_countWork = count != 0;
     return 0;
 }
 
