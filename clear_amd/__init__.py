@@ -61,6 +61,8 @@ def lib():
         "clr_lbvh_build": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
         "clr_lbvh_build_soup": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "clr_lbvh_reserve": (i32, [vp, u32, u32]),
+        "clr_lbvh_build_soup_noalloc": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "clr_lbvh_noalloc_max_tris": (u32, []),
         "clr_lbvh_build_segmented": (i32, [vp, vp, vp, u32, u32, u32, vp, vp, vp]),
         "clr_lbvh_reserve_segmented": (i32, [vp, u32, u32, u32]),
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),

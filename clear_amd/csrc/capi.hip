@@ -173,6 +173,9 @@ int clr_create(clr_ctx** out, int device) {
         clr_destroy(ctx);
         return CLR_ERR_OUT_OF_MEMORY;
     }
+    // Device address of the no-allocation build's static array; a failure
+    // here only disables that entry point (it then returns the error).
+    if (lbvhNoAllocInit(ctx) != CLR_OK) ctx->noAllocScratch = nullptr;
     *out = ctx;
     return CLR_OK;
 }
@@ -300,6 +303,21 @@ int clr_lbvh_build_soup(clr_ctx* ctx, const float* tris, uint32_t numTris, uint3
     return lbvhBuild(ctx, in, numTris, numBitsPerDim, internalNodes, leafNodes,
                      static_cast<hipStream_t>(stream));
 }
+
+int clr_lbvh_build_soup_noalloc(clr_ctx* ctx, const float* tris, uint32_t numTris, uint32_t numBitsPerDim,
+                                clr_lbvh_inode* internalNodes, clr_lbvh_leaf* leafNodes, void* stream) {
+    int rc = checkBuildArgs(ctx, numTris, numBitsPerDim, internalNodes, leafNodes);
+    if (rc) return rc;
+    if (!tris) return fail(ctx, CLR_ERR_INVALID_ARG, "null triangle soup");
+    if ((reinterpret_cast<uintptr_t>(tris) & 3u) != 0)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "soup must be 4-B aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return lbvhBuildSoupNoAlloc(ctx, tris, numTris, numBitsPerDim, internalNodes, leafNodes,
+                                static_cast<hipStream_t>(stream));
+}
+
+uint32_t clr_lbvh_noalloc_max_tris(void) { return lbvhNoAllocMaxTris(); }
 
 int clr_lbvh_reserve(clr_ctx* ctx, uint32_t maxFaces, uint32_t numBitsPerDim) {
     if (!ctx) return CLR_ERR_INVALID_ARG;

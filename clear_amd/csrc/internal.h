@@ -58,6 +58,9 @@ struct clr_ctx {
     // Traversal-stack overflow region of the ordered trace kernel.
     uint32_t* traceSpill = nullptr;
     size_t traceSpillBytes = 0;
+    // Device address of this device's static working array of
+    // clr_lbvh_build_soup_noalloc (looked up by clr_create; not owned).
+    void* noAllocScratch = nullptr;
 };
 
 namespace clr {
@@ -133,6 +136,13 @@ int lbvhBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bitsPerDim
 // Fills `count` 32-bit words with zero on `stream` (kernel, not hipMemset).
 void fillZero(uint32_t* words, size_t count, hipStream_t stream);
 size_t lbvhSegScratchBytes(uint32_t totalTris, uint32_t numSegments, uint32_t bitsPerDim);
+// The build that allocates nothing (clr_lbvh_build_soup_noalloc): looks up
+// this device's static working array once per device (with the device
+// current) and records it in ctx; the build itself enqueues kernels only.
+int lbvhNoAllocInit(clr_ctx* ctx);
+uint32_t lbvhNoAllocMaxTris();
+int lbvhBuildSoupNoAlloc(clr_ctx* ctx, const float* tris, uint32_t numTris, uint32_t bitsPerDim,
+                         clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream);
 // segOffsets: device array of numSegments + 1 offsets, ending at numTris.
 int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffsets,
                        uint32_t numSegments, uint32_t numTris, uint32_t bitsPerDim, clr_lbvh_inode* inodes,

@@ -32,9 +32,10 @@
 // end, which fixes every node's index on the way up.
 #include <hip/hip_runtime.h>
 
-#include <vector>
-
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "internal.h"
 
@@ -1083,15 +1084,22 @@ inline unsigned gridFor(uint32_t n, unsigned cap) {
     return g < 1 ? 1 : (g > cap ? cap : g);
 }
 
+// Working memory of one build: the layout region, the hierarchy's exchange
+// words (six per internal node) and the epoch word that tags them. The
+// context's pool and exchange buffer, or the no-allocation entry point's
+// static array.
+struct BuildScratch {
+    void* base;
+    uint64_t* xchg;
+    size_t xchgNodes;
+    uint32_t* epochWord;
+};
+
 template <typename KeyT, bool SOUP>
 int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
-             clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
-    int rc = ensureStatus(ctx);
-    if (rc) return rc;
-    rc = ensureExchange(ctx, n);
-    if (rc) return rc;
-    uint32_t* epochWord = ctx->devStatus + kWordEpoch;
-    Layout L = layoutFor(ctx->pool, n, bits);
+             clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream, const BuildScratch& W) {
+    uint32_t* epochWord = W.epochWord;
+    Layout L = layoutFor(W.base, n, bits);
     const int P = numPassesFor(bits);
     const uint32_t tile = tileFor(bits);
     const uint32_t parts = (n + tile - 1) / tile;
@@ -1102,7 +1110,7 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     };
     ev(0);
     k_init<<<gridFor((uint32_t)std::min<size_t>(L.clearWords, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
-        L.hist, L.clearWords, L.sceneOrd, epochWord, reinterpret_cast<uint32_t*>(ctx->xchg), ctx->xchgNodes * 12);
+        L.hist, L.clearWords, L.sceneOrd, epochWord, reinterpret_cast<uint32_t*>(W.xchg), W.xchgNodes * 12);
     // One set of same-address atomics per block: a few blocks per CU.
     const unsigned reduceGrid = (unsigned)std::max(1, ctx->numCUs) * 4u;
     // Box centres live in buffers the first radix pass has not written yet:
@@ -1138,10 +1146,10 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     ev(3);
     const uint32_t localBlocks = (n + kLocalLeaves - 1) / kLocalLeaves;
     k_hier_local<SOUP, KeyT><<<localBlocks, kHierThreads, 0, stream>>>(
-        src, kin, vin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, inodes, leaves, L.pend, ctx->devStatus);
+        src, kin, vin, n, keyBits, L.otherEnd, W.xchg, epochWord, inodes, leaves, L.pend, ctx->devStatus);
     ev(4);
     k_hier_global<KeyT><<<(2 * localBlocks + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        kin, n, keyBits, L.otherEnd, ctx->xchg, epochWord, ctx->devStatus, inodes, leaves, L.pend,
+        kin, n, keyBits, L.otherEnd, W.xchg, epochWord, ctx->devStatus, inodes, leaves, L.pend,
         2 * localBlocks);
     ev(5);
     if (prof) ++ctx->profiledBuilds;
@@ -1408,9 +1416,6 @@ int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffse
     return hipCheck(ctx, hipGetLastError(), "segmented LBVH build launch");
 }
 
-namespace {
-}  // namespace
-
 void fillZero(uint32_t* words, size_t count, hipStream_t stream) {
     k_init<<<gridFor((uint32_t)std::min<size_t>(count, 0xFFFFFFFFu), 2048), kBlock, 0, stream>>>(
         words, count, nullptr);
@@ -1423,13 +1428,75 @@ size_t lbvhScratchBytes(uint32_t n, uint32_t bitsPerDim) {
 int lbvhBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
               clr_lbvh_inode* inodes, clr_lbvh_leaf* leaves, hipStream_t stream) {
     int rc = ensurePool(ctx, lbvhScratchBytes(n, bits));
+    if (!rc) rc = ensureStatus(ctx);
+    if (!rc) rc = ensureExchange(ctx, n);
     if (rc) return rc;
+    const BuildScratch W{ctx->pool, ctx->xchg, ctx->xchgNodes, ctx->devStatus + kWordEpoch};
     const bool soup = in.tris != nullptr;
     if (bits <= 10)
-        return soup ? runBuild<uint32_t, true>(ctx, in, n, bits, inodes, leaves, stream)
-                    : runBuild<uint32_t, false>(ctx, in, n, bits, inodes, leaves, stream);
-    return soup ? runBuild<uint64_t, true>(ctx, in, n, bits, inodes, leaves, stream)
-                : runBuild<uint64_t, false>(ctx, in, n, bits, inodes, leaves, stream);
+        return soup ? runBuild<uint32_t, true>(ctx, in, n, bits, inodes, leaves, stream, W)
+                    : runBuild<uint32_t, false>(ctx, in, n, bits, inodes, leaves, stream, W);
+    return soup ? runBuild<uint64_t, true>(ctx, in, n, bits, inodes, leaves, stream, W)
+                : runBuild<uint64_t, false>(ctx, in, n, bits, inodes, leaves, stream, W);
+}
+
+// ---- the build that allocates nothing ----------------------------------------
+// clr_lbvh_build_soup_noalloc's working memory: one static array per device
+// (each device that loads this code object holds its own copy until the
+// process ends), laid out as 256 B of control words (word 0: the build epoch
+// of this array's exchange words), the exchange words for kNoAllocMaxTris
+// triangles, then the build layout for kNoAllocMaxTris triangles with 30-bit
+// keys. Host code never passes `scratch` itself to a kernel (its host-side
+// name is not a device address): lbvhNoAllocInit looks up its device address
+// once per device and every later use goes through that pointer.
+constexpr uint32_t kNoAllocMaxTris = 1u << 20;
+constexpr size_t kNoAllocCtrlBytes = 256;
+constexpr size_t kNoAllocXchgBytes = (size_t)kNoAllocMaxTris * 6 * sizeof(uint64_t);
+constexpr size_t kNoAllocLayoutBytes = (size_t)56 << 20;  // >= layoutFor(kNoAllocMaxTris, 10 bits), checked
+constexpr size_t kNoAllocScratchBytes = kNoAllocCtrlBytes + kNoAllocXchgBytes + kNoAllocLayoutBytes;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noAllocMutex;
+void* g_noAllocPtr[kMaxDevices] = {};
+
+int lbvhNoAllocInit(clr_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "device ordinal beyond the no-allocation table");
+    if (layoutFor(nullptr, kNoAllocMaxTris, 10).bytes > kNoAllocLayoutBytes)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "no-allocation scratch smaller than its layout");
+    std::lock_guard<std::mutex> lock(g_noAllocMutex);
+    void*& slot = g_noAllocPtr[ctx->device];
+    if (!slot) {
+        void* ptr = nullptr;
+        int rc = hipCheck(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "no-allocation scratch address");
+        if (rc) return rc;
+        // Once per device: the epoch starts at 0 and no exchange word carries
+        // a later epoch.
+        fillZero(static_cast<uint32_t*>(ptr), (kNoAllocCtrlBytes + kNoAllocXchgBytes) / 4, nullptr);
+        rc = hipCheck(ctx, hipDeviceSynchronize(), "no-allocation scratch clear");
+        if (rc) return rc;
+        slot = ptr;
+    }
+    ctx->noAllocScratch = slot;
+    return CLR_OK;
+}
+
+uint32_t lbvhNoAllocMaxTris() { return kNoAllocMaxTris; }
+
+int lbvhBuildSoupNoAlloc(clr_ctx* ctx, const float* tris, uint32_t n, uint32_t bits, clr_lbvh_inode* inodes,
+                         clr_lbvh_leaf* leaves, hipStream_t stream) {
+    if (!ctx->noAllocScratch) return fail(ctx, CLR_ERR_HIP, "no-allocation scratch unavailable on this device");
+    if (n > kNoAllocMaxTris)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "numTris " + std::to_string(n) + " exceeds the no-allocation limit of " +
+                                                  std::to_string(kNoAllocMaxTris));
+    if (bits > 10) return fail(ctx, CLR_ERR_UNSUPPORTED, "the no-allocation build supports numBitsPerDim 1..10");
+    char* base = static_cast<char*>(ctx->noAllocScratch);
+    const BuildScratch W{base + kNoAllocCtrlBytes + kNoAllocXchgBytes, reinterpret_cast<uint64_t*>(base + kNoAllocCtrlBytes),
+                         kNoAllocMaxTris, reinterpret_cast<uint32_t*>(base)};
+    LbvhInput in;
+    in.tris = tris;
+    return runBuild<uint32_t, true>(ctx, in, n, bits, inodes, leaves, stream, W);
 }
 
 }  // namespace clr

@@ -142,6 +142,24 @@ int clr_lbvh_build_soup(clr_ctx* ctx, const float* tris, uint32_t numTris,
                         uint32_t numBitsPerDim, clr_lbvh_inode* internalNodes,
                         clr_lbvh_leaf* leafNodes, void* stream);
 
+/* clr_lbvh_build_soup that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Instead of the context's scratch pool it works in a static
+ * device array of 104 MiB + 256 B that every device loading the library
+ * holds until the process ends, sized for numTris <=
+ * clr_lbvh_noalloc_max_tris() (1,048,576) and numBitsPerDim 1..10; larger
+ * requests return CLR_ERR_UNSUPPORTED. The call enqueues kernels only: no
+ * hipMalloc, no hipFree, no host synchronisation. Every call on one device
+ * shares that device's array whatever its context, so calls that overlap on
+ * one device (other streams, contexts or threads) race and must be ordered by
+ * the caller. Output identical to clr_lbvh_build_soup. */
+int clr_lbvh_build_soup_noalloc(clr_ctx* ctx, const float* tris, uint32_t numTris,
+                                uint32_t numBitsPerDim, clr_lbvh_inode* internalNodes,
+                                clr_lbvh_leaf* leafNodes, void* stream);
+/* Largest numTris clr_lbvh_build_soup_noalloc accepts. */
+uint32_t clr_lbvh_noalloc_max_tris(void);
+
 /* Pre-grows the scratch pool and the hierarchy's exchange buffer for builds
  * of up to maxFaces triangles (mirrors the capacity argument of the
  * LBVHBuilder ctor). Synchronous. After it, builds of up to maxFaces
