@@ -26,13 +26,14 @@ void hipOk(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device buffer of `bytes` (zeroed), filled from `host` when it is not null.
 template <class T>
 T* upload(const void* host, size_t bytes) {
     void* d = nullptr;
     const size_t alloc = bytes < 16 ? 16 : bytes;  // empty arrays become small dummies, as the reference's host does
     hipOk(hipMalloc(&d, alloc), "hipMalloc");
     hipOk(hipMemset(d, 0, alloc), "hipMemset");
-    if (bytes) hipOk(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice), "upload");
+    if (host && bytes) hipOk(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice), "upload");
     return static_cast<T*>(d);
 }
 
