@@ -302,14 +302,13 @@ __device__ __forceinline__ uint32_t digitExclusiveScan(uint32_t v, uint32_t* s_t
     return pre + x - v;
 }
 
-// Keys from the box centres, the hierarchy's arrival words, and the digit
-// histograms of every radix pass, in input order.
+// Keys from the box centres and the digit histograms of every radix pass, in
+// input order.
 template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ cx, const float* __restrict__ cy,
                                                    const float* __restrict__ cz, uint32_t n, uint32_t bits,
                                                    const uint32_t* __restrict__ sceneOrd,
                                                    KeyT* __restrict__ keys,
-                                                   uint32_t* __restrict__ otherEnd,
                                                    uint32_t* __restrict__ hist, int numPasses) {
     __shared__ uint32_t s_hist[kMaxPasses * 256];
     for (int j = threadIdx.x; j < numPasses * 256; j += kBlock) s_hist[j] = 0u;
@@ -332,7 +331,6 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ cx,
         }
         KeyT key = spread3<KeyT>(q[0]) | (spread3<KeyT>(q[1]) << 1) | (spread3<KeyT>(q[2]) << 2);
         keys[i] = key;
-        if (i + 1 < n) otherEnd[i] = kEmpty;
         for (int p = 0; p < numPasses; ++p)
             atomicAdd(&s_hist[p * 256 + (uint32_t)((key >> (8 * p)) & 0xFFu)], 1u);
     };
@@ -698,8 +696,11 @@ __device__ __forceinline__ Box awaitBox(const uint64_t* slot, uint32_t epoch,
 // leaf k's box until internal node k is completed (leaf k lies inside node k
 // and was joined into its parent before). A climb whose parent splits at a
 // block boundary (bs-1 or be-1) stops and leaves a pending record (at most two
-// per block); after the last step the block exports its half-filled slots
-// (one child arrived) to the global arrival words, publishing that child's box.
+// per block); after the last step the block writes the global arrival words
+// of every slot it did not complete: the far end of the one child that
+// arrived (publishing that child's box) or empty, and empty for the split
+// between it and the next block. Those are all the words phase 2 reads, so no
+// earlier pass clears them.
 // Phase 2 (k_hier_global): one thread per pending record continues with the
 // global epoch-tagged protocol. Only nodes whose range crosses a block
 // boundary (a small fraction) are finished there.
@@ -708,6 +709,9 @@ constexpr int kHierThreads = 256;
 constexpr int kHierPer = kLocalLeaves / kHierThreads;
 static_assert(kLocalLeaves % kHierThreads == 0 && kLocalLeaves <= 65536, "hierarchy block shape");
 constexpr uint32_t kDone = 0xFFFFFFFEu;
+constexpr uint32_t kAuxFlagShift = 10;  // log2(kLocalLeaves)
+constexpr uint32_t kAuxEmpty = 0xFFFFu;
+static_assert((1u << kAuxFlagShift) == (uint32_t)kLocalLeaves, "completed-node record layout");
 
 struct Pending {
     uint32_t valid, l, r, split, internal;  // split: unused (a pending node is never a root)
@@ -744,9 +748,11 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
     constexpr int B = kLocalLeaves;
     __shared__ uint32_t s_other[B];
     __shared__ float s_box[6][B];
-    __shared__ int s_delta[B + 1];  // delta(x) for x = bs-1 .. be-1 at [x - bs + 1]
-    __shared__ uint32_t s_q[2][B];  // climbers: l | r << 16 (block-local)
-    __shared__ uint32_t s_aux[B];   // node completed here at slot j: split - bs | flags << 16
+    // Deltas fit in 8 bits (at most keyBits + 31 <= 95) and completed-node
+    // records in 16 (split - bs and two flags): 39 KB, four blocks per CU.
+    __shared__ int8_t s_delta[B + 1];  // delta(x) for x = bs-1 .. be-1 at [x - bs + 1]
+    __shared__ uint32_t s_q[2][B];     // climbers: l | r << 16 (block-local)
+    __shared__ uint16_t s_aux[B];      // node completed here at slot j: split - bs | flags << kAuxFlagShift
     __shared__ uint32_t s_qc[3];
     __shared__ uint32_t s_seg[SEG ? B : 1];
 
@@ -763,7 +769,7 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
     for (int k = 0; k < kHierPer; ++k) {
         const uint32_t j = (uint32_t)k * kHierThreads + t;
         s_other[j] = kEmpty;
-        s_aux[j] = kEmpty;
+        s_aux[j] = kAuxEmpty;
         rp[k] = sidx[min(bs + j, be - 1u)];
     }
     float4 ra[kHierPer], rb[kHierPer];
@@ -887,7 +893,8 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
                             s_box[k][pSelf - bs] = pb.lo[k];
                             s_box[3 + k][pSelf - bs] = pb.hi[k];
                         }
-                        s_aux[pSelf - bs] = (ps - bs) | (ps == pL ? 1u << 16 : 0u) | (ps + 1u == pR ? 1u << 17 : 0u);
+                        s_aux[pSelf - bs] = (ps - bs) | (ps == pL ? 1u << kAuxFlagShift : 0u) |
+                                          (ps + 1u == pR ? 1u << (kAuxFlagShift + 1) : 0u);
                         if (!root) {
                             push = true;
                             packed = (pL - bs) | ((pR - bs) << 16);
@@ -908,16 +915,20 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
         __syncthreads();
         cnt = *qcount;
     }
-    // The loop ends with a barrier. Export slots where exactly one child
-    // arrived: phase 2 brings the other.
+    // The loop ends with a barrier. Export the arrival words of phase 2: the
+    // slots where exactly one child arrived (phase 2 brings the other) hold
+    // that child's far end, the slots where neither did are empty, and so is
+    // the split between this block and the next. Completed slots are never
+    // reached again.
+    if (t == 0 && be < n) otherEnd[be - 1] = kEmpty;
 #pragma unroll
     for (int k = 0; k < kHierPer; ++k) {
         const uint32_t j = (uint32_t)k * kHierThreads + t;
         if (bs + j + 1 < be) {
             const uint32_t v = s_other[j];
+            if (v != kDone) otherEnd[bs + j] = v;
             if (v != kEmpty && v != kDone) {
                 const uint32_t ps = bs + j;
-                otherEnd[ps] = v;
                 const bool leftArrived = v <= ps;
                 const uint32_t idx = leftArrived ? ps : ps + 1u;
                 const bool isLeaf = leftArrived ? (v == ps) : (v == ps + 1u);
@@ -936,7 +947,7 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
     for (uint32_t q = t; q < 3u * nl; q += kHierThreads) {
         const uint32_t j = q / 3u, part = q - 3u * j;
         const uint32_t aux = s_aux[j];
-        if (aux == kEmpty) continue;
+        if (aux == kAuxEmpty) continue;
         uint32_t seg = 0, segS = 0;
         if constexpr (SEG) {
             seg = s_seg[j];
@@ -944,8 +955,8 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
         }
         uint4 v;
         if (part == 2u) {
-            const uint32_t split = bs + (aux & 0xFFFFu);
-            const uint32_t flags = ((aux >> 16) & 1u) | (((aux >> 17) & 1u) << 8);
+            const uint32_t split = bs + (aux & ((1u << kAuxFlagShift) - 1u));
+            const uint32_t flags = ((aux >> kAuxFlagShift) & 1u) | (((aux >> (kAuxFlagShift + 1)) & 1u) << 8);
             v = make_uint4(flags, split - segS, split + 1u - segS, 0u);
         } else {
             v = make_uint4(__float_as_uint(s_box[3 * part][j]), __float_as_uint(s_box[3 * part + 1][j]),
@@ -1124,7 +1135,7 @@ int runBuild(clr_ctx* ctx, const LbvhInput& in, uint32_t n, uint32_t bits,
     ev(1);
     PrimSrc src{in.tris, L.boxes};
     k_morton<KeyT><<<gridFor(n, reduceGrid), kBlock, 0, stream>>>(
-        cx, cy, cz, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.otherEnd, L.hist, P);
+        cx, cy, cz, n, bits, L.sceneOrd, static_cast<KeyT*>(L.keysA), L.hist, P);
     ev(2);
     KeyT* kin = static_cast<KeyT*>(L.keysA);
     KeyT* kout = static_cast<KeyT*>(L.keysB);
@@ -1282,15 +1293,14 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds(const float* __restrict__
     }
 }
 
-// Box records, scene-tagged 64-bit keys, per-position scene ids, arrival words
-// and the digit histograms of all passes.
+// Box records, scene-tagged 64-bit keys, per-position scene ids and the digit
+// histograms of all passes.
 __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__ tris, float4* __restrict__ boxes,
                                                        uint32_t n, uint32_t bits, const uint4* __restrict__ chunks,
                                                        const uint32_t* __restrict__ words,
                                                        const uint32_t* __restrict__ segBox,
                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ segOf,
-                                                       uint32_t* __restrict__ otherEnd, uint32_t* __restrict__ hist,
-                                                       int numPasses) {
+                                                       uint32_t* __restrict__ hist, int numPasses) {
     if (blockIdx.x >= words[kSegWordChunks]) return;  // the grid covers the bound on chunks
     __shared__ uint32_t s_hist[kMaxPasses * 256];
     for (int j = threadIdx.x; j < numPasses * 256; j += kBlock) s_hist[j] = 0u;
@@ -1320,7 +1330,6 @@ __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__
             tag | spread3<uint64_t>(q[0]) | (spread3<uint64_t>(q[1]) << 1) | (spread3<uint64_t>(q[2]) << 2);
         keys[i] = key;
         segOf[i] = ch.x;
-        if (i + 1 < n) otherEnd[i] = kEmpty;
         for (int p = 0; p < numPasses; ++p)
             atomicAdd(&s_hist[p * 256 + (uint32_t)((key >> (8 * p)) & 0xFFu)], 1u);
     }
@@ -1387,7 +1396,7 @@ int lbvhBuildSegmented(clr_ctx* ctx, const float* tris, const uint32_t* segOffse
     k_seg_chunks<<<1, kSegScanThreads, 0, stream>>>(segOffsets, numSegments, n, S.chunks, S.words, ctx->devStatus);
     k_seg_bounds<<<maxChunks, kBlock, 0, stream>>>(tris, S.chunks, S.words, S.segBox);
     k_seg_morton<<<maxChunks, kBlock, 0, stream>>>(tris, L.boxes, n, bits, S.chunks, S.words, S.segBox,
-                                                   static_cast<uint64_t*>(L.keysA), S.segOf, L.otherEnd, L.hist, P);
+                                                   static_cast<uint64_t*>(L.keysA), S.segOf, L.hist, P);
     uint64_t* kin = static_cast<uint64_t*>(L.keysA);
     uint64_t* kout = static_cast<uint64_t*>(L.keysB);
     uint32_t* vin = L.valsA;
