@@ -395,19 +395,67 @@ __device__ __forceinline__ float emissionHitMis(uint32_t flags, float dirPDF, fl
 // ---- BSDF (reflection.cl) --------------------------------------------------
 enum : uint32_t { kRefl = 1, kTrans = 2, kDiff = 4, kGlossy = 8, kSpec = 16, kNonSingular = 12, kAll = 31 };
 
+// One decoded BSDF element. Its type flags follow from the element id
+// (bxType) and its Fresnel record is an offset into the scene's other
+// resources, so an element takes 10 registers.
 struct BxDF {
-    uint32_t id, type;
+    uint32_t id;
     V3 R;   // R / T / Rs (Ashikhmin specular, Ashikhmin diffuse)
     V3 R2;  // Rd (Ashikhmin diffuse)
     float a, b;
-    const uint8_t* fresnel;
+    uint32_t fres;  // Fresnel record: oth + fres (ids 1, 2, 6)
 };
 struct BSDF {
     int n;
     BxDF c[4];
     V3 N, S, T, NG;
+    const uint8_t* oth;  // the scene's other resources (Fresnel records)
 };
 
+// c ? a : b field by field (a struct-wide select becomes a copy through a
+// selected address, which keeps the element array out of registers).
+__device__ __forceinline__ BxDF bxSelect(bool c, const BxDF& a, const BxDF& b) {
+    BxDF r;
+    r.id = c ? a.id : b.id;
+    r.R = v3(c ? a.R.x : b.R.x, c ? a.R.y : b.R.y, c ? a.R.z : b.R.z);
+    r.R2 = v3(c ? a.R2.x : b.R2.x, c ? a.R2.y : b.R2.y, c ? a.R2.z : b.R2.z);
+    r.a = c ? a.a : b.a;
+    r.b = c ? a.b : b.b;
+    r.fres = c ? a.fres : b.fres;
+    return r;
+}
+
+// Type flags of element ids 0..6 (the fxType set with each id,
+// reflection.cl:241-329): diffuse (0) and Ashikhmin diffuse (5) are diffuse
+// reflection, new Ward (3), Ashikhmin specular (4) and the Beckmann coat (6)
+// glossy reflection, 1 specular reflection, 2 specular transmission.
+__device__ __forceinline__ uint32_t bxType(uint32_t id) {
+    constexpr uint64_t kTable = (uint64_t)(kRefl | kDiff) | (uint64_t)(kRefl | kSpec) << 5 |
+                                (uint64_t)(kTrans | kSpec) << 10 | (uint64_t)(kRefl | kGlossy) << 15 |
+                                (uint64_t)(kRefl | kGlossy) << 20 | (uint64_t)(kRefl | kDiff) << 25 |
+                                (uint64_t)(kRefl | kGlossy) << 30;
+    return id < 7u ? (uint32_t)(kTable >> (5u * id)) & 31u : 0u;
+}
+
+// BSDF element storage. REG keeps the four elements in registers: element
+// i is read and written by field-wise selects over constant indices (an access
+// at a variable index would put the array in scratch memory), which suits a
+// kernel that shades and ends (the wavefront shade pass). The default indexes
+// the array directly, so the compiler keeps it in scratch memory: a
+// megakernel that also traverses keeps its registers for the walk.
+template <bool REG>
+__device__ __forceinline__ BxDF bxGet(const BSDF& B, int i);
+template <bool REG>
+__device__ __forceinline__ void bxPut(BSDF& B, int i, const BxDF& f) {
+    if constexpr (REG) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) B.c[k] = bxSelect(k == i, f, B.c[k]);
+    } else {
+        B.c[i] = f;
+    }
+}
+
+template <bool REG = false>
 __device__ void bsdfAlloc(const RScene& S, uint32_t off, const SurfPt& sp, BSDF& B) {
     const uint8_t* m = S.mat + off;
     B.n = m[0];
@@ -433,20 +481,20 @@ __device__ void bsdfAlloc(const RScene& S, uint32_t off, const SurfPt& sp, BSDF&
     }
     B.T = cross(B.N, B.S);
     B.NG = sp.gN;
+    B.oth = S.oth;
     uint32_t p = off + 8;
     for (int i = 0; i < 4; ++i) {
         if (i >= B.n) break;
         p = (p + 3u) & ~3u;
         const uint8_t* e = S.mat + p;
-        BxDF& f = B.c[i];
+        BxDF f;
         f.id = e[0];
         f.R = v3z();
         f.R2 = v3z();
         f.a = f.b = 0.0f;
-        f.fresnel = nullptr;
+        f.fres = 0u;
         switch (e[0]) {
             case 0: {
-                f.type = kRefl | kDiff;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);
                 float s2 = evalFloat(S.tex + ld<uint32_t>(e + 8), sp.u, sp.v);
                 s2 *= s2;
@@ -456,44 +504,39 @@ __device__ void bsdfAlloc(const RScene& S, uint32_t off, const SurfPt& sp, BSDF&
                 break;
             }
             case 1:
-                f.type = kRefl | kSpec;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);
-                f.fresnel = S.oth + ld<uint32_t>(e + 8);
+                f.fres = ld<uint32_t>(e + 8);
                 p += 12;
                 break;
             case 2:
-                f.type = kTrans | kSpec;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);
                 f.a = ld<float>(e + 8);
                 f.b = ld<float>(e + 12);
-                f.fresnel = S.oth + ld<uint32_t>(e + 16);
+                f.fres = ld<uint32_t>(e + 16);
                 p += 20;
                 break;
             case 3:
             case 4:
-                f.type = kRefl | kGlossy;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);
                 f.a = evalFloat(S.tex + ld<uint32_t>(e + 8), sp.u, sp.v);
                 f.b = evalFloat(S.tex + ld<uint32_t>(e + 12), sp.u, sp.v);
                 p += 16;
                 break;
             case 5:
-                f.type = kRefl | kDiff;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);   // Rs
                 f.R2 = evalColor(S.tex + ld<uint32_t>(e + 8), sp.u, sp.v);  // Rd
                 p += 12;
                 break;
             case 6:
-                f.type = kRefl | kGlossy;
                 f.R = evalColor(S.tex + ld<uint32_t>(e + 4), sp.u, sp.v);
                 f.a = evalFloat(S.tex + ld<uint32_t>(e + 8), sp.u, sp.v);
-                f.fresnel = S.oth + ld<uint32_t>(e + 12);
+                f.fres = ld<uint32_t>(e + 12);
                 p += 16;
                 break;
             default:
-                f.type = 0;
                 break;
         }
+        bxPut<REG>(B, i, f);
     }
 }
 
@@ -565,7 +608,7 @@ __device__ __forceinline__ V3 halfPlusZ(V3 wo, V3 wi) {
     return h.z < 0.0f ? -h : h;
 }
 
-__device__ V3 fx(const BxDF& f, V3 wo, V3 wi) {
+__device__ V3 fx(const BxDF f, const uint8_t* oth, V3 wo, V3 wi) {
     if (wi.z * wo.z <= 0.0f) return v3z();
     switch (f.id) {
         case 0: {
@@ -601,7 +644,7 @@ __device__ V3 fx(const BxDF& f, V3 wo, V3 wi) {
             const V3 h = halfPlusZ(wo, wi);
             const float D = beckD(h, f.a);
             const float G = beckG1(wo, f.a) * beckG1(wi, f.a);
-            const V3 F = fresnel(f.fresnel, dot(wo, h));
+            const V3 F = fresnel(oth + f.fres, dot(wo, h));
             const float k = D * G / (4.0f * fabsf(wo.z) * fabsf(wi.z));
             return (f.R * F) * k;
         }
@@ -639,18 +682,18 @@ __device__ float fxPdf(const BxDF& f, V3 wo, V3 wi) {
     }
 }
 
-__device__ V3 sampleFx(const BxDF& f, V3 wo, float u0, float u1, V3& wi, float& pdf) {
+__device__ V3 sampleFx(const BxDF& f, const uint8_t* oth, V3 wo, float u0, float u1, V3& wi, float& pdf) {
     switch (f.id) {
         case 0:
         case 5:
             wi = cosineHemisphere(u0, u1);
             if (wo.z < 0.0f) wi.z *= -1;
             pdf = fabsf(wi.z) / kPiF;
-            return fx(f, wo, wi);
+            return fx(f, oth, wo, wi);
         case 1: {
             wi = v3(-wo.x, -wo.y, wo.z);
             pdf = 1.0f;
-            const V3 r = f.R * fresnel(f.fresnel, wo.z);
+            const V3 r = f.R * fresnel(oth + f.fres, wo.z);
             return r / fabsf(wi.z);
         }
         case 2: {
@@ -668,7 +711,7 @@ __device__ V3 sampleFx(const BxDF& f, V3 wo, float u0, float u1, V3& wi, float& 
             if (entering) cost = -cost;
             wi = v3(eta * -wo.x, eta * -wo.y, cost);
             pdf = 1.0f;
-            const V3 r = (v3s(1.0f) - fresnel(f.fresnel, wo.z)) * f.R;
+            const V3 r = (v3s(1.0f) - fresnel(oth + f.fres, wo.z)) * f.R;
             return r / fabsf(wi.z);
         }
         case 3: {
@@ -703,7 +746,7 @@ __device__ V3 sampleFx(const BxDF& f, V3 wo, float u0, float u1, V3& wi, float& 
                 return v3z();
             }
             pdf = fxPdf(f, wo, wi);
-            return fx(f, wo, wi);
+            return fx(f, oth, wo, wi);
         }
         case 6: {
             const float a = f.a;
@@ -719,7 +762,7 @@ __device__ V3 sampleFx(const BxDF& f, V3 wo, float u0, float u1, V3& wi, float& 
                 return v3z();
             }
             pdf = fxPdf(f, wo, wi);
-            return fx(f, wo, wi);
+            return fx(f, oth, wo, wi);
         }
         default:
             pdf = 0.0f;
@@ -728,53 +771,78 @@ __device__ V3 sampleFx(const BxDF& f, V3 wo, float u0, float u1, V3& wi, float& 
 }
 
 __device__ __forceinline__ bool matchType(uint32_t type, uint32_t mask) { return (type & mask) == type; }
+
+// Element i of a BSDF (see bxPut). Loops over the elements stay rolled, so
+// each inlines the element code once.
+template <bool REG>
+__device__ __forceinline__ BxDF bxGet(const BSDF& B, int i) {
+    if constexpr (REG) {
+        BxDF f = B.c[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) f = bxSelect(k == i, B.c[k], f);
+        return f;
+    } else {
+        return B.c[i];
+    }
+}
 __device__ __forceinline__ bool hasNonSpecular(const BSDF& B) {
     bool r = false;
+#pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (i < B.n) r |= (B.c[i].type & kNonSingular) != 0;
+        if (i < B.n) r |= (bxType(B.c[i].id) & kNonSingular) != 0;
     return r;
 }
 
+template <bool REG = false>
 __device__ V3 sampleFs(const BSDF& B, V3 vout, float uc, float u0, float u1, V3& vin, float& pdf,
                        uint32_t& sampledType) {
     const V3 wo = toLocal(B.S, B.T, B.N, vout);
     const uint32_t which = randUInt(uc, (uint32_t)B.n);
-    BxDF f = B.c[0];
-    for (int i = 1; i < 4; ++i)
-        if ((uint32_t)i == which) f = B.c[i];
+    const BxDF f = bxGet<REG>(B, (int)which);
     V3 wi;
-    V3 ret = sampleFx(f, wo, u0, u1, wi, pdf);
+    V3 ret = sampleFx(f, B.oth, wo, u0, u1, wi, pdf);
     if (pdf == 0.0f) {
         sampledType = 0;
         return v3z();
     }
-    sampledType = f.type;
+    const uint32_t fType = bxType(f.id);
+    sampledType = fType;
     vin = toWorld(B.S, B.T, B.N, wi);
-    if (!(f.type & kSpec) && B.n > 1)
-        for (int i = 0; i < 4; ++i)
-            if (i < B.n && (uint32_t)i != which && matchType(B.c[i].type, kAll)) pdf += fxPdf(B.c[i], wo, wi);
+    if (!(fType & kSpec) && B.n > 1)
+        for (int i = 0; i < B.n; ++i) {
+            const BxDF e = bxGet<REG>(B, i);
+            if ((uint32_t)i != which && matchType(bxType(e.id), kAll)) pdf += fxPdf(e, wo, wi);
+        }
     pdf /= (float)B.n;
-    if (!(f.type & kSpec)) {
+    if (!(fType & kSpec)) {
         ret = v3z();
         const uint32_t flags = dot(vin, B.NG) * dot(vout, B.NG) > 0 ? (kAll & ~kTrans) : (kAll & ~kRefl);
-        for (int i = 0; i < 4; ++i)
-            if (i < B.n && matchType(B.c[i].type, flags)) ret += fx(B.c[i], wo, wi);
+        for (int i = 0; i < B.n; ++i) {
+            const BxDF e = bxGet<REG>(B, i);
+            if (matchType(bxType(e.id), flags)) ret += fx(e, B.oth, wo, wi);
+        }
     }
     return ret;
 }
+template <bool REG = false>
 __device__ V3 evalFs(const BSDF& B, V3 vout, V3 vin) {
     const V3 wo = toLocal(B.S, B.T, B.N, vout), wi = toLocal(B.S, B.T, B.N, vin);
     const uint32_t flags = dot(vin, B.NG) * dot(vout, B.NG) > 0 ? (kAll & ~kTrans) : (kAll & ~kRefl);
     V3 r = v3z();
-    for (int i = 0; i < 4; ++i)
-        if (i < B.n && matchType(B.c[i].type, flags)) r += fx(B.c[i], wo, wi);
+    for (int i = 0; i < B.n; ++i) {
+        const BxDF e = bxGet<REG>(B, i);
+        if (matchType(bxType(e.id), flags)) r += fx(e, B.oth, wo, wi);
+    }
     return r;
 }
+template <bool REG = false>
 __device__ float evalFsPdf(const BSDF& B, V3 vout, V3 vin) {
     const V3 wo = toLocal(B.S, B.T, B.N, vout), wi = toLocal(B.S, B.T, B.N, vin);
     float p = 0.0f;
-    for (int i = 0; i < 4; ++i)
-        if (i < B.n && matchType(B.c[i].type, kAll)) p += fxPdf(B.c[i], wo, wi);
+    for (int i = 0; i < B.n; ++i) {
+        const BxDF e = bxGet<REG>(B, i);
+        if (matchType(bxType(e.id), kAll)) p += fxPdf(e, wo, wi);
+    }
     return p / (float)B.n;
 }
 
@@ -993,6 +1061,7 @@ __device__ __forceinline__ void cameraRayAt(const RScene& Sc, float c0, float c1
 // (Ashikhmin diffuse); transmission and other elements add nothing.
 __device__ __forceinline__ V3 baseColor(const BSDF& B) {
     V3 c = v3z();
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (i >= B.n) break;
         const BxDF& f = B.c[i];

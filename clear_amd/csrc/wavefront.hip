@@ -39,7 +39,6 @@
 namespace clr {
 
 int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hipStream_t stream);
-void launchResolveScenes(dev::RScene* table, uint32_t count, hipStream_t stream);
 
 namespace {
 
@@ -411,7 +410,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
 
 // Surface vertices (hit queue of extension queue `in`).
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
+__global__ __launch_bounds__(256, 3) void k_wf_shade(WfArgs A, uint32_t in) {
     RScene S = A.S;
     if (!BATCH) finishScene(S);
     const WfBufs& B = A.B;
@@ -476,7 +475,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
             }
             if (live) {
                 BSDF Bs;
-                bsdfAlloc(S, F.matPtr, sp, Bs);
+                bsdfAlloc<true>(S, F.matPtr, sp, Bs);
                 if (hasNonSpecular(Bs)) {
                     const float lu = rng.uniform(), l0 = rng.uniform(), l1 = rng.uniform();
                     SurfPt lp;
@@ -492,10 +491,10 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
                     }
                     // With BSDF samples alone no shadow ray is cast (the numbers are drawn).
                     if (lightPDF > 0.0f && lightPDF < INFINITY && !(A.flags & CLR_RENDER_BSDF_SAMPLING_ONLY)) {
-                        const float fsPDF = evalFsPdf(Bs, vout, vinL) * absCosEDF(E, vinL) / dist2;
+                        const float fsPDF = evalFsPdf<true>(Bs, vout, vinL) * absCosEDF(E, vinL) / dist2;
                         const float mis = lightSampleMis(A.flags, lightPDF, fsPDF);
                         const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(Bs.N, vinL)) / dist2 / lightPDF;
-                        contrib = wgt * ((alpha * evalLe(E, -vinL)) * evalFs(Bs, vout, vinL));
+                        contrib = wgt * ((alpha * evalLe(E, -vinL)) * evalFs<true>(Bs, vout, vinL));
                         sorg = sp.p + vinL * kRayEps;
                         sdir = vinL;
                         target = atInf ? kNone : lp.face;
@@ -504,7 +503,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, uint32_t in) {
                 }
                 const float bc = rng.uniform(), b0 = rng.uniform(), b1 = rng.uniform();
                 V3 nd = v3z();
-                const V3 f = sampleFs(Bs, vout, bc, b0, b1, nd, dirPDF, sampledType);
+                const V3 f = sampleFs<true>(Bs, vout, bc, b0, b1, nd, dirPDF, sampledType);
                 if ((f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) || dirPDF == 0.0f) {
                     live = false;
                 } else {
@@ -764,6 +763,20 @@ struct Launch {
     uint32_t grid, block;
 };
 
+// Scene table entries written from kernel arguments, blob headers resolved.
+constexpr uint32_t kSceneChunk = 16;
+struct SceneChunk {
+    RScene s[kSceneChunk];
+};
+static_assert(sizeof(SceneChunk) + 16 <= 4096, "scene table chunk exceeds the kernel argument space");
+__global__ void k_put_scenes(RScene* __restrict__ table, SceneChunk chunk, uint32_t count) {
+    const uint32_t i = threadIdx.x;
+    if (i >= count) return;
+    RScene S = chunk.s[i];
+    finishScene(S);
+    table[i] = S;
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------
@@ -870,6 +883,14 @@ void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStr
     }
 }
 
+// Whether `stream` is being captured into a graph (by the caller).
+int streamCapturing(clr_ctx* ctx, hipStream_t stream, bool& capturing) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const int rc = hipCheck(ctx, hipStreamIsCapturing(stream, &cs), "capture status");
+    capturing = cs != hipStreamCaptureStatusNone;
+    return rc;
+}
+
 void dropGraph(WfState& w) {
     if (w.exec) (void)hipGraphExecDestroy(w.exec);
     if (w.graph) (void)hipGraphDestroy(w.graph);
@@ -933,12 +954,17 @@ int wavefrontSetConfig(clr_ctx* ctx, const clr_wavefront_config* cfg) {
 // chunks of at most maxSlots path slots. A carries the scene(s) and outputs.
 int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_render_params& rp,
               hipStream_t stream) {
-    int rc = CLR_OK;
+    bool capturing = false;
+    int rc = streamCapturing(ctx, stream, capturing);
+    if (rc) return rc;
     const uint32_t P = (uint32_t)std::min<uint64_t>(numPos, w.cfg.maxSlots);
     const Launch tl = traceLaunch(w);
     const uint32_t lanes = tl.grid * tl.block;
     const size_t need = wfBytes(P, lanes);
     if (ctx->wfPoolBytes < need) {
+        if (capturing)
+            return fail(ctx, CLR_ERR_INVALID_ARG,
+                        "the wavefront pool must be sized by an eager render of this size before capture");
         rc = hipCheck(ctx, hipDeviceSynchronize(), "wavefront pool regrow sync");
         if (rc) return rc;
         if (ctx->wfPool) (void)hipFree(ctx->wfPool);
@@ -962,7 +988,12 @@ int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_re
     // Each sample takes at most maxDepth + 1 iterations (vertices 0..maxDepth).
     const uint64_t bound = (uint64_t)rp.samplesPerPixel * (rp.maxDepth + 1ull) + 1ull;
     const uint32_t G = w.cfg.itersPerGraph;
-    if (w.cfg.useGraph) {
+    // Under the caller's stream capture the iterations are captured as plain
+    // launches (no nested graph) and all `bound` of them run: the early stop
+    // reads a count back to the host. Iterations with empty queues return at
+    // once.
+    const bool useGraph = w.cfg.useGraph && !capturing;
+    if (useGraph) {
         rc = ensureGraph(ctx, w, A);
         if (rc) return rc;
     }
@@ -977,7 +1008,7 @@ int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_re
         if (rc) return rc;
         uint64_t launched = 0, batches = 0;
         while (launched < bound) {
-            if (w.cfg.useGraph) {
+            if (useGraph) {
                 rc = hipCheck(ctx, hipGraphLaunch(w.exec, stream), "graph launch");
                 if (rc) return rc;
             } else {
@@ -987,7 +1018,7 @@ int runChunks(clr_ctx* ctx, WfState& w, WfArgs& A, uint64_t numPos, const clr_re
             }
             launched += G;
             ++batches;
-            if (w.cfg.checkEvery && batches % w.cfg.checkEvery == 0 && launched < bound) {
+            if (!capturing && w.cfg.checkEvery && batches % w.cfg.checkEvery == 0 && launched < bound) {
                 // Early stop: G is even, so the next iteration reads queue 0.
                 rc = hipCheck(ctx, hipMemcpyAsync(w.hostCount, A.B.counts + kCntExt0, 4, hipMemcpyDeviceToHost, stream),
                               "count readback");
@@ -1037,6 +1068,12 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
         bytes += accelNodeBytes(n, scenes[s].bvhNodes != nullptr) + align(48ull * n) + align(4ull * n);
     }
     if (ctx->batchBytes < bytes) {
+        bool capturing = false;
+        rc = streamCapturing(ctx, stream, capturing);
+        if (rc) return rc;
+        if (capturing)
+            return fail(ctx, CLR_ERR_INVALID_ARG,
+                        "the batch buffers must be sized by an eager render of this batch before capture");
         rc = hipCheck(ctx, hipDeviceSynchronize(), "batch buffer regrow sync");
         if (rc) return rc;
         if (ctx->batch) (void)hipFree(ctx->batch);
@@ -1078,12 +1115,15 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
         S.envLight = -1;
         S.status = ctx->devStatus;
     }
-    rc = hipCheck(ctx, hipMemcpyAsync(dTable, table.data(), sizeof(RScene) * numScenes, hipMemcpyHostToDevice, stream),
-                  "scene table copy");
-    // The copy reads pageable host memory that goes away with this call.
-    if (!rc) rc = hipCheck(ctx, hipStreamSynchronize(stream), "scene table sync");
-    if (rc) return rc;
-    launchResolveScenes(dTable, numScenes, stream);
+    // The table travels in kernel arguments (no host copy to wait for, and a
+    // captured graph keeps the values).
+    for (uint32_t s0 = 0; s0 < numScenes; s0 += kSceneChunk) {
+        SceneChunk chunk;
+        std::memset(&chunk, 0, sizeof(chunk));
+        const uint32_t count = std::min<uint32_t>(kSceneChunk, numScenes - s0);
+        for (uint32_t k = 0; k < count; ++k) chunk.s[k] = table[s0 + k];
+        k_put_scenes<<<1, kSceneChunk, 0, stream>>>(dTable + s0, chunk, count);
+    }
     rc = hipCheck(ctx, hipGetLastError(), "batch set-up launch");
     if (rc) return rc;
     WfArgs A;

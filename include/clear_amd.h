@@ -332,7 +332,16 @@ int clr_render(clr_ctx* ctx, const clr_scene_dev* scene, const clr_render_params
  * for that scene alone. The reference renders one scene per process
  * (main.cpp:475-494); this is the multi-scene form of that loop, paired with
  * clr_lbvh_build_segmented. numScenes * W * H < 2^31;
- * CLR_RENDER_REFERENCE_TRAVERSAL is not supported here. */
+ * CLR_RENDER_REFERENCE_TRAVERSAL is not supported here.
+ * The call may be captured into a hipGraph on `stream` (together with the
+ * segmented build, the state seeding and the image clear of a frame) once
+ * an eager call with the same scene count, sizes and params has sized the
+ * context's buffers; a capture that would have to grow them fails with
+ * CLR_ERR_INVALID_ARG. Under capture the scene table is written by kernels
+ * whose arguments carry it, and all samplesPerPixel * (maxDepth + 1) + 1
+ * iterations are recorded as plain launches (the eager call stops early by
+ * reading a path count back). The graph stays valid while the context's
+ * batch and wavefront buffers are not regrown. */
 int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
                      const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
                      void* stream);

@@ -117,3 +117,63 @@ def test_captured_segmented_build_replays(gpu_ctx):
             if m > 1:
                 np.testing.assert_array_equal(gi[base - s:base - s + m - 1], o["inodes"],
                                               err_msg=f"replay {r} scene {s}")
+
+
+def test_captured_batch_pipeline_replays(gpu_ctx):
+    """One graph holds a batch's whole frame: the segmented build of every
+    scene's LBVH (clr_lbvh_build_segmented), the seeding of the pixel states,
+    the clearing of the image and the batched wavefront render
+    (clr_render_batch), which under capture runs its iteration bound as plain
+    launches (no early-stop readback, no nested graph) and uploads its scene
+    table through kernel arguments. The scenes' vertices (and the triangle
+    soup the build reads) move between replays; each replay's nodes must equal
+    the oracle's per-scene builds of the moved geometry, and its states and
+    image the eager pipeline's on the same geometry, bit for bit."""
+    import torch
+
+    W, H, spp, depth = 40, 28, 2, 4
+    sizes = [1500, 4000, 900]
+    hosts = [ca.HostScene(ca.SCENE_CLUSTERS, W, H, seed=200 + i, num_triangles=m) for i, m in enumerate(sizes)]
+    batch = ca.SceneBatch(gpu_ctx, hosts)
+    k = len(hosts)
+    st = torch.empty(k * W * H * 4, dtype=torch.int32, device="cuda")
+    px = torch.zeros(k * W * H * 4, dtype=torch.float32, device="cuda")
+
+    def frame():
+        batch.build(10)
+        for s in range(k):
+            ca.seed_pixel_states(gpu_ctx, W, H, st[s * W * H * 4:(s + 1) * W * H * 4])
+        px.zero_()
+        ca.render_batch(gpu_ctx, batch, spp, depth, st, px)
+
+    frame()  # eager: sizes the batch buffers and the wavefront pool
+    torch.cuda.synchronize()
+    assert gpu_ctx.device_status() == 0
+    g = capture(frame)
+    verts = [h.vertices.copy() for h in hosts]
+    for r in range(REPLAYS):
+        if r:
+            d = np.float32(0.01 * r)
+            for s in range(k):
+                verts[s][:, 0] += d
+                batch.t[s]["vertices"].view(torch.float32).view(-1, 4)[:, 0] += float(d)
+            batch.tris.view(-1, 3)[:, 0] += float(d)
+        px.fill_(-1.0)
+        torch.cuda.synchronize()
+        print(f"[graph capture] batch pipeline replay {r}", flush=True)
+        g.replay()
+        torch.cuda.synchronize()
+        assert gpu_ctx.device_status() == 0
+        got = [t.clone() for t in (px, st, batch.inodes, batch.leaves)]
+        assert float(px.abs().sum()) > 0.0
+        gl = batch.leaves.cpu().numpy().reshape(-1, 48)
+        gi = batch.inodes.cpu().numpy().reshape(-1, 48)
+        for s, h in enumerate(hosts):
+            o, m = int(batch.offsets[s]), h.num_faces
+            ob = orc.lbvh_build(verts[s], h.faces, 10)
+            np.testing.assert_array_equal(gl[o:o + m], ob["leaves"], err_msg=f"replay {r} scene {s}")
+            np.testing.assert_array_equal(gi[o - s:o - s + m - 1], ob["inodes"], err_msg=f"replay {r} scene {s}")
+        frame()
+        torch.cuda.synchronize()
+        for a, b, what in zip(got, (px, st, batch.inodes, batch.leaves), ("image", "states", "inodes", "leaves")):
+            assert torch.equal(a, b), f"replay {r}: captured {what} differ from the eager frame's"
