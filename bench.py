@@ -19,8 +19,8 @@ Secondary legs of the same metric family, each on its stated workload:
            vertices + 56-B Face records of the same 2^24 triangles; its nodes
            must equal the soup build's;
   trace    2^22 random rays (origins in the unit cube, directions uniform,
-           tmax = inf) against the 2^24 BVH: Mrays/s, reference-order walk and
-           ordered walk;
+           tmax = inf) against the 2^24 BVH: Mrays/s, reference-order walk,
+           ordered walk and its 4-wide form (the renderers' walk);
   render   the 3,117,058-triangle outdoor scene (fBm terrain 1024^2, 200
            spheres, coat BSDF, normal map, 1024x512 HDR sky, thin lens 0.02) at
            1920x1080, 64 spp, max depth 8: Msamples/s of the plain megakernel
@@ -56,20 +56,23 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # Per kernel: bytes per triangle it reads and writes by design. k_bounds reads
 # the soup (36) and writes the 32-B box record and the 12-B box centre;
-# k_morton reads the centre and writes the key and the hierarchy's arrival
-# word; the radix sort moves 4-B keys and payloads in 4 passes of 8-bit digits
-# (64 B); the hierarchy is the Karras topology (20 B) + refit (164 B), all
-# attributed to the block-local kernel, which builds every node except the few
-# above block boundaries.
-STAGES = [("bounds", "k_bounds", 80.0, 1), ("morton", "k_morton", 20.0, 1),
+# k_morton reads the centre and writes the key (the hierarchy pass writes the
+# arrival words of the splits it leaves open); the radix sort moves 4-B keys
+# and payloads in 4 passes of 8-bit digits (64 B); the hierarchy is the Karras
+# topology (20 B) + refit (164 B), all attributed to the block-local kernel,
+# which builds every node except the few above block boundaries.
+STAGES = [("bounds", "k_bounds", 80.0, 1), ("morton", "k_morton", 16.0, 1),
           ("sort", "k_onesweep", 64.0, 4), ("hier_local", "k_hier_local", 184.0, 1),
           ("hier_global", "k_hier_global", 0.0, 1)]
 # Algorithmic bytes per triangle of the whole build (bounds 36, Morton 40,
 # sort 64, Karras 20, refit 164: the build's contract model, 8-bit digits).
 BUILD_BYTES_PER_TRI = 324.0
-# Traversal work model per internal-node visit (V) and triangle test (T).
+# Traversal work model per internal-node visit (V) and triangle test (T), and
+# per visit of a 4-wide record (W: four slab tests, 4 x 24 B of boxes and
+# 4 x 4 B of references).
 TRACE_FLOPS_V, TRACE_FLOPS_T = 40.0, 51.0
 TRACE_BYTES_V, TRACE_BYTES_T = 56.0, 40.0
+TRACE_FLOPS_W, TRACE_BYTES_W = 80.0, 112.0
 
 
 def log(msg: str) -> None:
@@ -192,23 +195,25 @@ def trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev):
     tv = torch.from_numpy(verts).to(dev)
     tf = torch.from_numpy(faces.view(np.uint8)).to(dev)
     out = {"rays": R, "workload": f"{R} random rays vs the {n}-triangle BVH"}
-    for name, ordered in [("reference_walk", False), ("ordered_walk", True)]:
-        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, st, ordered=ordered)
+    for name, ordered, wide in [("reference_walk", False, False), ("ordered_walk", True, False),
+                                ("wide_walk", True, True)]:
+        fv, bv = (TRACE_FLOPS_W, TRACE_BYTES_W) if wide else (TRACE_FLOPS_V, TRACE_BYTES_V)
+        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, st, ordered=ordered, wide=wide)
         torch.cuda.synchronize()
         stv = st.view(R, 2).to(torch.int64)
         V, T = int(stv[:, 0].sum()), int(stv[:, 1].sum())
         hits = int((hit != -1).sum())
-        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered)
+        ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered, wide=wide)
         reps = 3
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered)
+            ca.trace_closest(ctx, inodes, leaves, n, tv, tf, rays, R, hit, ht, None, ordered=ordered, wide=wide)
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
-        flops = TRACE_FLOPS_V * V + TRACE_FLOPS_T * T
-        byts = TRACE_BYTES_V * V + TRACE_BYTES_T * T
+        flops = fv * V + TRACE_FLOPS_T * T
+        byts = bv * V + TRACE_BYTES_T * T
         roof = min(FP32_PEAK_TFLOPS, flops / byts * HBM_PEAK_GBS / 1e3)
         tf_s = flops / (ms * 1e-3) / 1e12
         out[name] = {"ms": round(ms, 3), "Mrays_s": round(R / (ms * 1e-3) / 1e6, 1),
@@ -271,10 +276,11 @@ def render_leg(ca, np, torch, ctx, args):
 
 def render_roofline(render, work, W, H, spp):
     """Roofline of the render legs from the oracle's traversal counts per
-    camera sample (ordered walk, all rays cast: camera, bounce and shadow
-    rays; the wavefront renderer walks environment shadow rays any-hit), with
-    the traversal work model per node visit and triangle test. Shading flops
-    are not counted, so the fractions are lower bounds."""
+    camera sample (the ordered walk's 4-wide form that both renderers use,
+    all rays cast: camera, bounce and shadow rays; the wavefront renderer
+    walks environment shadow rays any-hit), with the traversal work model per
+    record visit and triangle test. Shading flops are not counted, so the
+    fractions are lower bounds."""
     samples = W * H * spp
     per = work["samples"]
     legs = {"megakernel": (work["visits"], work["tests"]),
@@ -283,8 +289,8 @@ def render_roofline(render, work, W, H, spp):
     out = {"work_sample": work["sample"], "rays_per_sample": round(work["rays"] / per, 3)}
     for name, (v, t) in legs.items():
         V, T = v / per * samples, t / per * samples
-        flops = TRACE_FLOPS_V * V + TRACE_FLOPS_T * T
-        byts = TRACE_BYTES_V * V + TRACE_BYTES_T * T
+        flops = TRACE_FLOPS_W * V + TRACE_FLOPS_T * T
+        byts = TRACE_BYTES_W * V + TRACE_BYTES_T * T
         roof = min(FP32_PEAK_TFLOPS, flops / byts * HBM_PEAK_GBS / 1e3)
         tf_s = flops / (render[name]["ms"] * 1e-3) / 1e12
         out[name] = {"bound": "latency (traversal)", "visits_per_sample": round(v / per, 2),
@@ -515,7 +521,8 @@ def cpu_render(ca, np, host, depth):
                    pix_begin=y * W, pix_end=(y + 1) * W, threads=host_threads())
     work = orc.work_counters(reset=True, count_any_hit=False)
     work["samples"] = len(rows) * W * 2
-    work["sample"] = f"oracle renders (ordered walk) of every 16th row ({len(rows)} rows x {W}), 2 spp, depth {depth}"
+    work["sample"] = (f"oracle renders (ordered walk, 4-wide form) of every 16th row ({len(rows)} rows x {W}), "
+                      f"2 spp, depth {depth}")
     return out, work
 
 

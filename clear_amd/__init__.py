@@ -67,6 +67,7 @@ def lib():
         "clr_lbvh_reserve_segmented": (i32, [vp, u32, u32, u32]),
         "clr_trace_closest": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_trace_closest_ordered": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
+        "clr_trace_closest_wide": (i32, [vp, vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp]),
         "clr_trace_closest_bvh": (i32, [vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, i32, vp]),
         "clr_gen_cluster_soup": (i32, [u32, i32, u32, f32, f32, f32, vp]),
         "clr_gen_random_rays": (i32, [u32, i32, vp]),
@@ -235,10 +236,12 @@ def trace_closest_bvh(ctx: Context, nodes, num_faces: int, vertices, faces, rays
 
 def trace_closest(ctx: Context, internal_nodes, leaf_nodes, num_faces: int, vertices, faces,
                   rays, num_rays: int, hit_face, hit_t, stats=None, stream=None,
-                  ordered: bool = False) -> None:
-    """Closest hit per ray: the reference's walk (clr_trace_closest) or, with
-    ordered=True, the ordered traversal (clr_trace_closest_ordered)."""
-    fn = lib().clr_trace_closest_ordered if ordered else lib().clr_trace_closest
+                  ordered: bool = False, wide: bool = False) -> None:
+    """Closest hit per ray: the reference's walk (clr_trace_closest), with
+    ordered=True the ordered traversal (clr_trace_closest_ordered), with
+    wide=True its 4-wide form (clr_trace_closest_wide)."""
+    fn = (lib().clr_trace_closest_wide if wide else
+          lib().clr_trace_closest_ordered if ordered else lib().clr_trace_closest)
     ctx.check(fn(ctx.handle, _ptr(internal_nodes), _ptr(leaf_nodes), num_faces, _ptr(vertices),
                  _ptr(faces), _ptr(rays), num_rays, _ptr(hit_face), _ptr(hit_t), _ptr(stats),
                  _stream(stream)))

@@ -399,6 +399,24 @@ int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
                         hitFace, hitT, stats, static_cast<hipStream_t>(stream));
 }
 
+int clr_trace_closest_wide(clr_ctx* ctx, const clr_lbvh_inode* internalNodes, const clr_lbvh_leaf* leafNodes,
+                           uint32_t numFaces, const clr_float3* vertices, const clr_face* faces, const float* rays,
+                           uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats, void* stream) {
+    if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (numFaces == 0) return fail(ctx, CLR_ERR_INVALID_ARG, "numFaces must be >= 1");
+    if (!leafNodes || (numFaces > 1 && !internalNodes) || !vertices || !faces)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null scene buffer");
+    if (numRays > 0 && (!rays || !hitFace || !hitT))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "null ray buffer");
+    if (!aligned16(leafNodes) || (internalNodes && !aligned16(internalNodes)) ||
+        !aligned16(vertices))
+        return fail(ctx, CLR_ERR_INVALID_ARG, "nodes and vertices must be 16-byte aligned");
+    DeviceScope ds(ctx->device);
+    if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    return traceOrdered(ctx, internalNodes, leafNodes, nullptr, numFaces, vertices, faces, rays, numRays,
+                        hitFace, hitT, stats, static_cast<hipStream_t>(stream), true);
+}
+
 int clr_trace_closest_bvh(clr_ctx* ctx, const clr_bvh_node* nodes, uint32_t numFaces, const clr_float3* vertices,
                           const clr_face* faces, const float* rays, uint32_t numRays, uint32_t* hitFace,
                           float* hitT, uint32_t* stats, int ordered, void* stream) {

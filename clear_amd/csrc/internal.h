@@ -178,19 +178,24 @@ void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
                       hipStream_t stream);
 // Bytes of the traversal layout's node array (256-B aligned).
 size_t accelNodeBytes(uint32_t n, bool midpoint);
+// The renderers' 4-wide records (128 B per node slot) and their build.
+size_t accelWideBytes(uint32_t n, bool midpoint);
+void launchAccelWide(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh, uint32_t n,
+                     float4* wnodes, hipStream_t stream);
 // Face -> first light map and first environment light of one scene.
 void launchFaceLights(const clr_light_info* lights, uint32_t numLights, uint32_t numFaces, uint32_t* faceLight,
                       int* envLight, hipStream_t stream);
 int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
                          const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
 // Builds the ordered walk's traversal layout from the drop-in nodes on `stream`
-// into context memory and returns its two arrays.
+// into context memory and returns its two arrays, and with `wnodes` the
+// renderers' 4-wide records too.
 int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                uint32_t n, const clr_float3* vertices, const clr_face* faces, const float4** anodes,
-               const float4** atris, hipStream_t stream);
+               const float4** atris, hipStream_t stream, const float4** wnodes = nullptr);
 int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
-                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream);
+                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream, bool wide = false);
 int traceClosest(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays,
                  uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats,

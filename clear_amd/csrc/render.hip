@@ -471,12 +471,12 @@ int prepareRenderScene(clr_ctx* ctx, const clr_scene_dev& sd, dev::RScene& S, hi
     if (sd.numLights)
         k_face_lights_fill<<<(sd.numLights + 255) / 256, 256, 0, stream>>>(sd.lights, sd.numLights,
                                                                            sd.numFaces, faceLight, envLight);
-    const float4 *anodes = nullptr, *atris = nullptr;
+    const float4 *anodes = nullptr, *atris = nullptr, *wnodes = nullptr;
     rc = buildAccel(ctx, sd.internalNodes, sd.leafNodes, sd.bvhNodes, sd.numFaces, sd.vertices, sd.faces, &anodes,
-                    &atris, stream);
+                    &atris, stream, &wnodes);
     if (rc) return rc;
     S.g = dev::GeomScene{sd.internalNodes, sd.leafNodes, sd.numFaces, sd.vertices, sd.faces, anodes, atris,
-                         reinterpret_cast<const float4*>(sd.bvhNodes)};
+                         reinterpret_cast<const float4*>(sd.bvhNodes), wnodes};
     S.normals = sd.normals;
     S.tangents = sd.tangents;
     S.uvs = sd.uvs;

@@ -360,9 +360,9 @@ __device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint3
     TraceStats st{0u, 0u};
     bool over = false;
     uint32_t face;
-    if constexpr (WALK == kWalkOrdered)
-        face = orderedHit<false, false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
-                                                  TexAlpha{&S}, &over);
+    if constexpr (WALK == kWalkOrdered)  // over the 4-wide records (every render builds them)
+        face = wideHit<false, false, TexAlpha, LdsStack, kStackDepth>(S.g, org, dir, t, LdsStack{stack, stride}, st,
+                                                                      &b1, &b2, TexAlpha{&S}, &over);
     else if constexpr (WALK == kWalkMidpoint)
         face = bvhWalk<false, false>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2, TexAlpha{&S}, &over);
     else

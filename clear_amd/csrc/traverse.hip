@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 
 #include "internal.h"
@@ -126,6 +127,105 @@ __global__ void k_accel_build_bvh(const float4* __restrict__ bvh, uint32_t n, co
     }
 }
 
+// 4-wide records (GeomScene::wnodes) of the LBVH: record i holds the
+// children of each internal child of node i, or a leaf child itself, in slot
+// order left.left, left.right, right.left, right.right (a leaf child takes
+// the first of its two slots; the other stays empty).
+struct WideSlots {
+    float lo[3][4], hi[3][4];
+    uint32_t ref[4];
+    __device__ __forceinline__ void clear() {
+        for (int c = 0; c < 4; ++c) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a][c] = INFINITY;
+                hi[a][c] = -INFINITY;
+            }
+            ref[c] = dev::kNone;
+        }
+    }
+    __device__ __forceinline__ void put(int c, float4 l, float4 h, uint32_t r) {
+        lo[0][c] = l.x;
+        lo[1][c] = l.y;
+        lo[2][c] = l.z;
+        hi[0][c] = h.x;
+        hi[1][c] = h.y;
+        hi[2][c] = h.z;
+        ref[c] = r;
+    }
+    __device__ __forceinline__ void store(float4* o) const {
+        for (int a = 0; a < 3; ++a) {
+            o[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+            o[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+        }
+        o[6] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
+                           __uint_as_float(ref[3]));
+        o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+};
+
+__global__ void k_accel_wide(const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves,
+                             uint32_t n, float4* __restrict__ wnodes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= n) return;
+    WideSlots w;
+    w.clear();
+    const uint4 c = reinterpret_cast<const uint4*>(inodes + i)[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool leaf = ((c.x >> (8 * k)) & 0xFFu) != 0;
+        const uint32_t ci = k ? c.z : c.y;
+        if (leaf) {
+            if (ci >= n) continue;  // malformed nodes leave the slot empty
+            const float4* p = reinterpret_cast<const float4*>(leaves + ci);
+            w.put(2 * k, p[0], p[1], leaves[ci].objIdx | dev::kLeafRef);
+            continue;
+        }
+        if (ci + 1 >= n) continue;
+        const uint4 cc = reinterpret_cast<const uint4*>(inodes + ci)[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const bool gl = ((cc.x >> (8 * m)) & 0xFFu) != 0;
+            const uint32_t gi = m ? cc.z : cc.y;
+            if (gl ? gi >= n : gi + 1 >= n) continue;
+            const float4* p = gl ? reinterpret_cast<const float4*>(leaves + gi)
+                                 : reinterpret_cast<const float4*>(inodes + gi);
+            w.put(2 * k + m, p[0], p[1], gl ? (leaves[gi].objIdx | dev::kLeafRef) : gi);
+        }
+    }
+    w.store(wnodes + 8 * (size_t)i);
+}
+
+// The same records for a CPU midpoint BVH, by node number (leaf node slots
+// stay unused).
+__global__ void k_accel_wide_bvh(const float4* __restrict__ bvh, uint32_t n, float4* __restrict__ wnodes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numNodes = 2 * n - 1;
+    if (i >= numNodes) return;
+    const uint2 c = *reinterpret_cast<const uint2*>(bvh + 3 * (size_t)i + 2);
+    if (c.y == kEmpty || c.x >= numNodes || c.y >= numNodes) return;
+    WideSlots w;
+    w.clear();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t ci = k ? c.y : c.x;
+        const float4* p = bvh + 3 * (size_t)ci;
+        const uint2 cc = *reinterpret_cast<const uint2*>(p + 2);
+        if (cc.y == kEmpty) {  // a leaf: its face
+            w.put(2 * k, p[0], p[1], cc.x | dev::kLeafRef);
+            continue;
+        }
+        if (cc.x >= numNodes || cc.y >= numNodes) continue;  // malformed: the binary walk flags it
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const uint32_t gi = m ? cc.y : cc.x;
+            const float4* q = bvh + 3 * (size_t)gi;
+            const uint2 gc = *reinterpret_cast<const uint2*>(q + 2);
+            w.put(2 * k + m, q[0], q[1], gc.y == kEmpty ? (gc.x | dev::kLeafRef) : gi);
+        }
+    }
+    w.store(wnodes + 8 * (size_t)i);
+}
+
 // Ordered traversal, grid-stride over the rays with a fixed grid (waves per
 // CU x CUs). Each lane walks two rays at once, stepping them in turn, so one
 // ray's node fetch is in flight while the other's boxes and triangles are
@@ -136,13 +236,13 @@ constexpr int kOrdLds = 32;
 constexpr int kOrdBlock = 128;
 constexpr int kOrdHalf = kOrdLds / 2;
 
-template <bool STATS>
+template <bool STATS, bool WIDE>
 __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
     const clr_lbvh_inode* __restrict__ inodes, const clr_lbvh_leaf* __restrict__ leaves, uint32_t n,
     const clr_float3* __restrict__ verts, const clr_face* __restrict__ faces, const float* __restrict__ rays,
     uint32_t numRays, uint32_t* __restrict__ hitFace, float* __restrict__ hitT, uint32_t* __restrict__ stats,
     uint32_t* __restrict__ status, uint32_t* __restrict__ spill, const float4* __restrict__ anodes,
-    const float4* __restrict__ atris, const float4* __restrict__ bvh) {
+    const float4* __restrict__ atris, const float4* __restrict__ bvh, const float4* __restrict__ wnodes) {
     __shared__ uint32_t s_stack[kOrdLds * kOrdBlock];
     const uint32_t gid = blockIdx.x * kOrdBlock + threadIdx.x;
     const uint32_t lanes = gridDim.x * kOrdBlock;
@@ -150,8 +250,9 @@ __global__ __launch_bounds__(kOrdBlock) void k_trace_ordered(
     const S stk0{s_stack + threadIdx.x, kOrdBlock, spill + gid, lanes};
     const S stk1{s_stack + kOrdHalf * kOrdBlock + threadIdx.x, kOrdBlock,
                  spill + (size_t)(dev::kStackPairs - kOrdHalf) * lanes + gid, lanes};
-    const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris, bvh};
-    using W = dev::Ordered<STATS, dev::NoAlpha, S, true>;
+    const dev::GeomScene g{inodes, leaves, n, verts, faces, anodes, atris, bvh, wnodes};
+    using W = std::conditional_t<WIDE, dev::Wide<STATS, dev::NoAlpha, S, dev::kStackPairs>,
+                                 dev::Ordered<STATS, dev::NoAlpha, S, true>>;
     W w0(g, stk0, dev::NoAlpha()), w1(g, stk1, dev::NoAlpha());
     const size_t R = numRays;
     bool over = false;
@@ -206,11 +307,24 @@ size_t accelNodeBytes(uint32_t n, bool midpoint) {
     return (slots * 64 + 255) & ~size_t(255);
 }
 
+size_t accelWideBytes(uint32_t n, bool midpoint) { return 2 * accelNodeBytes(n, midpoint); }
+
+void launchAccelWide(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh, uint32_t n,
+                     float4* wnodes, hipStream_t stream) {
+    if (n < 2) return;
+    if (bvh)
+        k_accel_wide_bvh<<<(2 * n - 1 + 255) / 256, 256, 0, stream>>>(reinterpret_cast<const float4*>(bvh), n,
+                                                                       wnodes);
+    else
+        k_accel_wide<<<(n + 255) / 256, 256, 0, stream>>>(inodes, leaves, n, wnodes);
+}
+
 int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                uint32_t n, const clr_float3* vertices, const clr_face* faces, const float4** anodes,
-               const float4** atris, hipStream_t stream) {
+               const float4** atris, hipStream_t stream, const float4** wnodes) {
     const size_t nodeBytes = accelNodeBytes(n, bvh != nullptr);
-    const size_t need = nodeBytes + (size_t)n * 48;
+    const size_t triBytes = ((size_t)n * 48 + 255) & ~size_t(255);
+    const size_t need = nodeBytes + triBytes + (wnodes ? accelWideBytes(n, bvh != nullptr) : 0);
     if (ctx->accelBytes < need) {
         int rc = hipCheck(ctx, hipDeviceSynchronize(), "accel regrow sync");
         if (rc) return rc;
@@ -229,6 +343,11 @@ int buildAccel(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* 
     launchAccelBuild(inodes, leaves, bvh, n, vertices, faces, an, at, stream);
     *anodes = an;
     *atris = at;
+    if (wnodes) {
+        float4* wn = reinterpret_cast<float4*>(static_cast<char*>(ctx->accel) + nodeBytes + triBytes);
+        launchAccelWide(inodes, leaves, bvh, n, wn, stream);
+        *wnodes = wn;
+    }
     return hipCheck(ctx, hipGetLastError(), "traversal layout launch");
 }
 
@@ -244,7 +363,7 @@ void launchAccelBuild(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
 
 int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, const clr_bvh_node* bvh,
                  uint32_t n, const clr_float3* vertices, const clr_face* faces, const float* rays, uint32_t numRays,
-                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream) {
+                 uint32_t* hitFace, float* hitT, uint32_t* stats, hipStream_t stream, bool wide) {
     if (numRays == 0) return CLR_OK;
     int rc = ensureStatus(ctx);
     if (rc) return rc;
@@ -256,18 +375,21 @@ int traceOrdered(clr_ctx* ctx, const clr_lbvh_inode* inodes, const clr_lbvh_leaf
     const size_t lanes = (size_t)grid * kOrdBlock;
     rc = ensureTraceSpill(ctx, lanes * 2 * (dev::kStackPairs - kOrdHalf) * sizeof(uint32_t));
     if (rc) return rc;
-    const float4 *anodes = nullptr, *atris = nullptr;
-    rc = buildAccel(ctx, inodes, leaves, bvh, n, vertices, faces, &anodes, &atris, stream);
+    const float4 *anodes = nullptr, *atris = nullptr, *wnodes = nullptr;
+    rc = buildAccel(ctx, inodes, leaves, bvh, n, vertices, faces, &anodes, &atris, stream, wide ? &wnodes : nullptr);
     if (rc) return rc;
     const float4* b4 = reinterpret_cast<const float4*>(bvh);
-    if (stats)
-        k_trace_ordered<true><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
-                                                              hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
-                                                              anodes, atris, b4);
-    else
-        k_trace_ordered<false><<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays,
-                                                               hitFace, hitT, stats, ctx->devStatus, ctx->traceSpill,
-                                                               anodes, atris, b4);
+    auto launch = [&](auto kernel) {
+        kernel<<<grid, kOrdBlock, 0, stream>>>(inodes, leaves, n, vertices, faces, rays, numRays, hitFace, hitT, stats,
+                                               ctx->devStatus, ctx->traceSpill, anodes, atris, b4, wnodes);
+    };
+    if (wide) {
+        if (stats) launch(k_trace_ordered<true, true>);
+        else launch(k_trace_ordered<false, true>);
+    } else {
+        if (stats) launch(k_trace_ordered<true, false>);
+        else launch(k_trace_ordered<false, false>);
+    }
     return hipCheck(ctx, hipGetLastError(), "trace launch");
 }
 

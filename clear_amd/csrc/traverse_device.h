@@ -60,6 +60,13 @@ struct GeomScene {
     // reference-order walks use it instead of inodes / leaves, and the
     // traversal layout is indexed by its node numbers.
     const float4* bvh = nullptr;
+    // 4-wide records for the renderers' walk ([slots][8], slots as anodes):
+    // record i holds the up to four grandchildren of internal node i (the
+    // children of each internal child, a leaf child itself), as SoA
+    // float4s: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of slots 0..3, then the
+    // four references (face | kLeafRef, node, or kNone for an empty slot
+    // with an empty box). Unused when null.
+    const float4* wnodes = nullptr;
 };
 constexpr uint32_t kLeafRef = 0x80000000u;
 static_assert(sizeof(clr_lbvh_inode) == 48 && sizeof(clr_lbvh_leaf) == 48 && sizeof(clr_bvh_node) == 48,
@@ -545,6 +552,174 @@ struct Ordered {
         else if (depth == 0) done = true;
     }
 };
+
+// Ordered walk over the 4-wide records (GeomScene::wnodes): a visit loads
+// one 128-byte record and tests up to four grandchildren boxes, so the chain
+// of dependent node loads is half the binary walk's. Leaf entries are tested
+// nearest first at the visit; internal entries within t continue with the
+// nearest and push the others (reference, entry distance) farthest first;
+// a pop skips entries beyond the current t. A grandchild lies inside its
+// parent's box, so testing it alone admits every triangle the binary walk
+// would test, and the accepted hit is the same (closest t, then the smaller
+// face index, alpha rejections as in the binary walk). CAP: stack slots (two
+// per entry); a push beyond them is dropped and flagged.
+template <bool STATS, class Alpha, class Stack, int CAP>
+struct Wide {
+    GeomScene g;  // by value: rebind() switches scenes between rays
+    const Stack& stack;
+    Alpha alpha;
+    V3 o, d, inv;
+    float t, b1, b2;
+    uint32_t best, node, steps;
+    int depth;
+    bool done, over, any;
+    TraceStats st;
+
+    __device__ __forceinline__ Wide(const GeomScene& g_, const Stack& s_, const Alpha& a_)
+        : g(g_), stack(s_), alpha(a_) {}
+    __device__ __forceinline__ void rebind(const GeomScene& g_, const Alpha& a_) {
+        g = g_;
+        alpha = a_;
+    }
+
+    __device__ __forceinline__ void leaf(uint32_t face) {
+        if constexpr (STATS) ++st.tests;
+        float tt = t, x1, x2;
+        if (!triTestPacked(g.atris, o, d, face, tt, x1, x2)) return;
+        if (tt == t && face > best) return;
+        if (alpha.reject(face, x1, x2)) return;
+        t = tt;
+        best = face;
+        b1 = x1;
+        b2 = x2;
+        if (any && !isinf(tt)) done = true;
+    }
+
+    __device__ __forceinline__ void begin(V3 o_, V3 d_, float tmax, bool anyHit = false) {
+        o = o_;
+        d = d_;
+        inv = {safeInv(d.x), safeInv(d.y), safeInv(d.z)};
+        t = tmax;
+        b1 = b2 = 0.0f;
+        best = kNone;
+        node = kNone;
+        depth = 0;
+        done = false;
+        over = false;
+        any = anyHit;
+        steps = 0;
+        st = {0u, 0u};
+        if (g.n == 0) {
+            done = true;
+        } else if (g.n == 1) {
+            const float4* lp = g.bvh ? g.bvh : reinterpret_cast<const float4*>(g.leaves);
+            if (slabEnter(lp[0], lp[1], o, inv, t) <= t) leaf(reinterpret_cast<const uint32_t*>(lp + 2)[0]);
+            done = true;
+        } else {
+            const float4* np = g.bvh ? g.bvh : reinterpret_cast<const float4*>(g.inodes);
+            if (slabEnter(np[0], np[1], o, inv, t) <= t) node = 0;
+            else done = true;
+        }
+    }
+
+    __device__ __forceinline__ void step() {
+        // Each node is visited at most once: more than 2n steps means
+        // malformed records; stop and flag.
+        if (++steps > 2u * g.n + 2u) {
+            done = true;
+            over = true;
+            return;
+        }
+        if constexpr (STATS) ++st.visits;
+        const float4* wp = g.wnodes + 8 * (size_t)node;
+        const float4 lx = wp[0], ly = wp[1], lz = wp[2], hx = wp[3], hy = wp[4], hz = wp[5], rf = wp[6];
+        float k[4] = {slabEnter(make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), o, inv, t),
+                      slabEnter(make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), o, inv, t),
+                      slabEnter(make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), o, inv, t),
+                      slabEnter(make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), o, inv, t)};
+        uint32_t r[4] = {__float_as_uint(rf.x), __float_as_uint(rf.y), __float_as_uint(rf.z), __float_as_uint(rf.w)};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (!(k[c] <= t) || r[c] == kNone) {  // a miss (NaN) or an empty slot
+                k[c] = INFINITY;
+                r[c] = kNone;
+            }
+        }
+        // Nearest first: a five-exchange sorting network on (entry, ref).
+        auto cx = [&](int a, int b) {
+            if (k[b] < k[a]) {
+                const float tk = k[a];
+                k[a] = k[b];
+                k[b] = tk;
+                const uint32_t tr = r[a];
+                r[a] = r[b];
+                r[b] = tr;
+            }
+        };
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (r[c] != kNone && (r[c] & kLeafRef) && k[c] <= t) {
+                leaf(r[c] & ~kLeafRef);
+                if (done) return;
+            }
+        }
+        uint32_t next = kNone;
+        float enext = 0.0f;
+#pragma unroll
+        for (int c = 3; c >= 0; --c) {
+            if (!(r[c] & kLeafRef) && k[c] <= t) {
+                if (next != kNone) push(next, enext);
+                next = r[c];
+                enext = k[c];
+            }
+        }
+        if (next != kNone) {
+            node = next;
+            return;
+        }
+        node = kNone;
+        while (depth > 0) {
+            depth -= 2;
+            const uint32_t cand = stack.get(depth);
+            if (__uint_as_float(stack.get(depth + 1)) <= t) {
+                node = cand;
+                return;
+            }
+        }
+        done = true;
+    }
+
+    __device__ __forceinline__ void push(uint32_t ref, float entry) {
+        if (depth + 2 <= CAP) {
+            stack.put(depth, ref);
+            stack.put(depth + 1, __float_as_uint(entry));
+            depth += 2;
+        } else {
+            over = true;
+        }
+    }
+};
+
+// The 4-wide walk run to completion for one ray (as orderedHit).
+template <bool STATS, bool ANY, class Alpha = NoAlpha, class Stack = LdsStack, int CAP = kStackDepth>
+__device__ __forceinline__ uint32_t wideHit(const GeomScene& g, V3 o, V3 d, float& t, const Stack& stack,
+                                            TraceStats& st, float* b1Out = nullptr, float* b2Out = nullptr,
+                                            const Alpha& alpha = Alpha(), bool* overflow = nullptr) {
+    Wide<STATS, Alpha, Stack, CAP> w(g, stack, alpha);
+    w.begin(o, d, t, ANY);
+    while (!w.done) w.step();
+    t = w.t;
+    if (b1Out) *b1Out = w.b1;
+    if (b2Out) *b2Out = w.b2;
+    if (overflow && w.over) *overflow = true;
+    if constexpr (STATS) st = w.st;
+    return w.best;
+}
 
 // The ordered walk run to completion for one ray. Returns the face (kNone
 // when nothing was accepted) and updates t, b1, b2.

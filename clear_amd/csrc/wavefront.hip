@@ -50,6 +50,7 @@ constexpr uint32_t kFlagEndAfterShadow = 1u << 16;
 // Device-side counters (one 256-B line).
 enum : uint32_t {
     kCntExt0 = 0, kCntExt1 = 1, kCntShadow = 2, kCntHit = 3, kCntMiss = 4,
+    kCntTakeExt = 16, kCntTakeShadow = 24,  // 8 per-XCD work counters each (ChunkPool)
     kCntChunk = 8,
 };
 
@@ -176,19 +177,18 @@ __device__ __forceinline__ uint32_t appendIndex(uint32_t* counter, bool want) {
     return r;
 }
 
-// Work distribution of the persistent traversal kernels: each wave claims
-// chunks of kPoolChunk queue entries with one atomic and hands them to its
-// idle lanes (ballot + popcount ranks), refilling only once at least
-// kRefillMin lanes are idle (or none is busy) so that ray set-up runs with
-// most lanes together. All state is wave-uniform.
-// Work distribution of the persistent traversal kernels, without atomics:
-// each wave owns one contiguous range of the queue and hands its entries to
-// idle lanes (ballot + popcount ranks) once at least refillMin lanes are idle
-// (or none is busy), so that ray set-up runs with most lanes together.
-// Ranges are grouped by XCD (workgroups are dispatched to the 8 XCDs
-// round-robin, so block b runs on XCD b % 8): the waves of one XCD own one
-// contiguous eighth of the queue, keeping neighbouring pixels' rays in one
-// L2. The grid is a multiple of 8 blocks.
+
+// Work distribution of the persistent traversal kernels. Workgroups are
+// dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and the
+// waves of one XCD share one contiguous eighth of the queue, so neighbouring
+// pixels' rays meet in one L2. Inside its eighth a wave takes kChunk entries
+// at a time from that XCD's counter (one atomic by one lane) and hands them
+// to its idle lanes (ballot + popcount ranks) once at least refillMin lanes
+// are idle (or none is busy), so that ray set-up runs with most lanes
+// together; a wave that draws short walks takes more chunks, and the launch
+// ends about when the queue does. The counters are zeroed by a later kernel
+// of the same iteration (stream order, no memset: the iterations are
+// graph-captured).
 struct WavePool {
     uint32_t next, end;
     __device__ __forceinline__ explicit WavePool(uint32_t n) {
@@ -215,6 +215,47 @@ struct WavePool {
         return got;
     }
     __device__ __forceinline__ bool drained() const { return next >= end; }
+};
+
+struct ChunkPool {
+    static constexpr uint32_t kChunk = 64;
+    uint32_t lo, hi, next, end;  // this XCD's range; the wave's current chunk
+    uint32_t* take;              // this XCD's counter (entries taken from lo)
+    bool exhausted = false;
+    __device__ __forceinline__ ChunkPool(uint32_t n, uint32_t* takeBase) {
+        const uint32_t x = blockIdx.x % 8u;
+        lo = (uint32_t)((uint64_t)n * x / 8u);
+        hi = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
+        next = end = lo;
+        take = takeBase + x;
+    }
+    // Returns true in lanes that got an entry (index in `j`). Every lane of
+    // the wave must call it (the kernels' waves are full and never exit
+    // early).
+    __device__ __forceinline__ bool refill(uint32_t refillMin, bool busy, uint32_t& j) {
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t nIdle = (uint32_t)__popcll(idle);
+        if (nIdle == 0 || (nIdle < refillMin && __ballot(busy) != 0)) return false;
+        if (next >= end) {
+            if (exhausted) return false;
+            uint32_t off = 0;
+            if (__lane_id() == 0) off = atomicAdd(take, kChunk);
+            off = __shfl(off, 0);
+            if (off >= hi - lo) {
+                exhausted = true;
+                return false;
+            }
+            next = lo + off;
+            end = min(next + kChunk, hi);
+        }
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
+        const uint32_t got = min(nIdle, end - next);
+        const bool mine = !busy && rank < got;
+        if (mine) j = next + rank;
+        next += got;
+        return mine;
+    }
+    __device__ __forceinline__ bool drained() const { return exhausted; }
 };
 
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -343,7 +384,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(WfArgs A, uint32_t in) {
         const RScene* S0 = firstScene<BATCH>(A, S);
         const SpillStack<LD> stk1{s_stack + LD * blockDim.x + threadIdx.x, (int)blockDim.x,
                                   B.spill + (size_t)(kStackPairs - kMinLdsDepth) * lanes + gid, lanes};
-        using W = Ordered<false, TexAlpha, SpillStack<LD>, true>;
+        using W = Wide<false, TexAlpha, SpillStack<LD>, kStackPairs>;
         W w0(S0->g, stk, TexAlpha{S0}), w1(S0->g, stk1, TexAlpha{S0});
         WavePool pool(n);
         bool busy0 = false, busy1 = false;
@@ -618,7 +659,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
         const RScene* S0 = firstScene<BATCH>(A, S);
         const SpillStack<LD> stk1{s_stack + LD * blockDim.x + threadIdx.x, (int)blockDim.x,
                                   B.spill + (size_t)(kStackPairs - kMinLdsDepth) * lanes + gid, lanes};
-        using W = Ordered<false, TexAlpha, SpillStack<LD>, true>;
+        using W = Wide<false, TexAlpha, SpillStack<LD>, kStackPairs>;
         W w0(S0->g, stk, TexAlpha{S0}), w1(S0->g, stk1, TexAlpha{S0});
         WavePool pool(n);
         bool busy0 = false, busy1 = false;
@@ -714,6 +755,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
 // records.
 __global__ __launch_bounds__(256) void k_wf_compact(WfArgs A, uint32_t in) {
     const WfBufs& B = A.B;
+    if (blockIdx.x == 0 && threadIdx.x < 8) B.counts[kCntTakeExt + threadIdx.x] = 0u;  // for the next extend launch
     const uint32_t n = min(B.counts[in], B.slots);
     const uint32_t lanes = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
@@ -738,6 +780,7 @@ __global__ __launch_bounds__(256) void k_wf_shadow_end(WfArgs A, uint32_t in) {
     const WfBufs& B = A.B;
     const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
+    if (blockIdx.x == 0 && threadIdx.x < 8) B.counts[kCntTakeShadow + threadIdx.x] = 0u;  // for the next shadow launch
     const uint32_t n = min(B.counts[kCntShadow], B.slots);
     const uint32_t lanes = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
@@ -836,7 +879,7 @@ int ensureWfState(clr_ctx* ctx) {
 Launch traceLaunch(const WfState& w) {
     const uint32_t b = w.cfg.traceBlock;
     const uint32_t g = std::max(1u, (uint32_t)w.numCUs * w.cfg.traceWavesPerCU * 64u / b);
-    return {(g + 7u) / 8u * 8u, b};  // whole XCD groups (WavePool)
+    return {(g + 7u) / 8u * 8u, b};  // whole XCD groups (ChunkPool)
 }
 Launch shadeLaunch(const WfState& w) {
     const uint32_t b = w.cfg.shadeBlock;
@@ -1065,7 +1108,8 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
     size_t bytes = align(sizeof(RScene) * numScenes) + align(sizeof(int) * numScenes);
     for (uint32_t s = 0; s < numScenes; ++s) {
         const uint32_t n = scenes[s].numFaces;
-        bytes += accelNodeBytes(n, scenes[s].bvhNodes != nullptr) + align(48ull * n) + align(4ull * n);
+        bytes += accelNodeBytes(n, scenes[s].bvhNodes != nullptr) + accelWideBytes(n, scenes[s].bvhNodes != nullptr) +
+                 align(48ull * n) + align(4ull * n);
     }
     if (ctx->batchBytes < bytes) {
         bool capturing = false;
@@ -1095,14 +1139,16 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
         const clr_scene_dev& sd = scenes[s];
         const uint32_t n = sd.numFaces;
         float4* an = c.take<float4>(accelNodeBytes(n, sd.bvhNodes != nullptr) / sizeof(float4));
+        float4* wn = c.take<float4>(accelWideBytes(n, sd.bvhNodes != nullptr) / sizeof(float4));
         float4* at = c.take<float4>(3ull * n);
         uint32_t* fl = c.take<uint32_t>(n);
         launchAccelBuild(sd.internalNodes, sd.leafNodes, sd.bvhNodes, n, sd.vertices, sd.faces, an, at, stream);
+        launchAccelWide(sd.internalNodes, sd.leafNodes, sd.bvhNodes, n, wn, stream);
         launchFaceLights(sd.lights, sd.numLights, n, fl, envWords + s, stream);
         RScene& S = table[s];
         std::memset(&S, 0, sizeof(S));
         S.g = GeomScene{sd.internalNodes, sd.leafNodes, n, sd.vertices, sd.faces, an, at,
-                        reinterpret_cast<const float4*>(sd.bvhNodes)};
+                        reinterpret_cast<const float4*>(sd.bvhNodes), wn};
         S.normals = sd.normals;
         S.tangents = sd.tangents;
         S.uvs = sd.uvs;

@@ -230,12 +230,26 @@ int clr_trace_closest(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
  * two faces tie in t exactly (the reference keeps the last one its walk
  * tests) or a ray grazes a box within rounding. stats count entered internal
  * nodes and triangle tests of this traversal. No counterpart in the
- * reference; it is the traversal the renderers use by default. */
+ * reference; the renderers use its 4-wide form (clr_trace_closest_wide). */
 int clr_trace_closest_ordered(clr_ctx* ctx, const clr_lbvh_inode* internalNodes,
                               const clr_lbvh_leaf* leafNodes, uint32_t numFaces,
                               const clr_float3* vertices, const clr_face* faces,
                               const float* rays, uint32_t numRays, uint32_t* hitFace, float* hitT,
                               uint32_t* stats, void* stream);
+
+/* Same arguments, outputs and hits as clr_trace_closest_ordered, with the
+ * 4-wide form of the ordered traversal the renderers use: the LBVH's
+ * internal nodes are regrouped into 128-byte records of up to four
+ * grandchildren (the children of each internal child, a leaf child itself),
+ * a visit slab-tests all four, tests the leaf entries nearest first, enters
+ * the nearest internal one and pushes the others with their entry distance
+ * (a pop skips entries beyond the current t). Every grandchild lies inside
+ * its parent's box, so the accepted (t, face) is the ordered traversal's;
+ * stats count visited records (about half the binary walk's node visits)
+ * and triangle tests. No counterpart in the reference. */
+int clr_trace_closest_wide(clr_ctx* ctx, const clr_lbvh_inode* internalNodes, const clr_lbvh_leaf* leafNodes,
+                           uint32_t numFaces, const clr_float3* vertices, const clr_face* faces, const float* rays,
+                           uint32_t numRays, uint32_t* hitFace, float* hitT, uint32_t* stats, void* stream);
 
 /* Closest hit over the CPU midpoint BVH (nodes: device clr_bvh_node[2 *
  * numFaces - 1], e.g. from clr_bvh_build_midpoint). ordered == 0: the

@@ -125,6 +125,27 @@ def trace_ordered(inodes, leaves, n, verts, faces, rays, any_hit=False):
     return hit, t, st
 
 
+def trace_wide(inodes, leaves, n, verts, faces, rays, any_hit=False):
+    """4-wide form of the ordered traversal: (face, t, stats[R, 2]) with
+    visited records and tests."""
+    L = lib()
+    fn = L.orc_trace_wide
+    fn.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    fn.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint32, ctypes.c_int, vp, vp, vp]
+    R = rays.shape[1]
+    hit = np.zeros(R, dtype=np.uint32)
+    t = np.zeros(R, dtype=np.float32)
+    st = np.zeros((R, 2), dtype=np.uint32)
+    inodes = np.ascontiguousarray(inodes) if n > 1 else np.zeros((1, 48), np.uint8)
+    rc = fn(_p(inodes), _p(np.ascontiguousarray(leaves)), n, _p(np.ascontiguousarray(verts)),
+            _p(np.ascontiguousarray(faces)), _p(np.ascontiguousarray(rays, dtype=np.float32)), R,
+            int(any_hit), _p(hit), _p(t), _p(st))
+    if rc:
+        raise RuntimeError(f"orc_trace_wide failed: {rc}")
+    return hit, t, st
+
+
 def trace_bvh(nodes, verts, faces, rays):
     """Midpoint-BVH walk restatement (reference order): (face, t, stats[R, 2])."""
     L = lib()
@@ -262,7 +283,8 @@ def _probe_fn(name, argtypes):
 
 
 def work_counters(reset=True, count_any_hit=True):
-    """Traversal work of oracle renders (ordered walk) since the last reset:
+    """Traversal work of oracle renders (the ordered walk's 4-wide form: a
+    visit is one record of up to four grandchildren) since the last reset:
     dict of rays, visits, tests (closest-hit walks) and, for environment
     shadow rays, their count, closest-hit visits / tests and the visits /
     tests of an any-hit walk of the same rays. count_any_hit makes later

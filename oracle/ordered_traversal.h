@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <utility>
 
 #include "../include/clear_amd.h"
 
@@ -112,6 +113,124 @@ uint32_t traverse(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uin
             const uint32_t cand = stack[--depth];
             if (slabEnter(inodes[cand].min, inodes[cand].max, o, inv, t) <= t) {
                 node = cand;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// 4-wide form of the same traversal (the renderers' walk; the library keeps
+// records of up to four grandchildren per internal node). A visit of node N
+// gathers the children of each internal child of N, or a leaf child itself,
+// in the order left.left, left.right, right.left, right.right; it tests all
+// of their boxes against t, sorts them by entry distance (a five-exchange
+// network that swaps only on a strictly smaller entry), tests the leaf
+// entries in that order, then continues with the nearest internal entry whose
+// entry is still <= t and pushes the other such entries (with their entry
+// distance) farthest first; a pop skips entries beyond the current t. Each
+// gathered box lies inside its parent's, so the result is the same
+// lexicographic minimum of (t, face); only the visit counts (one per record)
+// differ. stackPairs bounds the stack (the library's: 32 pairs in the
+// megakernels' LDS stack, 64 in the wavefront and trace kernels); a push
+// beyond it is dropped and flagged.
+template <class TriFn>
+uint32_t traverseWide(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n, const float o[3],
+                      const float d[3], float& t, TriFn tri, Counts& cnt, bool anyHit, bool* overflow,
+                      int stackPairs = 64) {
+    const float inv[3] = {safeInv(d[0]), safeInv(d[1]), safeInv(d[2])};
+    uint32_t best = 0xFFFFFFFFu;
+    bool done = false;
+    auto leaf = [&](uint32_t li) {
+        const uint32_t obj = leaves[li].objIdx;
+        ++cnt.tests;
+        float tt;
+        if (!tri(obj, t, &tt)) return;
+        if (tt == t && obj > best) return;
+        t = tt;
+        best = obj;
+        if (anyHit && !std::isinf(tt)) done = true;
+    };
+    if (n == 0) return best;
+    if (n == 1) {
+        if (slabEnter(leaves[0].min, leaves[0].max, o, inv, t) <= t) leaf(0);
+        return best;
+    }
+    if (!(slabEnter(inodes[0].min, inodes[0].max, o, inv, t) <= t)) return best;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t stackRef[64];
+    float stackE[64];
+    int depth = 0;
+    uint32_t node = 0;
+    while (true) {
+        ++cnt.visits;
+        const clr_lbvh_inode& N = inodes[node];
+        // The up to four entries: index, leaf flag, box.
+        uint32_t ref[4] = {kNone, kNone, kNone, kNone};
+        bool isLeaf[4] = {false, false, false, false};
+        float k[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+        for (int c = 0; c < 2; ++c) {
+            if (N.isChild[c]) {
+                ref[2 * c] = N.c[c];
+                isLeaf[2 * c] = true;
+                continue;
+            }
+            const clr_lbvh_inode& C = inodes[N.c[c]];
+            for (int m = 0; m < 2; ++m) {
+                ref[2 * c + m] = C.c[m];
+                isLeaf[2 * c + m] = C.isChild[m] != 0;
+            }
+        }
+        for (int c = 0; c < 4; ++c) {
+            if (ref[c] == kNone) continue;
+            const float* mn = isLeaf[c] ? leaves[ref[c]].min : inodes[ref[c]].min;
+            const float* mx = isLeaf[c] ? leaves[ref[c]].max : inodes[ref[c]].max;
+            const float e = slabEnter(mn, mx, o, inv, t);
+            if (e <= t) k[c] = e;
+            else ref[c] = kNone;
+        }
+        auto cx = [&](int a, int b) {
+            if (k[b] < k[a]) {
+                std::swap(k[a], k[b]);
+                std::swap(ref[a], ref[b]);
+                std::swap(isLeaf[a], isLeaf[b]);
+            }
+        };
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
+        for (int c = 0; c < 4 && !done; ++c)
+            if (ref[c] != kNone && isLeaf[c] && k[c] <= t) leaf(ref[c]);
+        if (done) break;
+        uint32_t next = kNone;
+        float enext = 0.0f;
+        for (int c = 3; c >= 0; --c) {
+            if (ref[c] == kNone || isLeaf[c] || !(k[c] <= t)) continue;
+            if (next != kNone) {
+                if (depth < stackPairs && depth < 64) {
+                    stackRef[depth] = next;
+                    stackE[depth] = enext;
+                    ++depth;
+                } else if (overflow) {
+                    *overflow = true;
+                }
+            }
+            next = ref[c];
+            enext = k[c];
+        }
+        if (next != kNone) {
+            node = next;
+            continue;
+        }
+        bool found = false;
+        while (depth > 0) {
+            --depth;
+            if (stackE[depth] <= t) {
+                node = stackRef[depth];
                 found = true;
                 break;
             }

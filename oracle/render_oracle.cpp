@@ -73,9 +73,9 @@ enum Plant : int {
 int g_plant = 0;
 inline bool planted(int k) { return g_plant == k; }
 
-// Traversal work of renders with the ordered walk, for the render roofline
-// (orc_work_counters): [0] rays traced, [1] internal-node visits, [2]
-// triangle tests, all as closest-hit walks; of those, environment shadow rays:
+// Traversal work of renders with the ordered walk (its 4-wide form), for the
+// render roofline (orc_work_counters): [0] rays traced, [1] record visits,
+// [2] triangle tests, all as closest-hit walks; of those, environment shadow rays:
 // [3] count, [4] visits, [5] tests, and [6] visits, [7] tests of the any-hit
 // walk the wavefront renderer uses for them instead (counted on the side;
 // the restatement's results do not change).
@@ -461,10 +461,12 @@ void hitParams(const SceneC& s, V3 org, V3 dir, float t, const Isect& is, Surfac
             sp->uDir = normalize(uDir);
     }
 }
-// rayIntersection (bvh_traversal.cl:182-215), or the ordered traversal.
-// boxHits (optional): rayIntersectionVis's count (bvh_traversal.cl:216-319:
-// internal and leaf boxes passed on the LBVH, node boxes passed on the
-// midpoint BVH), or the ordered traversal's entered nodes plus tests.
+// rayIntersection (bvh_traversal.cl:182-215), or the ordered traversal: its
+// 4-wide form for renders (the library's renderers walk that form; the hit
+// is the same), the binary form for the BVH view. boxHits (optional):
+// rayIntersectionVis's count (bvh_traversal.cl:216-319: internal and leaf
+// boxes passed on the LBVH, node boxes passed on the midpoint BVH), or the
+// binary ordered traversal's entered nodes plus tests.
 bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp, uint32_t* boxHits = nullptr) {
     uint32_t hits = 0;
     struct Commit {
@@ -486,7 +488,9 @@ bool rayIntersection(const SceneC& s, V3 org, V3 dir, SurfacePoint* sp, uint32_t
             return true;
         };
         orc_ordered::Counts cnt;
-        const uint32_t f = orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr);
+        const uint32_t f =
+            boxHits ? orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr)
+                    : orc_ordered::traverseWide(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, false, nullptr);
         hits = cnt.visits + cnt.tests;
         ++t_work[0];
         t_work[1] += cnt.visits;
@@ -1185,7 +1189,7 @@ void countAnyHit(const SceneC& s, V3 org, V3 dir, uint64_t v0, uint64_t c0) {
     };
     float t = INFINITY;
     orc_ordered::Counts cnt;
-    orc_ordered::traverse(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, true, nullptr);
+    orc_ordered::traverseWide(s.inodes, s.leaves, s.numFaces, o, d, t, tri, cnt, true, nullptr);
     t_work[6] += cnt.visits;
     t_work[7] += cnt.tests;
 }

@@ -235,6 +235,38 @@ int orc_trace_ordered(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves,
     return 0;
 }
 
+// The 4-wide form of the ordered traversal (orc_ordered::traverseWide): same
+// outputs as orc_trace_ordered, stats count visited records and tests.
+int orc_trace_wide(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, uint32_t n, const clr_float3* verts,
+                   const clr_face* faces, const float* rays, uint32_t numRays, int anyHit, uint32_t* hitFace,
+                   float* hitT, uint32_t* stats) {
+    for (uint32_t r = 0; r < numRays; ++r) {
+        const size_t R = numRays;
+        const float o[3] = {rays[r], rays[R + r], rays[2 * R + r]};
+        const float d[3] = {rays[3 * R + r], rays[4 * R + r], rays[5 * R + r]};
+        float t = rays[6 * R + r];
+        const V3 org = {o[0], o[1], o[2]}, dir = {d[0], d[1], d[2]};
+        auto tri = [&](uint32_t face, float bound, float* tt) {
+            float x = bound;
+            if (!triTest(verts, faces, org, dir, face, &x)) return false;
+            *tt = x;
+            return true;
+        };
+        orc_ordered::Counts cnt;
+        bool over = false;
+        const uint32_t best = orc_ordered::traverseWide(inodes, leaves, n, o, d, t, tri, cnt, anyHit != 0, &over);
+        if (over) return 2;
+        const bool hit = best != 0xFFFFFFFFu && !std::isinf(t);
+        hitFace[r] = hit ? best : 0xFFFFFFFFu;
+        hitT[r] = t;
+        if (stats) {
+            stats[2 * (size_t)r + 0] = cnt.visits;
+            stats[2 * (size_t)r + 1] = cnt.tests;
+        }
+    }
+    return 0;
+}
+
 // Brute force with the ordered traversal's rule: the lexicographic minimum of
 // (t, face) over all faces the triangle test accepts.
 int orc_trace_bruteforce_minid(const clr_float3* verts, const clr_face* faces, uint32_t n, const float* rays,

@@ -9,7 +9,7 @@ import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(ctx, o_inodes, o_leaves, n, v, f, rays, stats=True, ordered=False):
+def run_gpu(ctx, o_inodes, o_leaves, n, v, f, rays, stats=True, ordered=False, wide=False):
     import torch
 
     R = rays.shape[1]
@@ -22,7 +22,7 @@ def run_gpu(ctx, o_inodes, o_leaves, n, v, f, rays, stats=True, ordered=False):
     hit = torch.empty(R, dtype=torch.int32, device="cuda")
     t = torch.empty(R, dtype=torch.float32, device="cuda")
     st = torch.empty(2 * R, dtype=torch.int32, device="cuda") if stats else None
-    ca.trace_closest(ctx, ti, tl, n, tv, tf, tr, R, hit, t, st, ordered=ordered)
+    ca.trace_closest(ctx, ti, tl, n, tv, tf, tr, R, hit, t, st, ordered=ordered, wide=wide)
     torch.cuda.synchronize()
     out = (hit.cpu().numpy().view(np.uint32), t.cpu().numpy())
     if stats:
@@ -115,6 +115,54 @@ def test_ordered_1m_tris_256k_rays_match_reference_walk(gpu_ctx):
     o = orc.lbvh_build(v, f, 10)
     gh, gt, _ = run_gpu(gpu_ctx, o["inodes"], o["leaves"], 1 << 20, v, f, rays)
     assert (gh == h).mean() > 0.99999
+
+
+# ---- 4-wide ordered walk (clr_trace_closest_wide) ----------------------------
+
+def check_wide(ctx, soup, rays, bits=10):
+    """The wide walk (the renderers' traversal) against the oracle's
+    restatement: hits, distances and per-ray record visits and tests; and its
+    hits against the binary ordered walk's."""
+    n = soup.shape[0]
+    print(f"[trace wide parity] n={n} rays={rays.shape[1]} bits={bits}", flush=True)
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, bits)
+    h, t, st = orc.trace_wide(o["inodes"], o["leaves"], n, v, f, rays)
+    gh, gt, gst = run_gpu(ctx, o["inodes"], o["leaves"], n, v, f, rays, wide=True)
+    np.testing.assert_array_equal(gh, h)
+    np.testing.assert_array_equal(gt.view(np.uint32), t.view(np.uint32))
+    np.testing.assert_array_equal(gst, st)
+    oh, ot, _ = run_gpu(ctx, o["inodes"], o["leaves"], n, v, f, rays, ordered=True)
+    np.testing.assert_array_equal(gh, oh)
+    np.testing.assert_array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    assert ctx.device_status() == 0
+    return h
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 1000])
+def test_wide_small_scenes(gpu_ctx, n):
+    soup = ca.gen_cluster_soup(n, seed=n, num_clusters=4, sigma=0.1, e_min=0.02, e_max=0.1)
+    check_wide(gpu_ctx, soup, ca.gen_random_rays(4096, seed=n + 1))
+
+
+def test_wide_axis_aligned_and_finite_tmax(gpu_ctx):
+    soup = ca.gen_cluster_soup(20000, seed=3, num_clusters=8, sigma=0.05, e_min=0.005, e_max=0.03)
+    rays = ca.gen_random_rays(8192, seed=4)
+    rays[3:6, :2048] = 0.0
+    rays[3 + (np.arange(2048) % 3), np.arange(2048)] = np.where(np.arange(2048) % 2, 1.0, -1.0)
+    rays[6, 4096:] = np.float32(0.3)
+    check_wide(gpu_ctx, soup, rays)
+
+
+def test_wide_1m_tris_several_rays_per_lane(gpu_ctx):
+    soup = ca.gen_cluster_soup(1 << 20, seed=1)
+    h = check_wide(gpu_ctx, soup, ca.gen_random_rays(1 << 19, seed=2))
+    assert (h != 0xFFFFFFFF).mean() > 0.1
+
+
+def test_wide_20_bits_per_axis(gpu_ctx):
+    soup = ca.gen_cluster_soup(1 << 16, seed=9)
+    check_wide(gpu_ctx, soup, ca.gen_random_rays(1 << 15, seed=3), bits=20)
 
 
 # ---- CPU midpoint BVH (clr_trace_closest_bvh) --------------------------------

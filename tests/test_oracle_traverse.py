@@ -124,6 +124,48 @@ def test_ordered_visits_fewer_nodes_on_clusters():
     assert st_ord[:, 1].sum() < st_ref[:, 1].sum()  # fewer triangle tests
 
 
+# ---- 4-wide form of the ordered traversal (traverseWide) -----------------------
+
+@pytest.mark.parametrize("n,seed", [(1, 3), (2, 4), (3, 9), (5, 8), (50, 5), (3000, 6), (20000, 7)])
+def test_wide_equals_bruteforce_min_t_then_face(n, seed):
+    """Testing grandchildren boxes without their parent's never culls an
+    accepted intersection either: faces and t equal the brute-force
+    lexicographic minimum of (t, face); any-hit agrees on hit / miss."""
+    soup, v, f = scene(n, seed)
+    o = orc.lbvh_build(v, f, 10)
+    rays = axis_and_bounded_rays(4000, seed + 100)
+    h, t, st = orc.trace_wide(o["inodes"], o["leaves"], n, v, f, rays)
+    hb, tb = orc.trace_bruteforce_minid(v, f, rays)
+    np.testing.assert_array_equal(h, hb)
+    np.testing.assert_array_equal(t.view(np.uint32), tb.view(np.uint32))
+    ha, _, _ = orc.trace_wide(o["inodes"], o["leaves"], n, v, f, rays, any_hit=True)
+    np.testing.assert_array_equal(ha != 0xFFFFFFFF, h != 0xFFFFFFFF)
+
+
+def test_wide_tie_goes_to_smaller_face():
+    tri = [0, 0, 1, 1, 0, 1, 0, 1, 1]
+    soup = np.float32([tri, tri, [5, 5, 5, 6, 5, 5, 5, 6, 5], tri])
+    v, f = ca.soup_to_mesh(soup)
+    o = orc.lbvh_build(v, f, 10)
+    rays = np.zeros((7, 1), np.float32)
+    rays[:, 0] = [0.2, 0.2, 0.0, 0.0, 0.0, 1.0, np.inf]
+    h, t, _ = orc.trace_wide(o["inodes"], o["leaves"], 4, v, f, rays)
+    assert h[0] == 0 and t[0] == 1.0
+
+
+def test_wide_visits_half_the_records():
+    """One record per visit stands for two levels of the binary tree: on a
+    cluster soup the wide walk makes at most 60 % of the binary walk's visits
+    and the same number of triangle tests within 5 %."""
+    soup, v, f = scene(20000, 11)
+    o = orc.lbvh_build(v, f, 10)
+    rays = ca.gen_random_rays(4000, seed=12)
+    _, _, st_ord = orc.trace_ordered(o["inodes"], o["leaves"], 20000, v, f, rays)
+    _, _, st_w = orc.trace_wide(o["inodes"], o["leaves"], 20000, v, f, rays)
+    assert st_w[:, 0].sum() <= 0.6 * st_ord[:, 0].sum()
+    assert abs(int(st_w[:, 1].sum()) - int(st_ord[:, 1].sum())) <= 0.05 * st_ord[:, 1].sum()
+
+
 # ---- BVH visualizer (orc_bvh_visualizer) ---------------------------------------
 
 @pytest.mark.parametrize("walk", ["lbvh", "midpoint", "ordered"])
