@@ -50,7 +50,6 @@ constexpr uint32_t kFlagEndAfterShadow = 1u << 16;
 // Device-side counters (one 256-B line).
 enum : uint32_t {
     kCntExt0 = 0, kCntExt1 = 1, kCntShadow = 2, kCntHit = 3, kCntMiss = 4,
-    kCntTakeExt = 16, kCntTakeShadow = 24,  // 8 per-XCD work counters each (ChunkPool)
     kCntChunk = 8,
 };
 
@@ -177,18 +176,16 @@ __device__ __forceinline__ uint32_t appendIndex(uint32_t* counter, bool want) {
     return r;
 }
 
-
-// Work distribution of the persistent traversal kernels. Workgroups are
-// dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and the
-// waves of one XCD share one contiguous eighth of the queue, so neighbouring
-// pixels' rays meet in one L2. Inside its eighth a wave takes kChunk entries
-// at a time from that XCD's counter (one atomic by one lane) and hands them
-// to its idle lanes (ballot + popcount ranks) once at least refillMin lanes
-// are idle (or none is busy), so that ray set-up runs with most lanes
-// together; a wave that draws short walks takes more chunks, and the launch
-// ends about when the queue does. The counters are zeroed by a later kernel
-// of the same iteration (stream order, no memset: the iterations are
-// graph-captured).
+// Work distribution of the persistent traversal kernels, without atomics:
+// each wave owns one contiguous range of the queue and hands its entries to
+// idle lanes (ballot + popcount ranks) once at least refillMin lanes are idle
+// (or none is busy), so that ray set-up runs with most lanes together.
+// Ranges are grouped by XCD (workgroups are dispatched to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8): the waves of one XCD own one
+// contiguous eighth of the queue, keeping neighbouring pixels' rays in one
+// L2. The grid is a multiple of 8 blocks. (Chunks of 64 entries taken from a
+// per-XCD atomic counter balance the waves better but rendered 12 % slower:
+// a wave's contiguous range keeps neighbouring rays together.)
 struct WavePool {
     uint32_t next, end;
     __device__ __forceinline__ explicit WavePool(uint32_t n) {
@@ -215,47 +212,6 @@ struct WavePool {
         return got;
     }
     __device__ __forceinline__ bool drained() const { return next >= end; }
-};
-
-struct ChunkPool {
-    static constexpr uint32_t kChunk = 64;
-    uint32_t lo, hi, next, end;  // this XCD's range; the wave's current chunk
-    uint32_t* take;              // this XCD's counter (entries taken from lo)
-    bool exhausted = false;
-    __device__ __forceinline__ ChunkPool(uint32_t n, uint32_t* takeBase) {
-        const uint32_t x = blockIdx.x % 8u;
-        lo = (uint32_t)((uint64_t)n * x / 8u);
-        hi = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
-        next = end = lo;
-        take = takeBase + x;
-    }
-    // Returns true in lanes that got an entry (index in `j`). Every lane of
-    // the wave must call it (the kernels' waves are full and never exit
-    // early).
-    __device__ __forceinline__ bool refill(uint32_t refillMin, bool busy, uint32_t& j) {
-        const uint64_t idle = __ballot(!busy);
-        const uint32_t nIdle = (uint32_t)__popcll(idle);
-        if (nIdle == 0 || (nIdle < refillMin && __ballot(busy) != 0)) return false;
-        if (next >= end) {
-            if (exhausted) return false;
-            uint32_t off = 0;
-            if (__lane_id() == 0) off = atomicAdd(take, kChunk);
-            off = __shfl(off, 0);
-            if (off >= hi - lo) {
-                exhausted = true;
-                return false;
-            }
-            next = lo + off;
-            end = min(next + kChunk, hi);
-        }
-        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
-        const uint32_t got = min(nIdle, end - next);
-        const bool mine = !busy && rank < got;
-        if (mine) j = next + rank;
-        next += got;
-        return mine;
-    }
-    __device__ __forceinline__ bool drained() const { return exhausted; }
 };
 
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -755,7 +711,6 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
 // records.
 __global__ __launch_bounds__(256) void k_wf_compact(WfArgs A, uint32_t in) {
     const WfBufs& B = A.B;
-    if (blockIdx.x == 0 && threadIdx.x < 8) B.counts[kCntTakeExt + threadIdx.x] = 0u;  // for the next extend launch
     const uint32_t n = min(B.counts[in], B.slots);
     const uint32_t lanes = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
@@ -780,7 +735,6 @@ __global__ __launch_bounds__(256) void k_wf_shadow_end(WfArgs A, uint32_t in) {
     const WfBufs& B = A.B;
     const uint32_t chunk = B.counts[kCntChunk];
     const uint32_t out = in ^ 1u;
-    if (blockIdx.x == 0 && threadIdx.x < 8) B.counts[kCntTakeShadow + threadIdx.x] = 0u;  // for the next shadow launch
     const uint32_t n = min(B.counts[kCntShadow], B.slots);
     const uint32_t lanes = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
@@ -879,7 +833,7 @@ int ensureWfState(clr_ctx* ctx) {
 Launch traceLaunch(const WfState& w) {
     const uint32_t b = w.cfg.traceBlock;
     const uint32_t g = std::max(1u, (uint32_t)w.numCUs * w.cfg.traceWavesPerCU * 64u / b);
-    return {(g + 7u) / 8u * 8u, b};  // whole XCD groups (ChunkPool)
+    return {(g + 7u) / 8u * 8u, b};  // whole XCD groups (WavePool)
 }
 Launch shadeLaunch(const WfState& w) {
     const uint32_t b = w.cfg.shadeBlock;
