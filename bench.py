@@ -39,6 +39,7 @@ Secondary legs of the same metric family, each on its stated workload:
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-trace]
                     [--no-render] [--no-adjoint] [--no-batch] [--no-dropin]
+                    [--wavefront-config key=value,...]
 """
 from __future__ import annotations
 
@@ -237,7 +238,14 @@ def render_leg(ca, np, torch, ctx, args):
     tune_t = time.perf_counter()
     # Tuned on 4-spp renders (about 30 ms each at 1080p): 1-spp timings are
     # noisy enough to settle on a different configuration from run to run.
-    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 4, depth)
+    if args.wavefront_config:  # a fixed configuration (profiling runs), no search
+        cfg = ca.wavefront_config(ctx)
+        for kv in args.wavefront_config.split(","):
+            k, v = kv.split("=")
+            setattr(cfg, k.strip(), int(v))
+        ca.set_wavefront_config(ctx, cfg)
+    else:
+        cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 4, depth)
     out["wavefront_config"] = cfg.as_dict()
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     legs = [("megakernel_plain", ca.RENDER_MEGAKERNEL, ca.RENDER_REFERENCE_TRAVERSAL),
@@ -542,6 +550,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (Face + vertices) build leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--wavefront-config", default=None,
+                    help="fixed wavefront configuration as key=value pairs (e.g. traceWavesPerCU=24,ldsDepth=16), "
+                         "used instead of the autotuner")
     ap.add_argument("--no-autotune", action="store_true",
                     help="render legs use the default wavefront configuration (for kernel profiles)")
     ap.add_argument("--batch-per-scene", type=int, default=32,

@@ -809,8 +809,10 @@ clr_wavefront_config defaultConfig() {
 
 int validConfig(const clr_wavefront_config& c) {
     auto okBlock = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    // Up to 128 waves per CU: beyond what a CU holds (at most 32), the
+    // dispatcher starts the remaining waves as resident ones finish.
     return okBlock(c.traceBlock) && okBlock(c.shadeBlock) && c.traceWavesPerCU >= 1 &&
-           c.traceWavesPerCU <= 32 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 32 &&
+           c.traceWavesPerCU <= 128 && c.shadeWavesPerCU >= 1 && c.shadeWavesPerCU <= 128 &&
            c.itersPerGraph >= 2 && c.itersPerGraph <= 64 && (c.itersPerGraph % 2) == 0 && c.maxSlots >= 64 &&
            c.refillMin >= 1 && c.refillMin <= 64 && (c.ldsDepth == 16 || c.ldsDepth == 32) &&
            (c.raysPerLane == 1 || c.raysPerLane == 2);

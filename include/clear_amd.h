@@ -455,9 +455,9 @@ int clr_seed_pixel_states(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_
  * flight, replayed from a captured hipGraph of itersPerGraph iterations. */
 typedef struct clr_wavefront_config {
     uint32_t traceBlock;      /* threads per block of the extend / shadow kernels: 64, 128, 256 */
-    uint32_t traceWavesPerCU; /* their grid: waves per CU, 1..32 */
+    uint32_t traceWavesPerCU; /* their grid: waves per CU, 1..128 (above 32, more waves than a CU holds) */
     uint32_t shadeBlock;      /* threads per block of the shade / miss kernels: 64, 128, 256 */
-    uint32_t shadeWavesPerCU; /* their grid: waves per CU, 1..32 */
+    uint32_t shadeWavesPerCU; /* their grid: waves per CU, 1..128 */
     uint32_t itersPerGraph;   /* iterations per captured graph: even, 2..64 */
     uint32_t checkEvery;      /* graph launches between early-stop checks (a host sync);
                                  0: never, the render stays fully asynchronous */
