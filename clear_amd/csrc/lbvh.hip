@@ -92,10 +92,10 @@ __device__ __forceinline__ Box boxOf3(const float* p0, const float* p1, const fl
     return b;
 }
 
-// Primitive source: a 36-B triangle soup, plus a box array of 32-B records
-// (lo.xyz, hi.x | hi.yz, triangle index, 0) in input order, written by the
-// first pass that sees the triangles. The hierarchy pass gathers one record
-// per leaf by the sorted index.
+// Primitive source: a 36-B triangle soup, or (the drop-in path) a box array
+// of 32-B records (lo.xyz, hi.x | hi.yz, triangle index, 0) in input order,
+// written by the bounds pass. The hierarchy pass gathers one triangle (soup)
+// or record per leaf by the sorted index.
 struct PrimSrc {
     const float* tris;  // soup (null on the drop-in path)
     float4* boxes;      // box array
@@ -127,23 +127,6 @@ __device__ __forceinline__ void streamStore(uint4* p, uint4 v) {
 __device__ __forceinline__ void storeBoxRec(float4* boxes, uint32_t pos, const Box& b, uint32_t obj) {
     streamStore(boxes + 2 * (size_t)pos, make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]));
     streamStore(boxes + 2 * (size_t)pos + 1, make_float4(b.hi[1], b.hi[2], __uint_as_float(obj), 0.0f));
-}
-
-__device__ __forceinline__ Box loadBoxRec(const float4* boxes, uint32_t pos, uint32_t* obj = nullptr) {
-    const float4 a = boxes[2 * (size_t)pos], c = boxes[2 * (size_t)pos + 1];
-    Box r;
-    r.lo[0] = a.x; r.lo[1] = a.y; r.lo[2] = a.z;
-    r.hi[0] = a.w; r.hi[1] = c.x; r.hi[2] = c.y;
-    if (obj) *obj = __float_as_uint(c.z);
-    return r;
-}
-
-template <bool SOUP>
-__device__ __forceinline__ Box loadBox(const PrimSrc& s, uint32_t i) {
-    if constexpr (SOUP)
-        return soupBox(s.tris, i);
-    else
-        return loadBoxRec(s.boxes, i);
 }
 
 __device__ __forceinline__ float waveMin(float v) {
@@ -216,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ tri
                   p2[3] = {v2.x, v2.y, v2.z};
             b = boxOf3(p0, p1, p2);
         }
-        storeBoxRec(boxesOut, i, b, i);
+        if constexpr (!SOUP) storeBoxRec(boxesOut, i, b, i);  // a soup's hierarchy pass gathers the triangle
         cx[i] = (b.lo[0] + b.hi[0]) * 0.5f;
         cy[i] = (b.lo[1] + b.hi[1]) * 0.5f;
         cz[i] = (b.lo[2] + b.hi[2]) * 0.5f;
@@ -775,8 +758,17 @@ __global__ __launch_bounds__(kHierThreads) void k_hier_local(
     float4 ra[kHierPer], rb[kHierPer];
 #pragma unroll
     for (int k = 0; k < kHierPer; ++k) {
-        ra[k] = src.boxes[2 * (size_t)rp[k]];
-        rb[k] = src.boxes[2 * (size_t)rp[k] + 1];
+        if constexpr (SOUP) {
+            // The triangle itself (36 B: one 128-B line, two for the 27 % of
+            // records that straddle one), its box formed as the bounds pass
+            // forms it: the soup path writes no box records.
+            const Box b = soupBox(src.tris, rp[k]);
+            ra[k] = make_float4(b.lo[0], b.lo[1], b.lo[2], b.hi[0]);
+            rb[k] = make_float4(b.hi[1], b.hi[2], __uint_as_float(rp[k]), 0.0f);
+        } else {
+            ra[k] = src.boxes[2 * (size_t)rp[k]];
+            rb[k] = src.boxes[2 * (size_t)rp[k] + 1];
+        }
     }
     uint32_t segS[kHierPer];
 #pragma unroll
@@ -1293,8 +1285,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_bounds(const float* __restrict__
     }
 }
 
-// Box records, scene-tagged 64-bit keys, per-position scene ids and the digit
-// histograms of all passes.
+// Scene-tagged 64-bit keys, per-position scene ids and the digit histograms
+// of all passes.
 __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__ tris, float4* __restrict__ boxes,
                                                        uint32_t n, uint32_t bits, const uint4* __restrict__ chunks,
                                                        const uint32_t* __restrict__ words,
@@ -1318,7 +1310,6 @@ __global__ __launch_bounds__(kBlock) void k_seg_morton(const float* __restrict__
     const uint64_t tag = (uint64_t)ch.x << (3 * bits);
     for (uint32_t i = ch.y + threadIdx.x; i < ch.z; i += kBlock) {
         const Box b = soupBox(tris, i);
-        storeBoxRec(boxes, i, b, i);
         uint32_t q[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {

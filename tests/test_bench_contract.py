@@ -23,7 +23,8 @@ def test_build_byte_model():
     stages = {name: (kernel, bpt, launches) for name, kernel, bpt, launches in bench.STAGES}
     assert stages["hier_local"] == ("k_hier_local", 184.0, 1)
     assert stages["sort"][1:] == (64.0, 4)
-    # k_bounds: soup 36 in, box record 32 + centre 12 out; k_morton: centre 12 in, key 4 out (the
-    # hierarchy pass writes the arrival words of the splits it leaves open)
-    assert stages["bounds"][1] == 36 + 32 + 12
+    # k_bounds: soup 36 in, centre 12 out (the hierarchy pass gathers the triangle for the leaf box);
+    # k_morton: centre 12 in, key 4 out (the hierarchy pass writes the arrival words of the splits it
+    # leaves open)
+    assert stages["bounds"][1] == 36 + 12
     assert stages["morton"][1] == 12 + 4
