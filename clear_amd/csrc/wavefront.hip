@@ -710,19 +710,59 @@ __global__ __launch_bounds__(256) void k_wf_shadow(WfArgs A, uint32_t in) {
 // Hit / miss queues of extension queue `in` from the ordered extend kernel's
 // records.
 __global__ __launch_bounds__(256) void k_wf_compact(WfArgs A, uint32_t in) {
+    // Each block takes one contiguous range of the queue, counts its hits
+    // and misses, claims room in both queues with one atomic each, then
+    // writes its entries in order (block-wide scans). Appends to one counter
+    // serialise in the memory system, so one pair per block instead of one
+    // per block and 256 entries is what this kernel's time is made of.
+    __shared__ uint32_t s_sum[2][4];
+    __shared__ uint32_t s_base[2];
     const WfBufs& B = A.B;
     const uint32_t n = min(B.counts[in], B.slots);
-    const uint32_t lanes = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += lanes) {
-        const uint32_t j = base + threadIdx.x;
-        const bool active = j < n;
-        const bool isHit = active && B.hit[j].x != kNone;
-        const uint32_t kh = appendIndex(B.counts + kCntHit, isHit);
-        const uint32_t km = appendIndex(B.counts + kCntMiss, active && !isHit);
-        if (active) {
-            if (isHit) B.hitQ[kh] = j;
-            else B.missQ[km] = j;
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = min(n, blockIdx.x * per), hi = min(n, lo + per);
+    if (lo >= hi) return;  // the whole block: lo and hi are block-uniform
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t / 64u, waves = blockDim.x / 64u;
+    auto isHitAt = [&](uint32_t j) { return reinterpret_cast<const uint32_t*>(B.hit + j)[0] != kNone; };
+    uint32_t hits = 0;
+    for (uint32_t j = lo + t; j < hi; j += blockDim.x) hits += isHitAt(j) ? 1u : 0u;
+    // Block totals, then one atomic per queue.
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
+    if (lane == 0) s_sum[0][wave] = hits;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t h = 0;
+        for (uint32_t w = 0; w < waves; ++w) h += s_sum[0][w];
+        s_base[0] = h ? atomicAdd(B.counts + kCntHit, h) : 0u;
+        s_base[1] = (hi - lo) - h ? atomicAdd(B.counts + kCntMiss, (hi - lo) - h) : 0u;
+    }
+    __syncthreads();
+    uint32_t baseH = s_base[0], baseM = s_base[1];
+    for (uint32_t j0 = lo; j0 < hi; j0 += blockDim.x) {
+        const uint32_t j = j0 + t;
+        const bool active = j < hi;
+        const bool isHit = active && isHitAt(j);
+        const uint64_t mh = __ballot(isHit), mm = __ballot(active && !isHit);
+        if (lane == 0) {
+            s_sum[0][wave] = (uint32_t)__popcll(mh);
+            s_sum[1][wave] = (uint32_t)__popcll(mm);
         }
+        __syncthreads();
+        uint32_t preH = 0, preM = 0, totH = 0, totM = 0;
+        for (uint32_t w = 0; w < waves; ++w) {
+            if (w < wave) {
+                preH += s_sum[0][w];
+                preM += s_sum[1][w];
+            }
+            totH += s_sum[0][w];
+            totM += s_sum[1][w];
+        }
+        const uint64_t below = (1ull << lane) - 1ull;
+        if (isHit) B.hitQ[baseH + preH + (uint32_t)__popcll(mh & below)] = j;
+        else if (active) B.missQ[baseM + preM + (uint32_t)__popcll(mm & below)] = j;
+        baseH += totH;
+        baseM += totM;
+        __syncthreads();  // s_sum is rewritten by the next chunk
     }
 }
 
