@@ -40,6 +40,7 @@ Secondary legs of the same metric family, each on its stated workload:
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-trace]
                     [--no-render] [--no-adjoint] [--no-batch] [--no-dropin]
                     [--wavefront-config key=value,...]
+                    [--batch-wavefront-config key=value,...]
 """
 from __future__ import annotations
 
@@ -227,6 +228,16 @@ def trace_leg(ca, np, torch, ctx, soup, inodes, leaves, args, dev):
     return out
 
 
+def fixed_wavefront_config(ca, ctx, spec: str):
+    """Sets the wavefront configuration from "key=value,..." and returns it."""
+    cfg = ca.wavefront_config(ctx)
+    for kv in spec.split(","):
+        k, v = kv.split("=")
+        setattr(cfg, k.strip(), int(v))
+    ca.set_wavefront_config(ctx, cfg)
+    return cfg
+
+
 def render_leg(ca, np, torch, ctx, args):
     W, H, spp, depth = 1920, 1080, args.render_spp, 8
     log("building the outdoor scene")
@@ -240,11 +251,7 @@ def render_leg(ca, np, torch, ctx, args):
     # Tuned on 4-spp renders (about 30 ms each at 1080p): 1-spp timings are
     # noisy enough to settle on a different configuration from run to run.
     if args.wavefront_config:  # a fixed configuration (profiling runs), no search
-        cfg = ca.wavefront_config(ctx)
-        for kv in args.wavefront_config.split(","):
-            k, v = kv.split("=")
-            setattr(cfg, k.strip(), int(v))
-        ca.set_wavefront_config(ctx, cfg)
+        cfg = fixed_wavefront_config(ca, ctx, args.wavefront_config)
     else:
         cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune(ctx, ds, 4, depth)
     out["wavefront_config"] = cfg.as_dict()
@@ -392,7 +399,10 @@ def batch_leg(ca, np, torch, ctx, args):
 
     saved = ca.wavefront_config(ctx)
     tune_t = time.perf_counter()
-    cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune_batch(ctx, batch, 4, depth)
+    if args.batch_wavefront_config:  # a fixed configuration (profiling runs), no search
+        cfg = fixed_wavefront_config(ca, ctx, args.batch_wavefront_config)
+    else:
+        cfg = ca.wavefront_config(ctx) if args.no_autotune else ca.wavefront_autotune_batch(ctx, batch, 4, depth)
     out["autotune_s"] = round(time.perf_counter() - tune_t, 2)
     out["wavefront_config"] = cfg.as_dict()
     seed_all()
@@ -554,6 +564,8 @@ def main() -> int:
     ap.add_argument("--wavefront-config", default=None,
                     help="fixed wavefront configuration as key=value pairs (e.g. traceWavesPerCU=24,ldsDepth=16), "
                          "used instead of the autotuner")
+    ap.add_argument("--batch-wavefront-config", default=None,
+                    help="fixed wavefront configuration of the batch leg, as --wavefront-config")
     ap.add_argument("--no-autotune", action="store_true",
                     help="render legs use the default wavefront configuration (for kernel profiles)")
     ap.add_argument("--batch-per-scene", type=int, default=32,
