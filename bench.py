@@ -491,7 +491,7 @@ def cpu_trace(np, built, verts, faces, rays_np):
 
     n = faces.shape[0]
     out = {}
-    # About 8 s of CPU work each (0.03 and 0.5 Mrays/s measured on the GPU box's host).
+    # About 8 s of CPU work each (0.03 and 0.5 Mrays/s measured on a 16-thread MI355X host).
     for label, threads, count in (("one_core", 1, 1 << 18), ("all_cores", host_threads(), rays_np.shape[1])):
         sample = np.ascontiguousarray(rays_np[:, :count])
         chunks = np.array_split(np.arange(count), max(1, threads * 8))
@@ -518,7 +518,7 @@ def cpu_render(ca, np, host, depth):
     W, H = host.width, host.height
     o = orc.lbvh_build(host.vertices, host.faces, 10)
     out = {}
-    # About 8 s of CPU work each (0.09 and 1.4 Msamples/s measured on the GPU box's host).
+    # About 8 s of CPU work each (0.09 and 1.4 Msamples/s measured on a 16-thread MI355X host).
     for label, threads, rows, spp in (("one_core", 1, 10, 40), ("all_cores", host_threads(), 96, 64)):
         st = orc.seed_states(W, H)
         px = np.zeros(W * H * 4, np.float32)
