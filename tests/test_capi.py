@@ -124,3 +124,28 @@ def test_cpp_wrapper_header_compiles_standalone(tmp_path):
                    '    clr::LBVHBuilder b(c, 16);\n    b.perform(nullptr, v, f, 16, 10, i, l);\n}\n')
     inc = Path(__file__).resolve().parent.parent / "include"
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{inc}", str(src)], check=True)
+
+
+def test_pack_by_kind_keeps_every_scene_array():
+    """pack_by_kind uploads one buffer per kind: each scene's view holds
+    exactly its own bytes, views of one kind lie in one buffer in scene order
+    at 256-byte aligned offsets, and in-place writes through a view reach
+    only that scene's bytes."""
+    import torch
+
+    hosts = [ca.HostScene(ca.SCENE_CLUSTERS, 32, 24, seed=100 + i, num_triangles=n)
+             for i, n in enumerate([1000, 37, 2500])]
+    t = ca.pack_by_kind([h.arrays for h in hosts], "cpu")
+    for k in hosts[0].arrays:
+        base = t[0][k].data_ptr()
+        prev_end = base
+        for s, h in enumerate(hosts):
+            ref = h.arrays[k].reshape(-1).view(np.uint8)
+            np.testing.assert_array_equal(t[s][k].numpy(), ref)
+            off = t[s][k].data_ptr() - base
+            assert off % 256 == 0 and t[s][k].data_ptr() >= prev_end
+            prev_end = t[s][k].data_ptr() + ref.nbytes
+    v = t[1]["vertices"].view(torch.float32)
+    v += 1.0
+    np.testing.assert_array_equal(t[0]["vertices"].numpy(), hosts[0].arrays["vertices"].reshape(-1).view(np.uint8))
+    np.testing.assert_array_equal(t[2]["vertices"].numpy(), hosts[2].arrays["vertices"].reshape(-1).view(np.uint8))
