@@ -87,6 +87,16 @@ def test_batch_equals_single_scene_renders(gpu_ctx, scenes):
         assert torch.equal(px[s * W * H * 4:(s + 1) * W * H * 4], mpx)
 
 
+def test_batch_with_large_scenes_equals_single_scene_renders(gpu_ctx):
+    """Scenes of the batch workload's largest sizes (up to 500k triangles)
+    render in a batch exactly as they render alone."""
+    big = [ca.HostScene(ca.SCENE_CLUSTERS, W, H, seed=300 + i, num_triangles=n)
+           for i, n in enumerate([450000, 1200, 320000])]
+    print(f"[batch] scenes of {[h.num_faces for h in big]} faces {W}x{H} spp=2 depth=5", flush=True)
+    batch = ca.SceneBatch(gpu_ctx, big)
+    _check_equal(gpu_ctx, big, batch, 2, 5, "large")
+
+
 def test_batch_chunks_straddle_scenes_and_direct_launches(gpu_ctx, scenes):
     saved = ca.wavefront_config(gpu_ctx)
     try:
