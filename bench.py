@@ -70,6 +70,22 @@ STAGES = [("bounds", "k_bounds", 48.0, 1), ("morton", "k_morton", 16.0, 1),
 # Algorithmic bytes per triangle of the whole build (bounds 36, Morton 40,
 # sort 64, Karras 20, refit 164: the build's contract model, 8-bit digits).
 BUILD_BYTES_PER_TRI = 324.0
+
+
+def lbvh_model(bits: int):
+    """(stages, whole-build bytes per triangle) for `bits` per axis: the model
+    above, with the sort's ceil(3 bits / 8) passes of 8-bit digits. Up to 10
+    bits the keys are 32-bit (16 B per pass: 4-B keys and payloads, read and
+    written). Above, keys are 64-bit (24 B per pass), and the Morton pass
+    writes, and the hierarchy reads, 8-B keys (4 B more each)."""
+    passes = (3 * bits + 7) // 8
+    wide = bits > 10
+    per_pass = 24.0 if wide else 16.0
+    extra = 4.0 if wide else 0.0
+    stages = [("bounds", "k_bounds", 48.0, 1), ("morton", "k_morton", 16.0 + extra, 1),
+              ("sort", "k_onesweep", per_pass * passes, passes),
+              ("hier_local", "k_hier_local", 184.0 + extra, 1), ("hier_global", "k_hier_global", 0.0, 1)]
+    return stages, BUILD_BYTES_PER_TRI - 64.0 + per_pass * passes + 2.0 * extra
 # Traversal work model per internal-node visit (V) and triangle test (T), and
 # per visit of a 4-wide record (W: four slab tests, 4 x 24 B of boxes and
 # 4 x 4 B of references).
@@ -132,23 +148,26 @@ def lbvh_leg(ca, np, torch, args, dev):
     ctx.set_profiling(False)
     assert ctx.device_status() == 0
     stage_info = {}
-    for (name, kernel, bpt, launches), ms in zip(STAGES, stages_ms):
+    stages, build_bpt = lbvh_model(bits)
+    for (name, kernel, bpt, launches), ms in zip(stages, stages_ms):
         e = {"kernel": kernel, "ms": round(ms, 4), "launches": launches, "bytes_per_tri": bpt}
         if bpt:
             gbs = bpt * n / (ms * 1e-3) / 1e9
             e.update({"GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)})
         stage_info[name] = e
     loc = stage_info["hier_local"]
-    achieved = 184.0 * n / (loc["ms"] * 1e-3) / 1e9
+    loc_bpt = loc["bytes_per_tri"]
+    achieved = loc_bpt * n / (loc["ms"] * 1e-3) / 1e9
     tr = load_traffic()
-    traffic = tr.get("hbm_bytes_per_launch") if tr and tr.get("tris") == n else None
+    # The committed counter profile is of the 10-bit build.
+    traffic = tr.get("hbm_bytes_per_launch") if tr and tr.get("tris") == n and bits == 10 else None
     roofline = {"bound": "hbm", "kernel": "k_hier_local", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes_per_launch": 184.0 * n,
+                "traffic": traffic, "algorithmic_bytes_per_launch": loc_bpt * n,
                 "ms_per_launch": loc["ms"]}
-    whole = BUILD_BYTES_PER_TRI * n / (dt / K) / 1e9
+    whole = build_bpt * n / (dt / K) / 1e9
     roofline["whole_build"] = {"achieved": round(whole, 1), "frac": round(whole / HBM_PEAK_GBS, 4),
-                               "algorithmic_bytes_per_tri": BUILD_BYTES_PER_TRI}
+                               "algorithmic_bytes_per_tri": build_bpt}
     return ctx, soup, tsoup, inodes, leaves, {
         "mtri_s": n * K / dt / 1e6, "ms_per_step": dt / K * 1e3, "stages": stage_info,
         "roofline": roofline, "build_effective_GB_s": round(whole, 1)}
@@ -180,7 +199,7 @@ def dropin_leg(ca, np, torch, ctx, soup, ref_inodes, ref_leaves, args, dev):
     same = torch.equal(inodes, ref_inodes) and torch.equal(leaves, ref_leaves)
     assert same, "the drop-in build's nodes differ from the soup build's"
     assert ctx.device_status() == 0
-    bpt = BUILD_BYTES_PER_TRI - 36.0 + 104.0
+    bpt = lbvh_model(bits)[1] - 36.0 + 104.0
     out = {"workload": f"clr_lbvh_build of the same {n} triangles as clr_float3 vertices + Face records",
            "ms": round(ms, 4), "Mtri_s": round(n / (ms * 1e-3) / 1e6, 1), "algorithmic_bytes_per_tri": bpt,
            "frac_hbm": round(bpt * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "nodes_equal_soup_build": same}
@@ -625,7 +644,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"LBVH build of a {args.tris}-triangle Gaussian-cluster soup "
                                "(64 clusters, sigma 0.02, edges 1e-3..1e-2, seed 1), "
-                               "30-bit Morton (10 bits/axis), 48-B node output",
+                               f"{3 * args.bits}-bit Morton ({args.bits} bits/axis), 48-B node output",
                    "tris": args.tris, "bits_per_dim": args.bits, "parallelism": "single GPU"},
         "roofline": lb["roofline"],
         "cpu_baseline": cpu,

@@ -28,3 +28,24 @@ def test_build_byte_model():
     # leaves open)
     assert stages["bounds"][1] == 36 + 12
     assert stages["morton"][1] == 12 + 4
+
+
+def test_build_byte_model_follows_the_key_width():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    # 10 bits per axis: the model above, four 8-bit passes over 32-bit keys
+    stages, whole = bench.lbvh_model(10)
+    assert stages == bench.STAGES and whole == bench.BUILD_BYTES_PER_TRI
+    # 20 bits per axis (the reference's default): 60-bit keys in 64-bit words,
+    # eight passes of 8-B keys and 4-B payloads read and written, and 4 B
+    # more in the Morton pass's key write and the hierarchy's key reads
+    stages, whole = bench.lbvh_model(20)
+    s = {name: (bpt, launches) for name, _, bpt, launches in stages}
+    assert s["sort"] == (8 * 2 * (8 + 4), 8)
+    assert s["morton"][0] == 12 + 8 and s["hier_local"][0] == 184 + 4
+    assert whole == bench.BUILD_BYTES_PER_TRI - 64 + 8 * 24 + 8
+    # 5 bits per axis: two passes of 32-bit keys
+    stages, whole = bench.lbvh_model(5)
+    assert {name: launches for name, _, _, launches in stages}["sort"] == 2
+    assert whole == bench.BUILD_BYTES_PER_TRI - 64 + 2 * 16
