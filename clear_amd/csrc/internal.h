@@ -92,6 +92,7 @@ int ensureStatus(clr_ctx* ctx);
 // Indices into clr_ctx::devStatus.
 constexpr uint32_t kWordStatus = 0;
 constexpr uint32_t kWordEpoch = 1;
+constexpr uint32_t kWordMegaTiles = 2;  // next 8x8 tile of a megakernel render
 
 // Bump allocator over the pool (256-B aligned sub-buffers).
 // Makes the context's device current for the duration of a C-ABI call and

@@ -353,21 +353,19 @@ enum : int { kWalkLbvh = 0, kWalkOrdered = 1, kWalkMidpoint = 2 };
 
 // rayIntersection with hit parameters, by WALK. Returns false (face = kNone)
 // on a miss.
-template <int WALK>
-__device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, uint32_t* stack, int stride,
-                                          SurfPt& sp) {
+template <int WALK, class Stack>
+__device__ __forceinline__ bool intersect(const RScene& S, V3 org, V3 dir, const Stack& stack, SurfPt& sp) {
     float t = INFINITY, b1 = 0.0f, b2 = 0.0f;
     TraceStats st{0u, 0u};
     bool over = false;
     uint32_t face;
     if constexpr (WALK == kWalkOrdered)  // over the 4-wide records (every render builds them)
-        face = wideHit<false, false, TexAlpha, LdsStack, kStackDepth>(S.g, org, dir, t, LdsStack{stack, stride}, st,
-                                                                      &b1, &b2, TexAlpha{&S}, &over);
+        face = wideHit<false, false, TexAlpha, Stack, kStackDepth>(S.g, org, dir, t, stack, st, &b1, &b2,
+                                                                   TexAlpha{&S}, &over);
     else if constexpr (WALK == kWalkMidpoint)
-        face = bvhWalk<false, false>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2, TexAlpha{&S}, &over);
+        face = bvhWalk<false, false>(S.g, org, dir, t, stack, st, &b1, &b2, TexAlpha{&S}, &over);
     else
-        face = lbvhClosest<false, TexAlpha>(S.g, org, dir, t, LdsStack{stack, stride}, st, &b1, &b2,
-                                            TexAlpha{&S}, &over);
+        face = lbvhClosest<false, TexAlpha>(S.g, org, dir, t, stack, st, &b1, &b2, TexAlpha{&S}, &over);
     if (over) atomicOr(S.status, kStatusStackOverflow);
     if (face == kNone || isinf(t)) {
         sp.face = kNone;
