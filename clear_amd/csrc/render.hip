@@ -277,6 +277,13 @@ constexpr int kTile = 8;  // 8x8 pixels per wave
 // LDS stack (10 waves per CU).
 constexpr int kMegaLds = 32;
 constexpr uint32_t kMegaWavesPerCU = 20;  // persistent grid
+// The adjoint's waves also hold 4 KiB of LDS gradient sums, so its stack
+// keeps 16 words in LDS (8 KiB per wave with the sums) and its register
+// budget is set for four waves per SIMD: 512x512, 32 spp, depth 5 of the
+// differentiation scene, 99 ms at 8 waves per CU (20 KiB of LDS each),
+// 100 ms at 13 (a 32-word stack), 68.6 ms at 16.
+constexpr uint32_t kAdjWavesPerCU = 16;
+constexpr int kAdjLds = 16;
 
 __global__ void k_mega_reset(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0u;
@@ -380,17 +387,17 @@ __global__ __launch_bounds__(64) void k_bvh_visualizer(dev::RScene scene, uint32
 }
 
 // Adjoint of k_render_mega: the same pixels, samples and random states, each
-// sample replayed with the adjoint hook; block sums of the gradients go to
-// global memory once per block.
+// sample replayed with the adjoint hook. Persistent waves take tiles from the
+// same kind of counter, with a split stack, and sum their gradients in LDS
+// over all their tiles; the sums go to global memory once per wave.
 template <int WALK>
-__global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32_t width, uint32_t height,
-                                                       uint32_t spp, uint32_t maxDepth, uint32_t flags,
-                                                       const uint32_t* __restrict__ states,
-                                                       const float4* __restrict__ dLdI,
-                                                       const uint16_t* __restrict__ table, uint32_t numMat,
-                                                       uint32_t lightProp, float* __restrict__ gAlb,
-                                                       float* __restrict__ gSig, float* __restrict__ gEm) {
-    __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint(
+    dev::RScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
+    const uint32_t* __restrict__ states, const float4* __restrict__ dLdI, const uint16_t* __restrict__ table,
+    uint32_t numMat, uint32_t lightProp, float* __restrict__ gAlb, float* __restrict__ gSig, float* __restrict__ gEm,
+    uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
+    constexpr int kLds = WALK == dev::kWalkOrdered ? kAdjLds : dev::kStackDepth;
+    __shared__ uint32_t s_stack[kLds * 64];
     __shared__ dev::AdjAcc s_acc;
     for (uint32_t i = threadIdx.x; i < dev::kMaxDiffMaterials * 3; i += 64) s_acc.alb[i] = 0.0f;
     for (uint32_t i = threadIdx.x; i < dev::kMaxDiffMaterials; i += 64) s_acc.sig[i] = 0.0f;
@@ -398,25 +405,36 @@ __global__ __launch_bounds__(64) void k_render_adjoint(dev::RScene scene, uint32
     __syncthreads();
     dev::RScene S = scene;
     dev::finishScene(S);
+    dev::V3 M = dev::v3z();
+    if (lightProp != 0xFFFFFFFFu) {
+        const uint8_t* e = S.mat + ((lightProp + 1u + 3u) & ~3u);
+        M = dev::evalColor(S.tex + dev::ld<uint32_t>(e + 4), 0.0f, 0.0f);
+    }
     const uint32_t tilesX = (width + kTile - 1) / kTile;
-    const uint32_t gx = (blockIdx.x % tilesX) * kTile + (threadIdx.x % kTile);
-    const uint32_t gy = (blockIdx.x / tilesX) * kTile + (threadIdx.x / kTile);
-    if (gx < width && gy < height) {
+    const uint32_t tiles = tilesX * ((height + kTile - 1) / kTile);
+    const uint32_t lanes = gridDim.x * 64u;
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    auto nextTile = [&]() {
+        uint32_t t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(tileCounter, 1u);
+        return (uint32_t)__shfl((int)t, 0);
+    };
+    const float inv = 1.0f / (float)spp;
+    for (uint32_t tile = nextTile(); tile < tiles; tile = nextTile()) {
+        const uint32_t gx = (tile % tilesX) * kTile + (threadIdx.x % kTile);
+        const uint32_t gy = (tile / tilesX) * kTile + (threadIdx.x / kTile);
+        if (gx >= width || gy >= height) continue;
         const size_t pi = (size_t)gy * width + gx;
         const uint4 st = reinterpret_cast<const uint4*>(states)[pi];
         dev::Rng rng{st.x, st.y, st.z, st.w};
-        dev::V3 M = dev::v3z();
-        if (lightProp != 0xFFFFFFFFu) {
-            const uint8_t* e = S.mat + ((lightProp + 1u + 3u) & ~3u);
-            M = dev::evalColor(S.tex + dev::ld<uint32_t>(e + 4), 0.0f, 0.0f);
-        }
         const float4 a = dLdI[pi];
-        const float inv = 1.0f / (float)spp;
         dev::AdjHook hook{S, table, lightProp, M, {a.x * inv, a.y * inv, a.z * inv}, &s_acc, 0};
-        for (uint32_t k = 0; k < spp; ++k) {
-            const dev::V3 L =
-                dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, dev::LdsStack{s_stack + threadIdx.x, 64}, hook);
-            hook.finish(L);
+        if constexpr (WALK == dev::kWalkOrdered) {
+            const dev::SpillStack<kLds> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
+            for (uint32_t k = 0; k < spp; ++k) hook.finish(dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook));
+        } else {
+            const dev::LdsStack stk{s_stack + threadIdx.x, 64};
+            for (uint32_t k = 0; k < spp; ++k) hook.finish(dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook));
         }
     }
     __syncthreads();
@@ -599,18 +617,26 @@ int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params
     const uint32_t tiles = ((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile);
     const uint32_t flags = rp.flags;
     const float4* adj = reinterpret_cast<const float4*>(dLdI);
+    int cus = 0;
+    rc = hipCheck(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
+    if (rc) return rc;
+    const uint32_t grid = std::max(1u, std::min(tiles, (uint32_t)cus * kAdjWavesPerCU));
+    rc = ensureTraceSpill(ctx, (size_t)grid * 64 * (dev::kStackDepth - kAdjLds) * sizeof(uint32_t));
+    if (rc) return rc;
+    uint32_t* counter = ctx->devStatus + kWordMegaTiles;
+    k_mega_reset<<<1, 64, 0, stream>>>(counter);
     if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
-        k_render_adjoint<dev::kWalkOrdered><<<tiles, 64, 0, stream>>>(
+        k_render_adjoint<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(
             S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
-            dp.lightProp, gAlb, gSig, gEm);
+            dp.lightProp, gAlb, gSig, gEm, counter, ctx->traceSpill);
     else if (sd.bvhNodes)
-        k_render_adjoint<dev::kWalkMidpoint><<<tiles, 64, 0, stream>>>(
+        k_render_adjoint<dev::kWalkMidpoint><<<grid, 64, 0, stream>>>(
             S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
-            dp.lightProp, gAlb, gSig, gEm);
+            dp.lightProp, gAlb, gSig, gEm, counter, ctx->traceSpill);
     else
-        k_render_adjoint<dev::kWalkLbvh><<<tiles, 64, 0, stream>>>(
+        k_render_adjoint<dev::kWalkLbvh><<<grid, 64, 0, stream>>>(
             S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, adj, table, dp.numMaterials,
-            dp.lightProp, gAlb, gSig, gEm);
+            dp.lightProp, gAlb, gSig, gEm, counter, ctx->traceSpill);
     return hipCheck(ctx, hipGetLastError(), "adjoint launch");
 }
 
