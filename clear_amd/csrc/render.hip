@@ -292,13 +292,13 @@ __global__ void k_mega_reset(uint32_t* counter) {
 // Persistent waves: each takes the next 8x8 tile from a device counter
 // (zeroed by k_mega_reset on the same stream) until the image is done, so
 // the spill region is sized by the grid, not by the image.
-template <int WALK>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega(
-    dev::RScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
-    uint32_t* __restrict__ states, clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
-    uint32_t* __restrict__ spill) {
-    constexpr int kLds = WALK == dev::kWalkOrdered ? kMegaLds : dev::kStackDepth;
-    __shared__ uint32_t s_stack[kLds * 64];
+// LDS: stack words per lane kept in LDS (the ordered walk spills the rest;
+// the reference walks need all kStackDepth).
+template <int WALK, int LDS>
+__device__ __forceinline__ void renderTiles(const dev::RScene& scene, uint32_t width, uint32_t height, uint32_t spp,
+                                            uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states,
+                                            clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
+                                            uint32_t* __restrict__ spill, uint32_t* s_stack) {
     dev::RScene S = scene;  // resolved copy; the kernel argument itself stays read-only
     dev::finishScene(S);
     const uint32_t tilesX = (width + kTile - 1) / kTile;
@@ -319,8 +319,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         dev::Rng rng{st.x, st.y, st.z, st.w};
         dev::V3 acc = dev::v3z();
         dev::NoHook hook;
-        if constexpr (WALK == dev::kWalkOrdered) {
-            const dev::SpillStack<kLds> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
+        if constexpr (LDS < dev::kStackDepth) {
+            const dev::SpillStack<LDS> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
             for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
         } else {
             const dev::LdsStack stk{s_stack + threadIdx.x, 64};
@@ -333,6 +333,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         p.z += acc.z;
         reinterpret_cast<float4*>(pixels)[pi] = p;
     }
+}
+
+// Large images: five waves per SIMD.
+template <int WALK>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega(
+    dev::RScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
+    uint32_t* __restrict__ states, clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
+    uint32_t* __restrict__ spill) {
+    constexpr int kLds = WALK == dev::kWalkOrdered ? kMegaLds : dev::kStackDepth;
+    __shared__ uint32_t s_stack[kLds * 64];
+    renderTiles<WALK, kLds>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill, s_stack);
+}
+
+// Images of at most kMegaFewTiles tiles per CU cannot fill many waves per
+// CU, so their waves keep the whole stack in LDS and all the registers they
+// want (no spills).
+constexpr uint32_t kMegaFewTiles = 4;
+template <int WALK>
+__global__ __launch_bounds__(64) void k_render_mega_few(dev::RScene scene, uint32_t width, uint32_t height,
+                                                        uint32_t spp, uint32_t maxDepth, uint32_t flags,
+                                                        uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
+                                                        uint32_t* __restrict__ tileCounter,
+                                                        uint32_t* __restrict__ spill) {
+    __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+    renderTiles<WALK, dev::kStackDepth>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill,
+                                        s_stack);
 }
 
 // bvh_visualizer (bvh_visualizer.cl:25-84): one ray through each pixel
@@ -561,11 +587,16 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     if (rc) return rc;
     uint32_t* counter = ctx->devStatus + kWordMegaTiles;
     k_mega_reset<<<1, 64, 0, stream>>>(counter);
-    if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL))
-        k_render_mega<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                  rp.maxDepth, flags, states, pixels, counter,
-                                                                  ctx->traceSpill);
-    else if (sd.bvhNodes)
+    if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)) {
+        if (tiles <= (uint32_t)cus * kMegaFewTiles)
+            k_render_mega_few<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
+                                                                          rp.maxDepth, flags, states, pixels, counter,
+                                                                          ctx->traceSpill);
+        else
+            k_render_mega<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
+                                                                      rp.maxDepth, flags, states, pixels, counter,
+                                                                      ctx->traceSpill);
+    } else if (sd.bvhNodes)
         k_render_mega<dev::kWalkMidpoint><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
                                                                    rp.maxDepth, flags, states, pixels, counter,
                                                                    ctx->traceSpill);
