@@ -30,8 +30,9 @@ Secondary legs of the same metric family, each on its stated workload:
            traversal counts per camera sample (every 16th row, 2 spp);
   adjoint  the 200,000-triangle, 64-material differentiation scene, 512x512,
            32 spp, depth 5, Russian roulette off: forward and adjoint time;
-  batch    256 cluster scenes: one segmented build vs 256 builds, batched vs
-           scene-by-scene rendering;
+  batch    256 cluster scenes: one segmented build vs 256 builds, batched
+           (wavefront, and megakernel waves sharing one tile counter; the two
+           images must be equal bit for bit) vs scene-by-scene rendering;
   cpu      the CPU oracle on the host: LBVH build of the same soup on 1 thread
            (cpu_baseline) and on all of the GPU's host threads, the reference
            closest-hit walk on 1 thread and on all threads, and the renderer
@@ -367,7 +368,8 @@ def batch_leg(ca, np, torch, ctx, args):
     """256 cluster-soup scenes (log-uniform 1k..500k triangles, seeds
     100..355): one segmented LBVH build against 256 single-scene builds, then
     one batched wavefront render (128x128, 32 spp, depth 5, launch shapes
-    autotuned on the batch) against rendering the scenes one by one."""
+    autotuned on the batch) and one batched megakernel render against
+    rendering the scenes one by one."""
     n_scenes, W, H, spp, depth = args.batch_scenes, 128, 128, 32, 5
     rng = np.random.default_rng(3)
     counts = np.exp(rng.uniform(np.log(1000), np.log(500000), n_scenes)).astype(np.int64)
@@ -429,6 +431,19 @@ def batch_leg(ca, np, torch, ctx, args):
     samples = n_scenes * npix * spp
     out["batched_wavefront"] = {"ms": round(ms, 2), "Msamples_s": round(samples / ms / 1e3, 1)}
     log(f"batched render: {out['batched_wavefront']}")
+    # The megakernel form of the batch (clr_render_batch_mode): persistent
+    # waves take 8x8 tiles of all scenes from one counter. Same calls from the
+    # same seeds, so its states and image must equal the wavefront's.
+    wave_px, wave_st = pixels.clone(), states.clone()
+    seed_all()
+    pixels.zero_()
+    ms = timed(lambda: ca.render_batch(ctx, batch, spp, depth, states, pixels, mode=ca.RENDER_MEGAKERNEL), 1)
+    out["batched_megakernel"] = {"ms": round(ms, 2), "Msamples_s": round(samples / ms / 1e3, 1),
+                                 "image_equals_batched_wavefront": bool(torch.equal(pixels, wave_px)
+                                                                        and torch.equal(states, wave_st))}
+    log(f"batched megakernel render: {out['batched_megakernel']}")
+    assert out["batched_megakernel"]["image_equals_batched_wavefront"], "batched renderers disagree"
+    del wave_px, wave_st
     # Scene-by-scene rendering is timed on every k-th scene (k = scenes /
     # batch_per_scene) and reported as a rate over those scenes' samples.
     step = max(1, n_scenes // max(1, args.batch_per_scene))
@@ -456,6 +471,8 @@ def batch_leg(ca, np, torch, ctx, args):
     if args.batch_per_scene:
         out["batched_vs_per_scene_megakernel"] = round(
             out["batched_wavefront"]["Msamples_s"] / out["per_scene_megakernel"]["Msamples_s"], 2)
+        out["batched_megakernel_vs_per_scene_megakernel"] = round(
+            out["batched_megakernel"]["Msamples_s"] / out["per_scene_megakernel"]["Msamples_s"], 2)
         out["batched_vs_per_scene_plain_megakernel"] = round(
             out["batched_wavefront"]["Msamples_s"] / out["per_scene_megakernel_plain"]["Msamples_s"], 2)
     del batch, hosts

@@ -76,6 +76,7 @@ def lib():
         "clr_seed_pixel_states": (i32, [vp, u32, u32, vp, vp]),
         "clr_render_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "clr_render_batch": (i32, [vp, vp, u32, vp, vp, vp, vp]),
+    "clr_render_batch_mode": (i32, [vp, vp, u32, vp, vp, vp, i32, vp]),
         "clr_bvh_visualizer": (i32, [vp, vp, vp, vp, vp]),
         "clr_post_clear": (i32, [vp, u32, u32, vp, vp]),
         "clr_post_scaling": (i32, [vp, u32, u32, u32, vp, vp, vp]),
@@ -586,12 +587,18 @@ def wavefront_autotune_batch(ctx: Context, batch: SceneBatch, spp: int, max_dept
 
 
 def render_batch(ctx: Context, batch: SceneBatch, spp: int, max_depth: int, states, pixels,
-                 stream=None, flags: int = 0) -> None:
+                 stream=None, flags: int = 0, mode: int | None = None) -> None:
     """clr_render_batch: every scene of `batch` in one wavefront pass; states
-    and pixels hold the scenes' images back to back."""
+    and pixels hold the scenes' images back to back. With `mode`
+    (RENDER_WAVEFRONT or RENDER_MEGAKERNEL) clr_render_batch_mode, whose
+    megakernel form takes 8x8 tiles of all scenes from one counter."""
     rp = RenderParams(batch.width, batch.height, spp, max_depth, flags)
-    ctx.check(lib().clr_render_batch(ctx.handle, batch.devs, len(batch), ctypes.byref(rp), _ptr(states),
-                                     _ptr(pixels), _stream(stream)))
+    if mode is None:
+        ctx.check(lib().clr_render_batch(ctx.handle, batch.devs, len(batch), ctypes.byref(rp), _ptr(states),
+                                         _ptr(pixels), _stream(stream)))
+    else:
+        ctx.check(lib().clr_render_batch_mode(ctx.handle, batch.devs, len(batch), ctypes.byref(rp), _ptr(states),
+                                              _ptr(pixels), int(mode), _stream(stream)))
 
 
 def seed_pixel_states(ctx: Context, width: int, height: int, states, stream=None) -> None:

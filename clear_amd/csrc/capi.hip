@@ -478,7 +478,15 @@ int clr_bvh_visualizer(clr_ctx* ctx, const clr_scene_dev* scene, const clr_rende
 int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
                      const clr_render_params* params, uint32_t* randStates, clr_float3* pixels,
                      void* stream) {
+    return clr_render_batch_mode(ctx, scenes, numScenes, params, randStates, pixels, CLR_RENDER_WAVEFRONT, stream);
+}
+
+int clr_render_batch_mode(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                          const clr_render_params* params, uint32_t* randStates, clr_float3* pixels, int mode,
+                          void* stream) {
     if (!ctx) return CLR_ERR_INVALID_ARG;
+    if (mode != CLR_RENDER_MEGAKERNEL && mode != CLR_RENDER_WAVEFRONT)
+        return fail(ctx, CLR_ERR_INVALID_ARG, "unknown render mode");
     if (!scenes || !randStates || !pixels || !params) return fail(ctx, CLR_ERR_INVALID_ARG, "null render argument");
     if (numScenes == 0) return CLR_OK;
     for (uint32_t s = 0; s < numScenes; ++s) {
@@ -487,7 +495,8 @@ int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScen
     }
     if (!aligned16(randStates) || !aligned16(pixels))
         return fail(ctx, CLR_ERR_INVALID_ARG, "buffers must be 16-byte aligned");
-    if (params->maxDepth > 255) return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
+    if (mode == CLR_RENDER_WAVEFRONT && params->maxDepth > 255)
+        return fail(ctx, CLR_ERR_UNSUPPORTED, "wavefront renderer supports maxDepth <= 255");
     if (params->flags & CLR_RENDER_REFERENCE_TRAVERSAL)
         return fail(ctx, CLR_ERR_UNSUPPORTED, "batched render uses the ordered traversal only");
     const uint64_t tiles = (uint64_t)((params->width + 7) / 8) * ((params->height + 7) / 8) * 64;
@@ -496,6 +505,9 @@ int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScen
     if (params->samplesPerPixel == 0) return CLR_OK;
     DeviceScope ds(ctx->device);
     if (ds.err != hipSuccess) return hipCheck(ctx, ds.err, "set device");
+    if (mode == CLR_RENDER_MEGAKERNEL)
+        return renderMegakernelBatch(ctx, scenes, numScenes, *params, randStates, pixels,
+                                     static_cast<hipStream_t>(stream));
     return renderWavefrontBatch(ctx, scenes, numScenes, *params, randStates, pixels, static_cast<hipStream_t>(stream));
 }
 

@@ -10,6 +10,9 @@
 
 namespace clr {
 struct WfState;
+namespace dev {
+struct RScene;
+}
 }
 
 struct clr_ctx {
@@ -155,6 +158,16 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
                      uint32_t* states, clr_float3* pixels, hipStream_t stream);
 int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp,
                     uint32_t* states, clr_float3* pixels, hipStream_t stream);
+// Batches (clr_render_batch, clr_render_batch_mode): the shared set-up and
+// the two renderers.
+int prepareBatchScenes(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes, hipStream_t stream,
+                       dev::RScene** tableOut, dev::RScene* firstOut);
+int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
+int renderMegakernelBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                          const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
+// Whether `stream` is being captured into a graph (by the caller).
+int streamCapturing(clr_ctx* ctx, hipStream_t stream, bool& capturing);
 int seedPixelStates(clr_ctx* ctx, uint32_t width, uint32_t height, uint32_t* states,
                     hipStream_t stream);
 int renderAdjoint(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, const clr_diff_params& dp,
@@ -186,8 +199,6 @@ void launchAccelWide(const clr_lbvh_inode* inodes, const clr_lbvh_leaf* leaves, 
 // Face -> first light map and first environment light of one scene.
 void launchFaceLights(const clr_light_info* lights, uint32_t numLights, uint32_t numFaces, uint32_t* faceLight,
                       int* envLight, hipStream_t stream);
-int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
-                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream);
 // Builds the ordered walk's traversal layout from the drop-in nodes on `stream`
 // into context memory and returns its two arrays, and with `wnodes` the
 // renderers' 4-wide records too.

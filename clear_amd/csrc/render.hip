@@ -289,6 +289,45 @@ __global__ void k_mega_reset(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0u;
 }
 
+// One 8x8 tile (tile index `tile` of an image tilesX tiles wide), all spp
+// samples of its pixels; states and pixels are the image's own arrays.
+template <int WALK, int LDS>
+__device__ __forceinline__ void renderTile(const dev::RScene& S, uint32_t tile, uint32_t tilesX, uint32_t width,
+                                           uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
+                                           uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
+                                           uint32_t* __restrict__ spill, uint32_t* s_stack) {
+    const uint32_t lanes = gridDim.x * 64u;
+    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t gx = (tile % tilesX) * kTile + (threadIdx.x % kTile);
+    const uint32_t gy = (tile / tilesX) * kTile + (threadIdx.x / kTile);
+    if (gx >= width || gy >= height) return;
+    const size_t pi = (size_t)gy * width + gx;
+    uint4 st = reinterpret_cast<uint4*>(states)[pi];
+    dev::Rng rng{st.x, st.y, st.z, st.w};
+    dev::V3 acc = dev::v3z();
+    dev::NoHook hook;
+    if constexpr (LDS < dev::kStackDepth) {
+        const dev::SpillStack<LDS> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
+        for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
+    } else {
+        const dev::LdsStack stk{s_stack + threadIdx.x, 64};
+        for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
+    }
+    reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+    float4 p = reinterpret_cast<float4*>(pixels)[pi];
+    p.x += acc.x;
+    p.y += acc.y;
+    p.z += acc.z;
+    reinterpret_cast<float4*>(pixels)[pi] = p;
+}
+
+// The next tile of a persistent wave: lane 0 takes it from the counter.
+__device__ __forceinline__ uint32_t nextTile(uint32_t* tileCounter) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(tileCounter, 1u);
+    return __builtin_amdgcn_readfirstlane(__shfl((int)t, 0));
+}
+
 // Persistent waves: each takes the next 8x8 tile from a device counter
 // (zeroed by k_mega_reset on the same stream) until the image is done, so
 // the spill region is sized by the grid, not by the image.
@@ -303,35 +342,30 @@ __device__ __forceinline__ void renderTiles(const dev::RScene& scene, uint32_t w
     dev::finishScene(S);
     const uint32_t tilesX = (width + kTile - 1) / kTile;
     const uint32_t tiles = tilesX * ((height + kTile - 1) / kTile);
-    const uint32_t lanes = gridDim.x * 64u;
-    const uint32_t lane = blockIdx.x * 64u + threadIdx.x;
-    auto nextTile = [&]() {
-        uint32_t t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(tileCounter, 1u);
-        return (uint32_t)__shfl((int)t, 0);
-    };
-    for (uint32_t tile = nextTile(); tile < tiles; tile = nextTile()) {
-        const uint32_t gx = (tile % tilesX) * kTile + (threadIdx.x % kTile);
-        const uint32_t gy = (tile / tilesX) * kTile + (threadIdx.x / kTile);
-        if (gx >= width || gy >= height) continue;
-        const size_t pi = (size_t)gy * width + gx;
-        uint4 st = reinterpret_cast<uint4*>(states)[pi];
-        dev::Rng rng{st.x, st.y, st.z, st.w};
-        dev::V3 acc = dev::v3z();
-        dev::NoHook hook;
-        if constexpr (LDS < dev::kStackDepth) {
-            const dev::SpillStack<LDS> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
-            for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
-        } else {
-            const dev::LdsStack stk{s_stack + threadIdx.x, 64};
-            for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
-        }
-        reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
-        float4 p = reinterpret_cast<float4*>(pixels)[pi];
-        p.x += acc.x;
-        p.y += acc.y;
-        p.z += acc.z;
-        reinterpret_cast<float4*>(pixels)[pi] = p;
+    for (uint32_t tile = nextTile(tileCounter); tile < tiles; tile = nextTile(tileCounter))
+        renderTile<WALK, LDS>(S, tile, tilesX, width, height, spp, maxDepth, flags, states, pixels, spill, s_stack);
+}
+
+// A batch of equally sized images: the tiles of all scenes share one
+// counter, tile t being tile t % tilesPerScene of scene t / tilesPerScene,
+// whose states and pixels start at scene * width * height. Consecutive tiles
+// stay in one scene, so the waves resident at a time walk a few scenes'
+// arrays. The table's entries have their blob headers resolved.
+template <int WALK, int LDS>
+__device__ __forceinline__ void renderBatchTiles(const dev::RScene* __restrict__ table, uint32_t numScenes,
+                                                 uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth,
+                                                 uint32_t flags, uint32_t* __restrict__ states,
+                                                 clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
+                                                 uint32_t* __restrict__ spill, uint32_t* s_stack) {
+    const uint32_t tilesX = (width + kTile - 1) / kTile;
+    const uint32_t tilesPerScene = tilesX * ((height + kTile - 1) / kTile);
+    const uint32_t tiles = tilesPerScene * numScenes;
+    const size_t npix = (size_t)width * height;
+    for (uint32_t tile = nextTile(tileCounter); tile < tiles; tile = nextTile(tileCounter)) {
+        const uint32_t s = tile / tilesPerScene;
+        const dev::RScene S = table[s];
+        renderTile<WALK, LDS>(S, tile - s * tilesPerScene, tilesX, width, height, spp, maxDepth, flags,
+                              states + 4 * npix * s, pixels + npix * s, spill, s_stack);
     }
 }
 
@@ -359,6 +393,24 @@ __global__ __launch_bounds__(64) void k_render_mega_few(dev::RScene scene, uint3
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
     renderTiles<WALK, dev::kStackDepth>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill,
                                         s_stack);
+}
+
+// Batches: the same two shapes.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega_batch(
+    const dev::RScene* __restrict__ table, uint32_t numScenes, uint32_t width, uint32_t height, uint32_t spp,
+    uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
+    uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t s_stack[kMegaLds * 64];
+    renderBatchTiles<dev::kWalkOrdered, kMegaLds>(table, numScenes, width, height, spp, maxDepth, flags, states,
+                                                  pixels, tileCounter, spill, s_stack);
+}
+__global__ __launch_bounds__(64) void k_render_mega_batch_few(
+    const dev::RScene* __restrict__ table, uint32_t numScenes, uint32_t width, uint32_t height, uint32_t spp,
+    uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
+    uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t s_stack[dev::kStackDepth * 64];
+    renderBatchTiles<dev::kWalkOrdered, dev::kStackDepth>(table, numScenes, width, height, spp, maxDepth, flags,
+                                                          states, pixels, tileCounter, spill, s_stack);
 }
 
 // bvh_visualizer (bvh_visualizer.cl:25-84): one ray through each pixel
@@ -605,6 +657,40 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
                                                                rp.maxDepth, flags, states, pixels, counter,
                                                                ctx->traceSpill);
     return hipCheck(ctx, hipGetLastError(), "render launch");
+}
+
+int renderMegakernelBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                          const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream) {
+    int cus = 0;
+    int rc = hipCheck(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
+    if (rc) return rc;
+    const uint64_t tiles = (uint64_t)((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile) * numScenes;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)cus * kMegaWavesPerCU));
+    const size_t spillBytes = (size_t)grid * 64 * (dev::kStackDepth - kMegaLds) * sizeof(uint32_t);
+    if (ctx->traceSpillBytes < spillBytes) {
+        bool capturing = false;
+        rc = streamCapturing(ctx, stream, capturing);
+        if (rc) return rc;
+        if (capturing)
+            return fail(ctx, CLR_ERR_INVALID_ARG,
+                        "the trace spill must be sized by an eager render of this batch before capture");
+    }
+    rc = ensureTraceSpill(ctx, spillBytes);
+    if (rc) return rc;
+    dev::RScene* table = nullptr;
+    dev::RScene first;
+    rc = prepareBatchScenes(ctx, scenes, numScenes, stream, &table, &first);
+    if (rc) return rc;
+    uint32_t* counter = ctx->devStatus + kWordMegaTiles;
+    k_mega_reset<<<1, 64, 0, stream>>>(counter);
+    if (tiles <= (uint64_t)cus * kMegaFewTiles)
+        k_render_mega_batch_few<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel,
+                                                         rp.maxDepth, rp.flags, states, pixels, counter,
+                                                         ctx->traceSpill);
+    else
+        k_render_mega_batch<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel,
+                                                     rp.maxDepth, rp.flags, states, pixels, counter, ctx->traceSpill);
+    return hipCheck(ctx, hipGetLastError(), "batch render launch");
 }
 
 int renderBvhVisualizer(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_params& rp, clr_float3* pixels,

@@ -922,14 +922,6 @@ void enqueueIterations(const WfArgs& A, const WfState& w, uint32_t iters, hipStr
     }
 }
 
-// Whether `stream` is being captured into a graph (by the caller).
-int streamCapturing(clr_ctx* ctx, hipStream_t stream, bool& capturing) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    const int rc = hipCheck(ctx, hipStreamIsCapturing(stream, &cs), "capture status");
-    capturing = cs != hipStreamCaptureStatusNone;
-    return rc;
-}
-
 void dropGraph(WfState& w) {
     if (w.exec) (void)hipGraphExecDestroy(w.exec);
     if (w.graph) (void)hipGraphDestroy(w.graph);
@@ -962,6 +954,14 @@ int ensureGraph(clr_ctx* ctx, WfState& w, const WfArgs& A) {
 }
 
 }  // namespace
+
+// Whether `stream` is being captured into a graph (by the caller).
+int streamCapturing(clr_ctx* ctx, hipStream_t stream, bool& capturing) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const int rc = hipCheck(ctx, hipStreamIsCapturing(stream, &cs), "capture status");
+    capturing = cs != hipStreamCaptureStatusNone;
+    return rc;
+}
 
 void destroyWfState(clr_ctx* ctx) {
     if (!ctx->wf) return;
@@ -1093,8 +1093,11 @@ int renderWavefront(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_para
 // context buffer, a device table holds the resolved scenes, and path slots
 // cover all scenes' tile-major positions (scene-major), so the queues mix
 // scenes and stay full.
-int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
-                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream) {
+// A batch's per-scene traversal arrays (built on `stream`) and its device
+// scene table, blob headers resolved, in the context's batch buffer (regrown
+// only outside a capture). *firstOut: scene 0's host-side entry.
+int prepareBatchScenes(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes, hipStream_t stream,
+                       RScene** tableOut, RScene* firstOut) {
     int rc = ensureWfState(ctx);
     if (rc) return rc;
     rc = ensureStatus(ctx);
@@ -1180,11 +1183,21 @@ int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t num
         for (uint32_t k = 0; k < count; ++k) chunk.s[k] = table[s0 + k];
         k_put_scenes<<<1, kSceneChunk, 0, stream>>>(dTable + s0, chunk, count);
     }
-    rc = hipCheck(ctx, hipGetLastError(), "batch set-up launch");
+    *tableOut = dTable;
+    *firstOut = table[0];
+    return hipCheck(ctx, hipGetLastError(), "batch set-up launch");
+}
+
+int renderWavefrontBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
+                         const clr_render_params& rp, uint32_t* states, clr_float3* pixels, hipStream_t stream) {
+    RScene* dTable = nullptr;
+    RScene first;
+    int rc = prepareBatchScenes(ctx, scenes, numScenes, stream, &dTable, &first);
     if (rc) return rc;
+    WfState& w = *ctx->wf;
     WfArgs A;
     std::memset(&A, 0, sizeof(A));
-    A.S = table[0];  // status word for the kernels' flags
+    A.S = first;  // status word for the kernels' flags
     A.table = dTable;
     A.posPerScene = ((rp.width + 7) / 8) * ((rp.height + 7) / 8) * 64;
     A.states = states;

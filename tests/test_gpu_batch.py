@@ -1,7 +1,8 @@
-"""Batched rendering (clr_render_batch) of many cluster-soup scenes whose LBVHs
-come from one segmented build: every scene's image must equal, bit for bit,
-the single-scene wavefront render (and the megakernel) of that scene with
-its own clr_lbvh_build, in graph and direct-launch modes and when path-slot
+"""Batched rendering (clr_render_batch, clr_render_batch_mode) of many
+cluster-soup scenes whose LBVHs come from one segmented build: every scene's
+image must equal, bit for bit, the single-scene wavefront render (and the
+megakernel) of that scene with its own clr_lbvh_build, for both batched
+renderers, in graph and direct-launch modes and when path-slot
 chunks straddle scene boundaries; one small scene is also checked against the
 CPU oracle at the renderer's image tolerance."""
 import os
@@ -32,22 +33,22 @@ def _single(ctx, host, spp, depth, mode):
     return st, px
 
 
-def _batch(ctx, batch, spp, depth):
+def _batch(ctx, batch, spp, depth, mode=None):
     import torch
 
-    n = len(batch)
-    st = torch.empty(n * W * H * 4, dtype=torch.int32, device="cuda")
+    n, w, h = len(batch), batch.width, batch.height
+    st = torch.empty(n * w * h * 4, dtype=torch.int32, device="cuda")
     for s in range(n):
-        ca.seed_pixel_states(ctx, W, H, st[s * W * H * 4:(s + 1) * W * H * 4])
-    px = torch.zeros(n * W * H * 4, dtype=torch.float32, device="cuda")
-    ca.render_batch(ctx, batch, spp, depth, st, px)
+        ca.seed_pixel_states(ctx, w, h, st[s * w * h * 4:(s + 1) * w * h * 4])
+    px = torch.zeros(n * w * h * 4, dtype=torch.float32, device="cuda")
+    ca.render_batch(ctx, batch, spp, depth, st, px, mode=mode)
     return st, px
 
 
-def _check_equal(ctx, scenes, batch, spp, depth, label):
+def _check_equal(ctx, scenes, batch, spp, depth, label, mode=None):
     import torch
 
-    st, px = _batch(ctx, batch, spp, depth)
+    st, px = _batch(ctx, batch, spp, depth, mode)
     torch.cuda.synchronize()
     assert ctx.device_status() == 0
     for s, host in enumerate(scenes):
@@ -95,6 +96,45 @@ def test_batch_with_large_scenes_equals_single_scene_renders(gpu_ctx):
     print(f"[batch] scenes of {[h.num_faces for h in big]} faces {W}x{H} spp=2 depth=5", flush=True)
     batch = ca.SceneBatch(gpu_ctx, big)
     _check_equal(gpu_ctx, big, batch, 2, 5, "large")
+    _check_equal(gpu_ctx, big, batch, 2, 5, "large, megakernel", mode=ca.RENDER_MEGAKERNEL)
+
+
+def test_megakernel_batch_equals_single_scene_renders(gpu_ctx, scenes):
+    """clr_render_batch_mode(MEGAKERNEL) on a batch of at most four tiles
+    per CU (the instantiation with the whole stack in LDS)."""
+    print(f"[batch] {len(scenes)} scenes {W}x{H} spp=3 depth=5, batched megakernel", flush=True)
+    batch = ca.SceneBatch(gpu_ctx, scenes)
+    _check_equal(gpu_ctx, scenes, batch, 3, 5, "megakernel", mode=ca.RENDER_MEGAKERNEL)
+
+
+def test_megakernel_batch_of_many_tiles_equals_wavefront_batch(gpu_ctx):
+    """A batch of more than four tiles per CU (the persistent instantiation
+    with the split stack, 24 scenes x 256 tiles): both batched renderers and
+    single-scene megakernel renders give the same images and states."""
+    import torch
+
+    sizes = [1000 + 1700 * i for i in range(24)]
+    hosts = [ca.HostScene(ca.SCENE_CLUSTERS, 128, 128, seed=500 + i, num_triangles=n)
+             for i, n in enumerate(sizes)]
+    print(f"[batch] {len(hosts)} scenes 128x128 spp=2 depth=5, batched megakernel vs wavefront", flush=True)
+    batch = ca.SceneBatch(gpu_ctx, hosts)
+    mst, mpx = _batch(gpu_ctx, batch, 2, 5, ca.RENDER_MEGAKERNEL)
+    wst, wpx = _batch(gpu_ctx, batch, 2, 5, ca.RENDER_WAVEFRONT)
+    torch.cuda.synchronize()
+    assert gpu_ctx.device_status() == 0
+    print(f"[batch] max |megakernel - wavefront| = {float((mpx - wpx).abs().max()):.3g}, "
+          f"mean {float(mpx.mean()):.4g}", flush=True)
+    assert float(mpx.mean()) > 0.0
+    assert torch.equal(mpx, wpx)
+    assert torch.equal(mst, wst)
+    npix = 128 * 128
+    for s in (0, 23):
+        ds = ca.DeviceScene(gpu_ctx, hosts[s])
+        st, px = ca.new_image(128, 128)
+        ca.seed_pixel_states(gpu_ctx, 128, 128, st)
+        ca.render(gpu_ctx, ds, 2, 5, st, px, mode=ca.RENDER_MEGAKERNEL)
+        assert torch.equal(mpx[s * npix * 4:(s + 1) * npix * 4], px)
+        assert torch.equal(mst[s * npix * 4:(s + 1) * npix * 4], st)
 
 
 def test_batch_chunks_straddle_scenes_and_direct_launches(gpu_ctx, scenes):
@@ -146,6 +186,11 @@ def test_batch_rejects_reference_traversal_and_mixed_sizes(gpu_ctx, scenes):
     px = torch.zeros(2 * W * H * 4, dtype=torch.float32, device="cuda")
     with pytest.raises(ca.ClrError):
         ca.render_batch(gpu_ctx, batch, 1, 5, st, px, flags=ca.RENDER_REFERENCE_TRAVERSAL)
+    with pytest.raises(ca.ClrError):
+        ca.render_batch(gpu_ctx, batch, 1, 5, st, px, flags=ca.RENDER_REFERENCE_TRAVERSAL,
+                        mode=ca.RENDER_MEGAKERNEL)
+    with pytest.raises(ca.ClrError):
+        ca.render_batch(gpu_ctx, batch, 1, 5, st, px, mode=7)
     other = ca.HostScene(ca.SCENE_CLUSTERS, 32, 32, seed=7, num_triangles=1000)
     with pytest.raises(ValueError):
         ca.SceneBatch(gpu_ctx, [scenes[0], other])

@@ -119,13 +119,15 @@ def test_captured_segmented_build_replays(gpu_ctx):
                                               err_msg=f"replay {r} scene {s}")
 
 
-def test_captured_batch_pipeline_replays(gpu_ctx):
+@pytest.mark.parametrize("mode", [None, ca.RENDER_MEGAKERNEL], ids=["wavefront", "megakernel"])
+def test_captured_batch_pipeline_replays(gpu_ctx, mode):
     """One graph holds a batch's whole frame: the segmented build of every
     scene's LBVH (clr_lbvh_build_segmented), the seeding of the pixel states,
-    the clearing of the image and the batched wavefront render
+    the clearing of the image and the batched render: the wavefront form
     (clr_render_batch), which under capture runs its iteration bound as plain
-    launches (no early-stop readback, no nested graph) and uploads its scene
-    table through kernel arguments. The scenes' vertices (and the triangle
+    launches (no early-stop readback, no nested graph), or the megakernel
+    form (clr_render_batch_mode); both upload their scene table through
+    kernel arguments. The scenes' vertices (and the triangle
     soup the build reads) move between replays; each replay's nodes must equal
     the oracle's per-scene builds of the moved geometry, and its states and
     image the eager pipeline's on the same geometry, bit for bit."""
@@ -144,9 +146,9 @@ def test_captured_batch_pipeline_replays(gpu_ctx):
         for s in range(k):
             ca.seed_pixel_states(gpu_ctx, W, H, st[s * W * H * 4:(s + 1) * W * H * 4])
         px.zero_()
-        ca.render_batch(gpu_ctx, batch, spp, depth, st, px)
+        ca.render_batch(gpu_ctx, batch, spp, depth, st, px, mode=mode)
 
-    frame()  # eager: sizes the batch buffers and the wavefront pool
+    frame()  # eager: sizes the batch buffers, the wavefront pool and the trace spill
     torch.cuda.synchronize()
     assert gpu_ctx.device_status() == 0
     g = capture(frame)
@@ -160,7 +162,8 @@ def test_captured_batch_pipeline_replays(gpu_ctx):
             batch.tris.view(-1, 3)[:, 0] += float(d)
         px.fill_(-1.0)
         torch.cuda.synchronize()
-        print(f"[graph capture] batch pipeline replay {r}", flush=True)
+        print(f"[graph capture] batch pipeline ({'wavefront' if mode is None else 'megakernel'}) replay {r}",
+              flush=True)
         g.replay()
         torch.cuda.synchronize()
         assert gpu_ctx.device_status() == 0
