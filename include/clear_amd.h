@@ -364,10 +364,12 @@ int clr_render_batch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScen
  * clr_render_batch; CLR_RENDER_MEGAKERNEL renders the batch with persistent
  * megakernel waves (ordered traversal) that take 8x8 tiles of all the scenes
  * from one counter. Both give every scene the image clr_render(...,
- * MEGAKERNEL or WAVEFRONT) gives it alone, bit for bit. Same arguments, limits
- * and capture rule as clr_render_batch (an eager call with the same scene
- * count, sizes and params sizes the buffers first). Like clr_render_batch,
- * the multi-scene form of main.cpp:475-494. */
+ * MEGAKERNEL or WAVEFRONT) gives it alone, bit for bit. Same arguments,
+ * limits and capture rule as clr_render_batch (an eager call with the same
+ * scene count, sizes and params sizes the buffers first), except that the
+ * megakernel form takes any maxDepth. Unknown modes return
+ * CLR_ERR_INVALID_ARG. Like clr_render_batch, the multi-scene form of
+ * main.cpp:475-494. */
 int clr_render_batch_mode(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t numScenes,
                           const clr_render_params* params, uint32_t* randStates, clr_float3* pixels, int mode,
                           void* stream);
