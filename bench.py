@@ -42,6 +42,7 @@ Secondary legs of the same metric family, each on its stated workload:
                     [--no-render] [--no-adjoint] [--no-batch] [--no-dropin]
                     [--wavefront-config key=value,...]
                     [--batch-wavefront-config key=value,...]
+                    [--batch-per-scene N] [--no-batch-build-compare]
 """
 from __future__ import annotations
 
@@ -403,11 +404,12 @@ def batch_leg(ca, np, torch, ctx, args):
             o, n = int(batch.offsets[s]), int(batch.offsets[s + 1] - batch.offsets[s])
             b.perform_soup(tp + 36 * o, n, 10, ip + ca.INODE_BYTES * (o - s), lp + ca.LEAF_BYTES * o)
 
-    per_ms = timed(per_scene_builds, 3)
     out["segmented_build"] = {"ms": round(seg_ms, 3), "Mtri_s": round(total / seg_ms / 1e3, 1)}
-    out["per_scene_builds"] = {"ms": round(per_ms, 3), "Mtri_s": round(total / per_ms / 1e3, 1)}
-    out["segmented_speedup"] = round(per_ms / seg_ms, 2)
-    log(f"batch builds: segmented {out['segmented_build']}, one by one {out['per_scene_builds']}")
+    if args.batch_build_compare:
+        per_ms = timed(per_scene_builds, 3)
+        out["per_scene_builds"] = {"ms": round(per_ms, 3), "Mtri_s": round(total / per_ms / 1e3, 1)}
+        out["segmented_speedup"] = round(per_ms / seg_ms, 2)
+    log(f"batch builds: segmented {out['segmented_build']}, one by one {out.get('per_scene_builds')}")
     assert ctx.device_status() == 0
 
     npix = W * H
@@ -606,6 +608,8 @@ def main() -> int:
                     help="render legs use the default wavefront configuration (for kernel profiles)")
     ap.add_argument("--batch-per-scene", type=int, default=32,
                     help="scenes rendered one by one for the batch comparison (0: skip)")
+    ap.add_argument("--no-batch-build-compare", dest="batch_build_compare", action="store_false",
+                    help="skip timing the 256 one-by-one builds of the batch leg (counter runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (calibrates "
                          "HBM byte counters)")
