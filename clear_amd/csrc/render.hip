@@ -150,6 +150,164 @@ __device__ V3 tracePath(const RScene& S, uint32_t gx, uint32_t gy, uint32_t maxD
     return pix;
 }
 
+// All spp camera paths of pixel (gx, gy), summed: the same terms, in the same
+// order and from the same random numbers as spp calls of tracePath, so the
+// result is bit-identical. The two loops of those calls (samples, and bounces
+// within a path) are merged into one: a lane whose path ends starts its next
+// sample at once, and every pass of the loop traces exactly one ray per active
+// lane (camera, shadow or extension) at one call site. A wave then runs as
+// long as its lanes' longest sum of paths rather than the sum of its longest
+// paths, and its traversals keep their lanes busy.
+template <int WALK, class Stack>
+__device__ V3 tracePixel(const RScene& S, uint32_t gx, uint32_t gy, uint32_t spp, uint32_t maxDepth,
+                         uint32_t flags, Rng& rng, const Stack& stack) {
+    const bool noRR = (flags & CLR_RENDER_NO_RR) != 0;
+    V3 acc = v3z();
+    if (spp == 0) return acc;
+    uint32_t k = 0;
+    // path state
+    V3 pix, alpha, org, dir, vout = v3z();
+    float initY, dirPDF;
+    uint32_t sampledType, depth;
+    bool first, shadow;
+    SurfPt sp;
+    BSDF B;
+    EDF E;
+    clr_face F;
+    // pending light sample: while `shadow`, org and dir hold the shadow ray
+    // (dir = vinL); the extension ray's are not needed again after it
+    SurfPt lp;
+    bool atInf = false;
+    float lightPDF = 0.0f, dist2 = 0.0f;
+
+    auto startSample = [&]() {
+        float w;
+        cameraRay(S, gx, gy, rng, org, dir, w, dirPDF);
+        alpha = v3s(1.0f) * w;
+        initY = lum(alpha);
+        pix = v3z();
+        sampledType = 0;
+        depth = 0;
+        first = true;
+        shadow = false;
+    };
+    // The path ends; envTerm: it left the scene (tracePath's traceContinue).
+    auto finishSample = [&](bool envTerm) {
+        if (envTerm) {
+            vout = -dir;
+            const float theta = acosf(clampf(dir.y, -1.0f, 1.0f));
+            const float phi = fmodf(atan2f(-dir.x, dir.z) + 2.0f * kPiF, 2 * kPiF);
+            const float u = phi / (2.0f * kPiF), v = theta / kPiF;
+            edfEnv(S, u, v, E);
+            float mis = 1.0f;
+            if ((sampledType & kNonSingular) != 0 && depth > 0) {
+                const float lightPDF = areaPdfEnv(S, u, v) * 1.0f / absCosEDF(E, vout);
+                mis = emissionHitMis(flags, dirPDF, lightPDF);
+            }
+            pix += (mis * alpha) * evalLe(E, vout);
+        }
+        acc += pix;
+        if (++k < spp) startSample();
+    };
+    // BSDF sample at sp: the next extension ray, or the end of the path.
+    auto sampleBounce = [&]() {
+        const float bc = rng.uniform(), b0 = rng.uniform(), b1 = rng.uniform();
+        V3 nd = v3z();
+        const V3 f = sampleFs(B, vout, bc, b0, b1, nd, dirPDF, sampledType);
+        if ((f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) || dirPDF == 0.0f) {
+            finishSample(false);
+            return;
+        }
+        alpha = alpha * (f * (fabsf(dot(B.N, nd)) / dirPDF));
+        dir = nd;
+        org = sp.p + dir * kRayEps;
+        ++depth;
+        shadow = false;
+    };
+
+    startSample();
+    while (k < spp) {
+        SurfPt hsp;
+        const bool hit = intersect<WALK>(S, org, dir, stack, hsp);
+        if (shadow) {
+            const V3 vinL = dir;
+            const bool visible = atInf ? !hit : (hit && hsp.face == lp.face);
+            // A zero or infinite light density (an environment sample at a
+            // pole, where sin(theta) = 0) contributes nothing.
+            if (visible && lightPDF > 0.0f && lightPDF < INFINITY) {
+                const V3 vinLrev = -vinL;
+                const float fsPDF = evalFsPdf(B, vout, vinL) * absCosEDF(E, vinL) / dist2;
+                const float mis = lightSampleMis(flags, lightPDF, fsPDF);
+                const float wgt = mis * absCosEDF(E, vinL) * fabsf(dot(B.N, vinL)) / dist2 / lightPDF;
+                const V3 term = wgt * ((alpha * evalLe(E, vinLrev)) * evalFs(B, vout, vinL));
+                pix += term;
+            }
+            sampleBounce();
+            continue;
+        }
+        if (!hit) {
+            finishSample(true);
+            continue;
+        }
+        sp = hsp;
+        F = S.g.faces[sp.face];
+        vout = -dir;
+        if (F.lightPtr != 0xFFFFu) {
+            edfArea(S, F.lightPtr, sp, sp.u, sp.v, E);
+            if (first) {
+                pix += alpha * evalLe(E, vout);
+            } else {
+                float mis = 1.0f;
+                if (sampledType & kNonSingular) {
+                    const V3 dd = sp.p - org;
+                    const float d2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
+                    const float lightPDF = areaPdfOfFace(S, sp.face) * d2 / absCosEDF(E, vout);
+                    mis = emissionHitMis(flags, dirPDF, lightPDF);
+                }
+                pix += (mis * alpha) * evalLe(E, vout);
+            }
+        }
+        if (!first) {
+            if (depth >= maxDepth) {
+                finishSample(false);
+                continue;
+            }
+            if (!noRR) {
+                const float cp = fminf(lum(alpha) * (1.0f / initY), 1.0f);
+                if (rng.uniform() < cp) {
+                    alpha = alpha * (1.0f / cp);
+                    dirPDF *= cp;
+                } else {
+                    finishSample(false);
+                    continue;
+                }
+            }
+        }
+        first = false;
+        bsdfAlloc(S, F.matPtr, sp, B);
+        if (hasNonSpecular(B)) {
+            const float lu = rng.uniform(), l0 = rng.uniform(), l1 = rng.uniform();
+            // With BSDF samples alone the light sample is skipped (its numbers are drawn).
+            if (!(flags & CLR_RENDER_BSDF_SAMPLING_ONLY)) {
+                sampleLight(S, lu, l0, l1, lp, atInf, E, lightPDF);
+                V3 vinL = lp.p - sp.p;
+                dist2 = dot(vinL, vinL);
+                vinL = vinL * (1.0f / sqrtf(dist2));
+                if (atInf) {
+                    vinL = lp.p;
+                    dist2 = 1.0f;
+                }
+                org = sp.p + vinL * kRayEps;
+                dir = vinL;
+                shadow = true;
+                continue;
+            }
+        }
+        sampleBounce();
+    }
+    return acc;
+}
+
 // ---- path-replay adjoint ----------------------------------------------------
 // Parameters: per differentiable material (a single Oren-Nayar / Lambert
 // element, reflection.cl:239-255) its albedo rho (RGB) and roughness sigma,
@@ -291,7 +449,7 @@ __global__ void k_mega_reset(uint32_t* counter) {
 
 // One 8x8 tile (tile index `tile` of an image tilesX tiles wide), all spp
 // samples of its pixels; states and pixels are the image's own arrays.
-template <int WALK, int LDS>
+template <int WALK, int LDS, bool MERGED>
 __device__ __forceinline__ void renderTile(const dev::RScene& S, uint32_t tile, uint32_t tilesX, uint32_t width,
                                            uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
                                            uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
@@ -306,12 +464,19 @@ __device__ __forceinline__ void renderTile(const dev::RScene& S, uint32_t tile, 
     dev::Rng rng{st.x, st.y, st.z, st.w};
     dev::V3 acc = dev::v3z();
     dev::NoHook hook;
+    // MERGED: the sample and bounce loops merged (tracePixel).
     if constexpr (LDS < dev::kStackDepth) {
         const dev::SpillStack<LDS> stk{s_stack + threadIdx.x, 64, spill + lane, lanes};
-        for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
+        if constexpr (MERGED)
+            acc = dev::tracePixel<WALK>(S, gx, gy, spp, maxDepth, flags, rng, stk);
+        else
+            for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
     } else {
         const dev::LdsStack stk{s_stack + threadIdx.x, 64};
-        for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
+        if constexpr (MERGED)
+            acc = dev::tracePixel<WALK>(S, gx, gy, spp, maxDepth, flags, rng, stk);
+        else
+            for (uint32_t k = 0; k < spp; ++k) acc += dev::tracePath<WALK>(S, gx, gy, maxDepth, flags, rng, stk, hook);
     }
     reinterpret_cast<uint4*>(states)[pi] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
     float4 p = reinterpret_cast<float4*>(pixels)[pi];
@@ -333,7 +498,7 @@ __device__ __forceinline__ uint32_t nextTile(uint32_t* tileCounter) {
 // the spill region is sized by the grid, not by the image.
 // LDS: stack words per lane kept in LDS (the ordered walk spills the rest;
 // the reference walks need all kStackDepth).
-template <int WALK, int LDS>
+template <int WALK, int LDS, bool MERGED>
 __device__ __forceinline__ void renderTiles(const dev::RScene& scene, uint32_t width, uint32_t height, uint32_t spp,
                                             uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states,
                                             clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
@@ -343,7 +508,8 @@ __device__ __forceinline__ void renderTiles(const dev::RScene& scene, uint32_t w
     const uint32_t tilesX = (width + kTile - 1) / kTile;
     const uint32_t tiles = tilesX * ((height + kTile - 1) / kTile);
     for (uint32_t tile = nextTile(tileCounter); tile < tiles; tile = nextTile(tileCounter))
-        renderTile<WALK, LDS>(S, tile, tilesX, width, height, spp, maxDepth, flags, states, pixels, spill, s_stack);
+        renderTile<WALK, LDS, MERGED>(S, tile, tilesX, width, height, spp, maxDepth, flags, states, pixels, spill,
+                                      s_stack);
 }
 
 // A batch of equally sized images: the tiles of all scenes share one
@@ -351,7 +517,7 @@ __device__ __forceinline__ void renderTiles(const dev::RScene& scene, uint32_t w
 // whose states and pixels start at scene * width * height. Consecutive tiles
 // stay in one scene, so the waves resident at a time walk a few scenes'
 // arrays. The table's entries have their blob headers resolved.
-template <int WALK, int LDS>
+template <int WALK, int LDS, bool MERGED>
 __device__ __forceinline__ void renderBatchTiles(const dev::RScene* __restrict__ table, uint32_t numScenes,
                                                  uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth,
                                                  uint32_t flags, uint32_t* __restrict__ states,
@@ -364,53 +530,61 @@ __device__ __forceinline__ void renderBatchTiles(const dev::RScene* __restrict__
     for (uint32_t tile = nextTile(tileCounter); tile < tiles; tile = nextTile(tileCounter)) {
         const uint32_t s = tile / tilesPerScene;
         const dev::RScene S = table[s];
-        renderTile<WALK, LDS>(S, tile - s * tilesPerScene, tilesX, width, height, spp, maxDepth, flags,
-                              states + 4 * npix * s, pixels + npix * s, spill, s_stack);
+        renderTile<WALK, LDS, MERGED>(S, tile - s * tilesPerScene, tilesX, width, height, spp, maxDepth, flags,
+                                      states + 4 * npix * s, pixels + npix * s, spill, s_stack);
     }
 }
 
-// Large images: five waves per SIMD.
-template <int WALK>
+// Large images: five waves per SIMD. MERGED: the ordered walk's merged
+// sample and bounce loop (tracePixel), used when Russian roulette is on.
+// Without it paths run to the bounce cap unless they leave the scene, little
+// imbalance is left to recover, and the per-sample loop needs fewer registers
+// (the 512x512 differentiation scene without roulette: 56 ms per-sample
+// against 61 ms merged).
+template <int WALK, bool MERGED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega(
     dev::RScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
     uint32_t* __restrict__ states, clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
     uint32_t* __restrict__ spill) {
     constexpr int kLds = WALK == dev::kWalkOrdered ? kMegaLds : dev::kStackDepth;
     __shared__ uint32_t s_stack[kLds * 64];
-    renderTiles<WALK, kLds>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill, s_stack);
+    renderTiles<WALK, kLds, MERGED>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill,
+                                    s_stack);
 }
 
 // Images of at most kMegaFewTiles tiles per CU cannot fill many waves per
 // CU, so their waves keep the whole stack in LDS and all the registers they
 // want (no spills).
 constexpr uint32_t kMegaFewTiles = 4;
-template <int WALK>
+template <int WALK, bool MERGED>
 __global__ __launch_bounds__(64) void k_render_mega_few(dev::RScene scene, uint32_t width, uint32_t height,
                                                         uint32_t spp, uint32_t maxDepth, uint32_t flags,
                                                         uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
                                                         uint32_t* __restrict__ tileCounter,
                                                         uint32_t* __restrict__ spill) {
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
-    renderTiles<WALK, dev::kStackDepth>(scene, width, height, spp, maxDepth, flags, states, pixels, tileCounter, spill,
-                                        s_stack);
+    renderTiles<WALK, dev::kStackDepth, MERGED>(scene, width, height, spp, maxDepth, flags, states, pixels,
+                                                tileCounter, spill, s_stack);
 }
 
 // Batches: the same two shapes.
+template <bool MERGED>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega_batch(
     const dev::RScene* __restrict__ table, uint32_t numScenes, uint32_t width, uint32_t height, uint32_t spp,
     uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
     uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
     __shared__ uint32_t s_stack[kMegaLds * 64];
-    renderBatchTiles<dev::kWalkOrdered, kMegaLds>(table, numScenes, width, height, spp, maxDepth, flags, states,
-                                                  pixels, tileCounter, spill, s_stack);
+    renderBatchTiles<dev::kWalkOrdered, kMegaLds, MERGED>(table, numScenes, width, height, spp, maxDepth, flags,
+                                                          states, pixels, tileCounter, spill, s_stack);
 }
+template <bool MERGED>
 __global__ __launch_bounds__(64) void k_render_mega_batch_few(
     const dev::RScene* __restrict__ table, uint32_t numScenes, uint32_t width, uint32_t height, uint32_t spp,
     uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
     uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
     __shared__ uint32_t s_stack[dev::kStackDepth * 64];
-    renderBatchTiles<dev::kWalkOrdered, dev::kStackDepth>(table, numScenes, width, height, spp, maxDepth, flags,
-                                                          states, pixels, tileCounter, spill, s_stack);
+    renderBatchTiles<dev::kWalkOrdered, dev::kStackDepth, MERGED>(table, numScenes, width, height, spp, maxDepth,
+                                                                  flags, states, pixels, tileCounter, spill, s_stack);
 }
 
 // bvh_visualizer (bvh_visualizer.cl:25-84): one ray through each pixel
@@ -639,23 +813,21 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     if (rc) return rc;
     uint32_t* counter = ctx->devStatus + kWordMegaTiles;
     k_mega_reset<<<1, 64, 0, stream>>>(counter);
+    auto launch = [&](auto kernel) {
+        kernel<<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, pixels,
+                                        counter, ctx->traceSpill);
+    };
+    const bool merged = !(flags & CLR_RENDER_NO_RR);
+    const bool few = tiles <= (uint32_t)cus * kMegaFewTiles;
     if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)) {
-        if (tiles <= (uint32_t)cus * kMegaFewTiles)
-            k_render_mega_few<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                          rp.maxDepth, flags, states, pixels, counter,
-                                                                          ctx->traceSpill);
+        if (few)
+            launch(merged ? k_render_mega_few<dev::kWalkOrdered, true> : k_render_mega_few<dev::kWalkOrdered, false>);
         else
-            k_render_mega<dev::kWalkOrdered><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                      rp.maxDepth, flags, states, pixels, counter,
-                                                                      ctx->traceSpill);
+            launch(merged ? k_render_mega<dev::kWalkOrdered, true> : k_render_mega<dev::kWalkOrdered, false>);
     } else if (sd.bvhNodes)
-        k_render_mega<dev::kWalkMidpoint><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                                   rp.maxDepth, flags, states, pixels, counter,
-                                                                   ctx->traceSpill);
+        launch(k_render_mega<dev::kWalkMidpoint, false>);
     else
-        k_render_mega<dev::kWalkLbvh><<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel,
-                                                               rp.maxDepth, flags, states, pixels, counter,
-                                                               ctx->traceSpill);
+        launch(k_render_mega<dev::kWalkLbvh, false>);
     return hipCheck(ctx, hipGetLastError(), "render launch");
 }
 
@@ -683,13 +855,15 @@ int renderMegakernelBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t nu
     if (rc) return rc;
     uint32_t* counter = ctx->devStatus + kWordMegaTiles;
     k_mega_reset<<<1, 64, 0, stream>>>(counter);
+    auto launch = [&](auto kernel) {
+        kernel<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
+                                        rp.flags, states, pixels, counter, ctx->traceSpill);
+    };
+    const bool merged = !(rp.flags & CLR_RENDER_NO_RR);
     if (tiles <= (uint64_t)cus * kMegaFewTiles)
-        k_render_mega_batch_few<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel,
-                                                         rp.maxDepth, rp.flags, states, pixels, counter,
-                                                         ctx->traceSpill);
+        launch(merged ? k_render_mega_batch_few<true> : k_render_mega_batch_few<false>);
     else
-        k_render_mega_batch<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel,
-                                                     rp.maxDepth, rp.flags, states, pixels, counter, ctx->traceSpill);
+        launch(merged ? k_render_mega_batch<true> : k_render_mega_batch<false>);
     return hipCheck(ctx, hipGetLastError(), "batch render launch");
 }
 

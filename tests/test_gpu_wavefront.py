@@ -65,6 +65,15 @@ def test_terrain_reduced(gpu_ctx):
     same_as_megakernel(gpu_ctx, host, 4, 8, cfg=default_cfg(gpu_ctx))
 
 
+@pytest.mark.parametrize("flags", [0, ca.RENDER_NO_RR], ids=["roulette", "no_roulette"])
+def test_terrain_many_tiles(gpu_ctx, flags):
+    """More than four 8x8 tiles per CU (48 x 36 tiles), so the megakernel runs
+    its persistent instantiation with the split stack: with Russian roulette
+    the merged sample/bounce loop, without it the per-sample loop."""
+    host = ca.HostScene(ca.SCENE_TERRAIN, 384, 288, detail=256)
+    same_as_megakernel(gpu_ctx, host, 2, 8, cfg=default_cfg(gpu_ctx), flags=flags)
+
+
 def test_reference_traversal_showcase(gpu_ctx):
     host = ca.HostScene(ca.SCENE_SHOWCASE, 96, 96)
     same_as_megakernel(gpu_ctx, host, 4, 8, cfg=default_cfg(gpu_ctx), flags=ca.RENDER_REFERENCE_TRAVERSAL)
