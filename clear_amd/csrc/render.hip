@@ -435,6 +435,10 @@ constexpr int kTile = 8;  // 8x8 pixels per wave
 // LDS stack (10 waves per CU).
 constexpr int kMegaLds = 32;
 constexpr uint32_t kMegaWavesPerCU = 20;  // persistent grid
+// The merged sample/bounce loop keeps more state alive across its one trace
+// call: at four waves per SIMD (128 VGPRs) it beat five (96 VGPRs, more
+// spills): 1080p 341-345 -> 330-334 ms, the 256-scene batch 289-294 -> 284-286.
+constexpr uint32_t kMegaMergedWavesPerCU = 16;
 // The adjoint's waves also hold 4 KiB of LDS gradient sums, so its stack
 // keeps 16 words in LDS (8 KiB per wave with the sums) and its register
 // budget is set for four waves per SIMD: 512x512, 32 spp, depth 5 of the
@@ -535,14 +539,14 @@ __device__ __forceinline__ void renderBatchTiles(const dev::RScene* __restrict__
     }
 }
 
-// Large images: five waves per SIMD. MERGED: the ordered walk's merged
+// Large images: five waves per SIMD (four with MERGED). MERGED: the ordered walk's merged
 // sample and bounce loop (tracePixel), used when Russian roulette is on.
 // Without it paths run to the bounce cap unless they leave the scene, little
 // imbalance is left to recover, and the per-sample loop needs fewer registers
 // (the 512x512 differentiation scene without roulette: 56 ms per-sample
 // against 61 ms merged).
 template <int WALK, bool MERGED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MERGED ? 4 : 5))) void k_render_mega(
     dev::RScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t maxDepth, uint32_t flags,
     uint32_t* __restrict__ states, clr_float3* __restrict__ pixels, uint32_t* __restrict__ tileCounter,
     uint32_t* __restrict__ spill) {
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(64) void k_render_mega_few(dev::RScene scene, uint3
 
 // Batches: the same two shapes.
 template <bool MERGED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_render_mega_batch(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MERGED ? 4 : 5))) void k_render_mega_batch(
     const dev::RScene* __restrict__ table, uint32_t numScenes, uint32_t width, uint32_t height, uint32_t spp,
     uint32_t maxDepth, uint32_t flags, uint32_t* __restrict__ states, clr_float3* __restrict__ pixels,
     uint32_t* __restrict__ tileCounter, uint32_t* __restrict__ spill) {
@@ -808,7 +812,9 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
     int cus = 0;
     rc = hipCheck(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
     if (rc) return rc;
-    const uint32_t grid = std::max(1u, std::min(tiles, (uint32_t)cus * kMegaWavesPerCU));
+    const bool merged = !(flags & CLR_RENDER_NO_RR) && !(flags & CLR_RENDER_REFERENCE_TRAVERSAL);
+    const uint32_t grid =
+        std::max(1u, std::min(tiles, (uint32_t)cus * (merged ? kMegaMergedWavesPerCU : kMegaWavesPerCU)));
     rc = ensureTraceSpill(ctx, (size_t)grid * 64 * (dev::kStackDepth - kMegaLds) * sizeof(uint32_t));
     if (rc) return rc;
     uint32_t* counter = ctx->devStatus + kWordMegaTiles;
@@ -817,7 +823,6 @@ int renderMegakernel(clr_ctx* ctx, const clr_scene_dev& sd, const clr_render_par
         kernel<<<grid, 64, 0, stream>>>(S, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth, flags, states, pixels,
                                         counter, ctx->traceSpill);
     };
-    const bool merged = !(flags & CLR_RENDER_NO_RR);
     const bool few = tiles <= (uint32_t)cus * kMegaFewTiles;
     if (!(rp.flags & CLR_RENDER_REFERENCE_TRAVERSAL)) {
         if (few)
@@ -837,7 +842,9 @@ int renderMegakernelBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t nu
     int rc = hipCheck(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
     if (rc) return rc;
     const uint64_t tiles = (uint64_t)((rp.width + kTile - 1) / kTile) * ((rp.height + kTile - 1) / kTile) * numScenes;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)cus * kMegaWavesPerCU));
+    const bool merged = !(rp.flags & CLR_RENDER_NO_RR);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(tiles, (uint64_t)cus * (merged ? kMegaMergedWavesPerCU : kMegaWavesPerCU)));
     const size_t spillBytes = (size_t)grid * 64 * (dev::kStackDepth - kMegaLds) * sizeof(uint32_t);
     if (ctx->traceSpillBytes < spillBytes) {
         bool capturing = false;
@@ -859,7 +866,6 @@ int renderMegakernelBatch(clr_ctx* ctx, const clr_scene_dev* scenes, uint32_t nu
         kernel<<<grid, 64, 0, stream>>>(table, numScenes, rp.width, rp.height, rp.samplesPerPixel, rp.maxDepth,
                                         rp.flags, states, pixels, counter, ctx->traceSpill);
     };
-    const bool merged = !(rp.flags & CLR_RENDER_NO_RR);
     if (tiles <= (uint64_t)cus * kMegaFewTiles)
         launch(merged ? k_render_mega_batch_few<true> : k_render_mega_batch_few<false>);
     else
