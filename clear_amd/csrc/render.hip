@@ -438,6 +438,7 @@ constexpr uint32_t kMegaWavesPerCU = 20;  // persistent grid
 // The merged sample/bounce loop keeps more state alive across its one trace
 // call: at four waves per SIMD (128 VGPRs) it beat five (96 VGPRs, more
 // spills): 1080p 341-345 -> 330-334 ms, the 256-scene batch 289-294 -> 284-286.
+// Three (168 VGPRs) took 378-381 and 325-326 ms.
 constexpr uint32_t kMegaMergedWavesPerCU = 16;
 // The adjoint's waves also hold 4 KiB of LDS gradient sums, so its stack
 // keeps 16 words in LDS (8 KiB per wave with the sums) and its register
